@@ -1,0 +1,375 @@
+"""CPU oracle for the sim-trhepd-rheed surface slice loop -- TEST INFRASTRUCTURE ONLY.
+
+This module drives ``liboracle.so`` (trhepd_oracle.c) and restates the deck setup of
+``surfio.f90:60-341`` and the angle loop of ``surflf.f90:7-171`` (paths relative to
+sim-trhepd-rheed/src in the reference).  It is the checker the HIP path is compared
+against: only tests/, ``__graft_entry__.smoke()`` and bench.py's cpu_baseline leg may
+import it; the product library never loads it.
+
+Pinning: ``surface_curve`` reproduces the reference's FP64 rocking curves
+test/test{23,47,521}/test-results/perf-{4,5,6,7}-<dz>.txt (tests/test_oracle_golden.py).
+The structure-factor orbits of the 16 plane groups the decks do not use are pinned by
+closure/multiplicity checks (tests/test_oracle_math.py).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import math
+import os
+import subprocess
+import sys
+from dataclasses import dataclass
+from typing import Dict, List, Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+sys.path.insert(0, os.path.join(REPO, "trhepd-opt_amd"))
+from trhepd import deckio  # noqa: E402  (format readers; data, not compute)
+
+MAB = 6
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return os.path.join(HERE, "liboracle.so")
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+        ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+        zp = np.ctypeslib.ndpointer(dtype=np.complex128, flags="C_CONTIGUOUS")
+        d, i = C.c_double, C.c_int
+        L.orc_asfcrr.argtypes = [d, C.POINTER(d), i, i, i, dp, dp, dp, dp, dp, dp, dp, dp, d, d]
+        L.orc_asfcef.argtypes = [i, i, ip, ip, d, d, d, i, i, dp, dp, dp, dp, dp, dp]
+        L.orc_strfac.argtypes = [i, i, i, i, i, i, dp, dp, d, d, d, d, zp]
+        L.orc_strfac.restype = i
+        L.orc_orbit.argtypes = [i, d, d, dp, dp]
+        L.orc_orbit.restype = i
+        L.orc_srfghk.argtypes = [i, i, ip, ip, i, ip, ip, ip]
+        L.orc_srfghk.restype = i
+        L.orc_table_cols.argtypes = [i, i]
+        L.orc_table_cols.restype = i
+        L.orc_table_depth.argtypes = [i, d, i, i]
+        L.orc_table_depth.restype = d
+        L.orc_table.argtypes = [C.c_void_p, i, d, i, zp, zp]
+        L.orc_table.restype = i
+        L.orc_integrate.argtypes = [i, d, i, i, i, zp, zp, ip, zp, zp, ip]
+        L.orc_integrate.restype = i
+        L.orc_gamma.argtypes = [i, ip, ip, d, d, d, d, d, d, zp]
+        L.orc_intensity.argtypes = [i, zp, zp, i, d, dp]
+        L.orc_matAbdivB.argtypes = [i, zp, i]
+        L.orc_matAbdivB.restype = i
+        L.orc_cond_est.argtypes = [i, zp, i]
+        L.orc_cond_est.restype = d
+        L.orc_vvi2mate.argtypes = [i, zp, zp, ip, zp, zp]
+        _lib = L
+    return _lib
+
+
+class _Pot(C.Structure):
+    _fields_ = [("nv", C.c_int), ("nvb", C.c_int), ("natms", C.c_int), ("natmb", C.c_int),
+                ("nelm", C.c_int), ("nabr", C.c_int), ("nabi", C.c_int),
+                ("cc", C.c_double), ("smesh", C.c_double),
+                ("ielm", C.c_void_p), ("z", C.c_void_p), ("dchg", C.c_void_p), ("ghk", C.c_void_p),
+                ("ocr", C.c_void_p), ("b", C.c_void_p), ("asf", C.c_void_p),
+                ("st0", C.c_void_p), ("st1", C.c_void_p), ("st2", C.c_void_p)]
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# --------------------------------------------------------------------------
+# element parameters: asfparam (asf.f90:7-433) and amass (asf.f90:437-472)
+
+_ELEM = None
+
+
+def _elements():
+    global _ELEM
+    if _ELEM is None:
+        _ELEM = json.load(open(os.path.join(REPO, "trhepd-opt_amd", "trhepd", "data", "elements.json")))
+    return _ELEM
+
+
+def asfparam(iz: int, iabr: int):
+    """(nabr, a[4], b[4]): Doyle-Turner when iabr == 0 and tabulated, else Peng reversed."""
+    E = _elements()
+    if iabr == 0 and str(iz) in E["doyle_turner"]:
+        e = E["doyle_turner"][str(iz)]
+        return 4, list(e["a"]), list(e["b"])
+    if 1 <= iz <= 98:
+        e = E["peng"][str(iz)]
+        return 4, list(reversed(e["a"])), list(reversed(e["b"]))
+    return 0, [], []
+
+
+def amass(iz: int) -> float:
+    m = _elements()["mass"]
+    return m[iz - 1] if 1 <= iz <= 110 else 1000.0
+
+
+# --------------------------------------------------------------------------
+# surface setup: surfio.f90:60-341
+
+
+@dataclass
+class Surface:
+    """Everything the slice loop of one domain needs (the mod_scpot3 globals + srfref args)."""
+    nb: int
+    nv: int
+    nvb: int
+    ns: int
+    dz: float
+    method: int
+    iv: np.ndarray         # (nb, nb) int32, Fortran order (1-based entries)
+    ih: np.ndarray
+    ik: np.ndarray
+    iord: np.ndarray
+    nbg: np.ndarray
+    wn: float
+    ghx: float
+    ghy: float
+    gky: float
+    azi: float
+    daz: float
+    naz: int
+    gi: float
+    dg: float
+    ng: int
+    pot: Dict[str, object]   # arrays of the potential descriptor
+    bulk_blocks: list       # [irep1][irep2][group] (n_g, n_g)
+
+    @property
+    def nb0(self) -> int:
+        for i in range(self.nb):
+            if self.ih[i] == 0 and self.ik[i] == 0:
+                return i + 1
+        raise ValueError("no (0,0) beam")
+
+    def angles(self):
+        """[(irep1, irep2, ga, az)] in the reference's loop order (surflf.f90:55-92)."""
+        out = []
+        nrep1, nrep2 = (self.naz, self.ng) if self.ng > self.naz else (self.ng, self.naz)
+        for r1 in range(nrep1):
+            for r2 in range(nrep2):
+                if self.ng > self.naz:
+                    az, ga = self.azi + r1 * self.daz, self.gi + r2 * self.dg
+                else:
+                    ga, az = self.gi + r1 * self.dg, self.azi + r2 * self.daz
+                out.append((r1, r2, ga, az))
+        return out
+
+    def printed_angle(self, ga: float, az: float) -> float:
+        return (ga if self.ng > self.naz else az) * 180.0 / 3.141592654
+
+    def scatter_f(self, r1: int, r2: int) -> np.ndarray:
+        """f(iord(i), iord(j)) = bulk block entries (surflf.f90:105-115)."""
+        f = np.zeros((self.nb, self.nb), dtype=np.complex128, order="F")
+        ib = 0
+        for blk in self.bulk_blocks[r1][r2]:
+            n = blk.shape[0]
+            idx = self.iord[ib:ib + n] - 1
+            f[np.ix_(idx, idx)] = blk
+            ib += n
+        return f
+
+
+def setup_surface(bp: "deckio.BulkP", sd: "deckio.SurfDeck", dz2: float, method: int, idom: int = 0) -> Surface:
+    L = lib()
+    pi = math.atan(1.0) * 4.0
+    nabr, nabi = bp.nabr, bp.nabi
+    nelmb, nelms = len(bp.elements), len(sd.elements)
+    nelm = nelmb + nelms
+    a = np.zeros((nelm, MAB))     # row = element: C-order (nelm, MAB) == Fortran (MAB, nelm)
+    b = np.zeros((nelm, MAB))
+    c = np.zeros((nelm, MAB))
+    dth, dtk, dtz, elm, dchg = (np.zeros(nelm) for _ in range(5))
+    for i, e in enumerate(bp.elements):
+        na = nabr + nabi
+        a[i, :na] = e.a[:na]
+        b[i, :na] = e.b[:na]
+        dth[i], dtk[i], dtz[i], elm[i], dchg[i] = e.dth, e.dtk, e.dtz, e.elm, e.dchg
+    for j, e in enumerate(sd.elements):
+        i = nelmb + j
+        dchg[i] = 0.0
+        elm[i] = amass(e.iz)
+        nr, pa, pb = asfparam(e.iz, bp.iabr)
+        if nr <= 0:
+            raise ValueError("no scattering factor for Z=%d" % e.iz)
+        nabr = nr
+        a[i, :nr] = pa
+        b[i, :nr] = pb
+        a[i, 0] -= e.da1
+        if bp.inegpos > 0:
+            a[i, :nabr] = -a[i, :nabr]
+        nabi = 1
+        a[i, nabr] = e.sap
+        b[i, nabr] = 0.0
+        dth[i], dtk[i], dtz[i] = abs(e.bh), abs(e.bk), abs(e.bz)
+    natmb, natms = len(bp.atoms), len(sd.atoms)
+    natm = natmb + natms
+    ielm = np.zeros(natm, dtype=np.int32)
+    ocr, x, y, z = (np.zeros(natm) for _ in range(4))
+    for i, at in enumerate(bp.atoms):
+        ielm[i], ocr[i], x[i], y[i], z[i] = at.ielm, at.ocr, at.x, at.y, at.z
+    for j, at in enumerate(sd.atoms):
+        i = natmb + j
+        if -nelmb <= at.ielm <= -1:
+            ielm[i] = -at.ielm
+        elif 1 <= at.ielm <= nelms:
+            ielm[i] = at.ielm + nelmb
+        else:
+            raise ValueError("bad element index %d" % at.ielm)
+        ocr[i], x[i], y[i], z[i] = at.ocr, at.x, at.y, at.z + bp.zoffset
+    # slice count: surfio.f90:197-210
+    if sd.nsgs < 1:
+        ns, dz = 0, bp.dz
+    else:
+        zz = 0.0
+        for i in range(natmb, natm):
+            zz = max(zz, z[i])
+        ns2 = int((zz + sd.dthick + bp.cc) / bp.dz) + 1
+        ns = int(np.rint((bp.dz * ns2) / dz2))
+        ns = max(ns, 1)
+        dz = (bp.dz * ns2) / ns
+    smesh = float(abs(sd.msa * sd.nsb - sd.msb * sd.nsa))
+    wn = C.c_double(0.0)
+    L.orc_asfcrr(bp.be, C.byref(wn), nelm, nabr, nabi, a.reshape(-1), b.reshape(-1), c.reshape(-1),
+                 dchg, dth, dtk, dtz, elm, bp.tem, bp.aa * bp.bb * math.sin(bp.gam) * smesh)
+    ghx = (pi + pi) / (bp.aa * bp.nh)
+    ghy = -(pi + pi) / (bp.aa * math.tan(bp.gam) * bp.nh)
+    gky = (pi + pi) / (bp.bb * math.sin(bp.gam) * bp.nk)
+    dom = bp.domains[idom]
+    ib0 = sum(bp.nb[:idom])
+    nb = bp.nb[idom]
+    ih = np.array(bp.ih[ib0:ib0 + nb], dtype=np.int32)
+    ik = np.array(bp.ik[ib0:ib0 + nb], dtype=np.int32)
+    nvm = 1 + nb * (nb - 1) // 2
+    nvb = len(dom.igh)
+    igh = np.zeros(max(nvm, nvb), dtype=np.int32)
+    igk = np.zeros(max(nvm, nvb), dtype=np.int32)
+    igh[:nvb], igk[:nvb] = dom.igh, dom.igk
+    iv = np.zeros((nb, nb), dtype=np.int32)        # C-order buffer viewed as Fortran (nb, nb)
+    nv = L.orc_srfghk(nvm, nvb, igh, igk, nb, ih, ik, iv.reshape(-1))
+    if nv < 0:
+        raise ValueError("srfghk: nv > nvm")
+    iv = iv.reshape(-1).reshape((nb, nb), order="F")
+    igh, igk = igh[:nv].copy(), igk[:nv].copy()
+    asf = np.zeros(nv * MAB * nelm)
+    ghk = np.zeros(nv)
+    L.orc_asfcef(nv, nelm, igh, igk, ghx, ghy, gky, nabr, nabi, a.reshape(-1), c.reshape(-1), asf, dth, dtk, ghk)
+    gh = (-(pi + pi) / bp.nh) * igh.astype(np.float64)
+    gk = (-(pi + pi) / bp.nk) * igk.astype(np.float64)
+    st0 = np.zeros((natms, nv), dtype=np.complex128)      # C (natms, nv) == Fortran (nv, natms)
+    st1 = np.zeros((natmb, nvb), dtype=np.complex128)
+    st2 = np.zeros((natmb, nvb), dtype=np.complex128)
+    for j in range(natms):
+        L.orc_strfac(sd.nsgs, nv, sd.msa, sd.msb, sd.nsa, sd.nsb, gh, gk, x[natmb + j], y[natmb + j],
+                     sd.dxs + bp.dx, sd.dys + bp.dy, st0[j])
+    for j in range(natmb):
+        L.orc_strfac(bp.nsg, nvb, 1, 0, 0, 1, gh, gk, x[j], y[j], bp.dx, bp.dy, st1[j])
+        L.orc_strfac(bp.nsg, nvb, 1, 0, 0, 1, gh, gk, x[j], y[j], 0.0, 0.0, st2[j])
+    pot = dict(nv=nv, nvb=nvb, natms=natms, natmb=natmb, nelm=nelm, nabr=nabr, nabi=nabi,
+               cc=bp.cc, smesh=smesh, ielm=ielm, z=z, dchg=dchg, ghk=ghk, ocr=ocr,
+               b=b.reshape(-1).copy(), asf=asf, st0=st0.reshape(-1), st1=st1.reshape(-1), st2=st2.reshape(-1),
+               igh=igh, igk=igk, x=x, y=y, a=a.reshape(-1).copy(), c=c.reshape(-1).copy(), dth=dth, dtk=dtk)
+    return Surface(nb, nv, nvb, ns, dz, method, iv, ih, ik, dom.iord, dom.nbg, wn.value, ghx, ghy, gky,
+                   bp.azi + bp.rdom[idom], bp.daz, bp.naz, bp.gi, bp.dg, bp.ng, pot, dom.blocks)
+
+
+def _pot_struct(p: Dict[str, object]):
+    keep = []
+    def ptr(k):
+        arr = np.ascontiguousarray(p[k])
+        keep.append(arr)
+        return _ptr(arr)
+    s = _Pot(p["nv"], p["nvb"], p["natms"], p["natmb"], p["nelm"], p["nabr"], p["nabi"], p["cc"], p["smesh"],
+             ptr("ielm"), ptr("z"), ptr("dchg"), ptr("ghk"), ptr("ocr"), ptr("b"), ptr("asf"),
+             ptr("st0"), ptr("st1"), ptr("st2"))
+    return s, keep
+
+
+def table(surf: Surface):
+    """The *_int_init table v, vi (nv, K) as C-order (K, nv) arrays."""
+    L = lib()
+    K = L.orc_table_cols(surf.method, surf.ns)
+    v = np.zeros((K, surf.nv), dtype=np.complex128)
+    vi = np.zeros((K, surf.nv), dtype=np.complex128)
+    s, keep = _pot_struct(surf.pot)
+    L.orc_table(C.byref(s), surf.method, surf.dz, surf.ns, v.reshape(-1), vi.reshape(-1))
+    return v, vi
+
+
+def table_depths(method: int, dz: float, ns: int) -> np.ndarray:
+    L = lib()
+    K = L.orc_table_cols(method, ns)
+    return np.array([L.orc_table_depth(method, dz, ns, c) for c in range(K)])
+
+
+def gamma(surf: Surface, ga: float, az: float) -> np.ndarray:
+    g = np.zeros(surf.nb, dtype=np.complex128)
+    lib().orc_gamma(surf.nb, surf.ih, surf.ik, surf.wn, surf.ghx, surf.ghy, surf.gky, ga, az, g)
+    return g
+
+
+def integrate(surf: Surface, f: np.ndarray, g: np.ndarray, v, vi, method: Optional[int] = None):
+    """One angle: returns (f_out Fortran-order (nb, nb), stats [n_renorm, n_gemm])."""
+    L = lib()
+    ff = np.asfortranarray(f, dtype=np.complex128).reshape(-1, order="F").copy()
+    st = np.zeros(2, dtype=np.int32)
+    ivf = np.ascontiguousarray(surf.iv.reshape(-1, order="F"))
+    rc = L.orc_integrate(surf.method if method is None else method, surf.dz, surf.ns, surf.nb, surf.nv,
+                         ff, g, ivf, v.reshape(-1), vi.reshape(-1), st)
+    if rc != 0:
+        raise ValueError("unknown int_method")
+    return ff.reshape((surf.nb, surf.nb), order="F"), st
+
+
+def intensity(surf: Surface, f: np.ndarray, g: np.ndarray, ga: float) -> np.ndarray:
+    f2 = np.zeros(surf.nb)
+    wnsga2 = surf.wn * math.sin(ga) ** 2
+    lib().orc_intensity(surf.nb, np.ascontiguousarray(f.reshape(-1, order="F")), g, surf.nb0, wnsga2, f2)
+    return f2
+
+
+def surface_curve(surf: Surface, points: Optional[List[int]] = None, stats: Optional[list] = None):
+    """Rocking-curve rows [(printed_angle, f2)] for the chosen point indices (default all)."""
+    v, vi = table(surf)
+    rows = []
+    angs = surf.angles()
+    sel = range(len(angs)) if points is None else points
+    for p in sel:
+        r1, r2, ga, az = angs[p]
+        g = gamma(surf, ga, az)
+        f = surf.scatter_f(r1, r2)
+        fo, st = integrate(surf, f, g, v, vi)
+        if stats is not None:
+            stats.append(st.copy())
+        rows.append((surf.printed_angle(ga, az), intensity(surf, fo, g, ga)))
+    return rows
+
+
+def load_case(path_dir: str, dz2: float, method: int) -> Surface:
+    bp = deckio.BulkP.read(os.path.join(path_dir, "bulkP.b"))
+    sd = deckio.SurfDeck.parse(open(os.path.join(path_dir, "surf.txt")).read(), bp.ndom)
+    return setup_surface(bp, sd, dz2, method)
+
+
+def rel_err(f2: np.ndarray, ref: np.ndarray) -> float:
+    """max over beams of |I - I_ref| / max(|I_ref|, 1e-12 * max_beams I_ref)."""
+    floor = 1e-12 * np.max(np.abs(ref))
+    den = np.maximum(np.abs(ref), floor)
+    if floor == 0:
+        den = np.where(den == 0, 1.0, den)
+    return float(np.max(np.abs(f2 - ref) / den))
