@@ -1,0 +1,142 @@
+/*
+ * trhepd.h -- C ABI of libtrhepd_hip.so, the gfx950 (MI355X) engine for the
+ * sim-trhepd-rheed surface slice loop.
+ *
+ * Conventions (identical to what the Fortran driver passes through iso_c_binding):
+ *   - complex(8) arrays are interleaved (re, im) doubles, passed as double*;
+ *   - every matrix is column-major with the Fortran shape given in the comment;
+ *   - index maps (iv, ielm, iord) are 1-based, exactly as the driver holds them;
+ *   - every entry point returns 0 on success and a non-zero TRHEPD_E_* status on
+ *     failure; trhepd_last_error() then describes it.  Nothing aborts inside the library.
+ *   - a context is used from one host thread at a time; different contexts are independent.
+ *
+ * Citations name the reference routine each entry point replaces, relative to the
+ * reference repository root (sim-trhepd-rheed/src unless stated).
+ */
+#ifndef TRHEPD_H
+#define TRHEPD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    TRHEPD_OK = 0,
+    TRHEPD_E_ARG = 1,        /* invalid argument (shape, method, pointer) */
+    TRHEPD_E_HIP = 2,        /* a HIP runtime call failed */
+    TRHEPD_E_NOMEM = 3,      /* device allocation failed */
+    TRHEPD_E_BUDGET = 4,     /* request exceeds the context's HBM budget */
+    TRHEPD_E_SINGULAR = 5,   /* exact zero pivot in an LU factorisation */
+    TRHEPD_E_NONFINITE = 6,  /* NaN/Inf reached an output */
+    TRHEPD_E_NOTFOUND = 7    /* unknown table / request id */
+};
+
+/* integrator codes: the int_method values of surflf.f90:28-41 */
+enum {
+    TRHEPD_M2 = 1, TRHEPD_M4 = 2, TRHEPD_M6 = 3, TRHEPD_LF = 4,
+    TRHEPD_SP4 = 5, TRHEPD_SP6 = 6, TRHEPD_RK4 = 7, TRHEPD_RK4GS = 8
+};
+
+/* arithmetic of the propagator products */
+enum {
+    TRHEPD_PREC_FP64 = 0,    /* all-FP64 complex */
+    TRHEPD_PREC_MIXED = 1    /* stage products as split BF16 MFMA with FP32 accumulate,
+                                FP64 state, FP64 table / LU / boundary transforms */
+};
+
+typedef struct trhepd_ctx trhepd_ctx;
+
+typedef struct {
+    int device;              /* HIP device ordinal */
+    int precision;           /* TRHEPD_PREC_* */
+    size_t hbm_budget;       /* bytes this context may hold on the device (0: no limit) */
+} trhepd_opts;
+
+/* Potential-table inputs: the module mod_scpot3 globals (surfio.f90:18-27) that
+ * comp_scpot (surfio.f90:29-41) reads.  Host pointers. */
+typedef struct {
+    int nv, nvb, natms, natmb, nelm, nabr, nabi;
+    double cc, smesh;
+    const int *ielm;         /* (natmb+natms), bulk atoms first, 1-based element index */
+    const double *z;         /* (natmb+natms), zoffset added */
+    const double *dchg;      /* (nelm) */
+    const double *ghk;       /* (nv) */
+    const double *ocr;       /* (natmb+natms) */
+    const double *b;         /* (6, nelm) */
+    const double *asf;       /* (nv, 6, nelm) */
+    const double *st0;       /* complex (nv, natms) surface structure factors */
+    const double *st1;       /* complex (nvb, natmb) top bulk unit */
+    const double *st2;       /* complex (nvb, natmb) second bulk unit */
+} trhepd_potential;
+
+/* ---- context ------------------------------------------------------------ */
+int trhepd_ctx_create(const trhepd_opts *opts, trhepd_ctx **out);
+int trhepd_ctx_destroy(trhepd_ctx *ctx);
+const char *trhepd_last_error(const trhepd_ctx *ctx);
+/* bytes currently held on the device by the context */
+size_t trhepd_ctx_bytes(const trhepd_ctx *ctx);
+
+/* ---- per-structure setup (replaces scpot.f90:11-110, surfsub.f90:9-41) -------- */
+
+/* asfcrr (scpot.f90:11-58): energy / Debye-Waller correction of the element
+ * coefficients, in place; a, b, c are (6, nelm).  Host arithmetic (per-element scalars). */
+int trhepd_asfcrr(double be, double *wn, int nelm, int nabr, int nabi, double *a, double *b,
+                  double *c, double *dchg, double *dth, double *dtk, const double *dtz,
+                  const double *elm, double tem, double smesh);
+
+/* srfghk (surfsub.f90:9-41): beam-pair -> Fourier-component map.  igh/igk hold nvb
+ * entries on entry and room for nvm; iv is (nb, nb), strict lower triangle set.
+ * *nv receives the extended component count.  Host integer work. */
+int trhepd_srfghk(int nvm, int nvb, int *igh, int *igk, int nb, const int *ih, const int *ik,
+                  int *iv, int *nv);
+
+/* asfcef (scpot.f90:62-95) on the device: asf (nv, 6, nelm) and ghk (nv). */
+int trhepd_asfcef(trhepd_ctx *ctx, int nv, int nelm, const int *igh, const int *igk, double ghx,
+                  double ghy, double gky, int nabr, int nabi, const double *a, const double *c,
+                  const double *dth, const double *dtk, double *asf, double *ghk);
+
+/* scpot_stf (scpot.f90:100-110) with strfac (strfac.f90:5-76): plane-group orbits on
+ * the host, phase sums on the device.  st is complex (nv, natm). */
+int trhepd_scpot_stf(trhepd_ctx *ctx, int natm, int nv, int nsg, int ma, int mb, int na, int nb,
+                     const double *gh, const double *gk, const double *x, const double *y,
+                     double dx, double dy, double *st);
+
+/* ---- potential table (replaces the *_int_init of surflf.f90:28-41) ------------- */
+
+/* Builds v, vi (nv, K) on the device for int_method `method` (K = table columns of
+ * that method: hp_int_init surf_prkn.f90:174-197, m2/m4_int_init surf_magnus.f90:4-42,
+ * rk4_int_init surf_rk.f90:4-25).  The table stays resident; *table_id names it. */
+int trhepd_table_build(trhepd_ctx *ctx, const trhepd_potential *pot, double dz, int ns,
+                       int method, int *table_id);
+/* number of columns K and components nv of a table */
+int trhepd_table_shape(trhepd_ctx *ctx, int table_id, int *nv, int *ncols);
+/* copies a table back to the host: v, vi complex (nv, K) */
+int trhepd_table_fetch(trhepd_ctx *ctx, int table_id, double *v, double *vi);
+int trhepd_table_release(trhepd_ctx *ctx, int table_id);
+
+/* ---- slice loop (replaces the angle-loop body of srfref, surflf.f90:79-153) ---- */
+
+/* One batch of nang independent angles that share table `table_id`:
+ *   iv      (nb, nb)          beam-pair map from srfghk
+ *   gamma   complex (nb, nang) z wave numbers (surflf.f90:94-103)
+ *   f       complex (nb, nb, nang) in: bulk reflection scattered by iord
+ *                                   out: surface reflection matrix
+ *   f2      (nb, nang) out intensities (surflf.f90:143-151), may be NULL
+ *   nb0     1-based index of the (0,0) beam
+ *   wnsga2  (nang) K sin^2(theta) per angle
+ * The integrator is the table's int_method. */
+int trhepd_propagate_batch(trhepd_ctx *ctx, int table_id, int nb, int nang, const int *iv,
+                           const double *gamma, double *f, double *f2, int nb0,
+                           const double *wnsga2);
+
+/* statistics of the last propagate call: renormalisations (LU solves) summed over the
+ * batch, stage products executed, and the device time in milliseconds. */
+int trhepd_last_stats(const trhepd_ctx *ctx, long long *n_renorm, long long *n_products,
+                      double *device_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TRHEPD_H */

@@ -1,0 +1,29 @@
+"""Copy the reference's own regression fixtures for the surface slice loop into
+tests/golden/ref/ (data only: decks, bulkP.b and the recorded FP64 rocking curves).
+
+Source: test/test{23,47,521}/ of the reference repository, produced by its authors'
+builds (see test/test23/setup.sh and perf-test.sh there).  Run once in a container
+where the reference is mounted; the GPU hosts only read the copies.
+"""
+import os
+import shutil
+import sys
+
+SRC = sys.argv[1] if len(sys.argv) > 1 else "/root/reference/test"
+DST = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref")
+PICK = {
+    "test23": ["0.68", "0.1", "0.01"],
+    "test47": ["0.68", "0.1"],
+    "test521": ["0.68", "0.47"],
+}
+for case, dzs in PICK.items():
+    os.makedirs(os.path.join(DST, case), exist_ok=True)
+    for f in ("bulk.txt", "surf.txt", "bulkP.b"):
+        shutil.copy(os.path.join(SRC, case, f), os.path.join(DST, case, f))
+    if case != "test521":
+        shutil.copy(os.path.join(SRC, case, "surf-bulkP-str-accurate.s"), os.path.join(DST, case, "sp4-0.001.s"))
+    for m in (4, 5, 6, 7):
+        for dz in dzs:
+            shutil.copy(os.path.join(SRC, case, "test-results", "perf-%d-%s.txt" % (m, dz)),
+                        os.path.join(DST, case, "perf-%d-%s.txt" % (m, dz)))
+print("copied to", DST)

@@ -1,0 +1,58 @@
+"""The C ABI: the shared library loads, exports every entry point include/trhepd.h
+declares, and the host-only entry points (no device needed) agree with the oracle."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import REF, REPO
+from trhepd import deckio, lib
+
+
+def test_library_exports_header_symbols():
+    hdr = open(os.path.join(REPO, "include", "trhepd.h")).read()
+    names = set(re.findall(r"\b(trhepd_[a-z0-9_]+)\s*\(", hdr))
+    assert names, "no declarations parsed"
+    L = lib.load()
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+    assert names == set(lib.SYMBOLS)
+
+
+def test_srfghk_matches_oracle():
+    bp = deckio.BulkP.read(os.path.join(REF, "test521", "bulkP.b"))
+    s = O.load_case(os.path.join(REF, "test521"), 0.68, 5)
+    nb = bp.nb[0]
+    nvm = 1 + nb * (nb - 1) // 2
+    nv, igh, igk, iv = lib.srfghk(nvm, bp.domains[0].igh, bp.domains[0].igk, bp.ih, bp.ik)
+    assert nv == s.nv
+    assert np.array_equal(igh, s.pot["igh"]) and np.array_equal(igk, s.pot["igk"])
+    assert np.array_equal(iv, s.iv)
+
+
+def test_asfcrr_matches_oracle():
+    rng = np.random.default_rng(3)
+    nelm = 3
+    a = rng.uniform(-2, 2, (nelm, 6))
+    b = rng.uniform(0.3, 40, (nelm, 6))
+    c = np.zeros((nelm, 6))
+    dchg = np.array([0.0, 0.5, 0.0])
+    dth, dtk, dtz = (rng.uniform(0.3, 1.0, nelm) for _ in range(3))
+    elm = np.array([28.0855, 15.9994, 1.00794])
+    for tem in (0.0, 300.0, -1.0):
+        out = lib.asfcrr(10.0, nelm, 4, 1, a, b, c, dchg, dth, dtk, dtz, elm, tem, 3.0)
+        a2, b2, c2, d2, h2, k2 = (np.array(x, dtype=float).copy() for x in (a, b, c, dchg, dth, dtk))
+        import ctypes as C
+        wn = C.c_double()
+        O.lib().orc_asfcrr(10.0, C.byref(wn), nelm, 4, 1, a2.reshape(-1), b2.reshape(-1), c2.reshape(-1),
+                           d2, h2, k2, np.array(dtz, dtype=float), elm, tem, 3.0)
+        assert out[0] == pytest.approx(wn.value, rel=1e-15)
+        for x, y in zip(out[1:], (a2, b2, c2, d2, h2, k2)):
+            np.testing.assert_allclose(x, y, rtol=1e-14, atol=0)
+
+
+def test_bad_arguments_are_rejected():
+    with pytest.raises(lib.TrhepdError):
+        lib.srfghk(2, [0], [0], [0, 1, 2], [0, 0, 0])     # nv would exceed nvm

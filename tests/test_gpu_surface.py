@@ -1,0 +1,98 @@
+"""GPU parity of the surface slice loop on the reference's own decks.
+
+Every case runs the same inputs (the oracle's setup of the deck) through the HIP path
+via the C ABI and through the CPU oracle, and checks the HIP intensities against both
+the oracle and the reference's recorded FP64 output (tests/golden/ref).
+Tolerance (the plain FP64 bar): per-intensity relative error with a 1e-12 * max floor
+<= 1e-10 for N <= 47; at N = 521 <= 5e-9 per beam and <= 1e-10 normalised L2 (the
+reference's own rounding noise, see test_oracle_golden.py).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import REF
+from trhepd import deckio
+
+pytestmark = pytest.mark.gpu
+
+
+def _inputs(surf):
+    angs = surf.angles()
+    G = np.array([O.gamma(surf, ga, az) for (_, _, ga, az) in angs])
+    F = np.array([surf.scatter_f(r1, r2) for (r1, r2, _, _) in angs])
+    W = np.array([surf.wn * np.sin(ga) ** 2 for (_, _, ga, _) in angs])
+    return angs, G, F, W
+
+
+def _oracle_rows(surf, angs, G, F):
+    v, vi = O.table(surf)
+    out_f, out_f2 = [], []
+    for p, (r1, r2, ga, az) in enumerate(angs):
+        fo, _ = O.integrate(surf, F[p], G[p], v, vi)
+        out_f.append(fo)
+        out_f2.append(O.intensity(surf, fo, G[p], ga))
+    return np.array(out_f), np.array(out_f2)
+
+
+@pytest.mark.parametrize("case,dz,method", [
+    ("test23", "0.68", 5), ("test23", "0.68", 6), ("test23", "0.68", 4),
+    ("test23", "0.1", 5), ("test23", "0.1", 6), ("test23", "0.1", 4),
+    ("test47", "0.68", 5), ("test47", "0.68", 6), ("test47", "0.68", 4), ("test47", "0.1", 5),
+])
+def test_surface_small_vs_oracle_and_reference(engine, case, dz, method):
+    print("case", case, dz, method, flush=True)
+    surf = O.load_case(os.path.join(REF, case), float(dz), method)
+    angs, G, F, W = _inputs(surf)
+    tid = engine.table_build(surf.pot, surf.dz, surf.ns, method)
+    try:
+        fo, f2 = engine.propagate_batch(tid, surf.iv, G, F, surf.nb0, W)
+    finally:
+        engine.table_release(tid)
+    of, of2 = _oracle_rows(surf, angs, G, F)
+    ref = deckio.read_s(os.path.join(REF, case, "perf-%d-%s.txt" % (method, dz)))
+    e_or = max(O.rel_err(f2[p], of2[p]) for p in range(len(angs)))
+    e_ref = max(O.rel_err(f2[p], ref.f2[p]) for p in range(len(angs)))
+    e_f = np.max(np.abs(fo - of)) / np.max(np.abs(of))
+    print("rel vs oracle %.3e  vs reference %.3e  F %.3e" % (e_or, e_ref, e_f))
+    assert e_or < 1e-10 and e_ref < 1e-10 and e_f < 1e-11
+
+
+def test_table_matches_oracle(engine):
+    surf = O.load_case(os.path.join(REF, "test47"), 0.1, 5)
+    tid = engine.table_build(surf.pot, surf.dz, surf.ns, 5)
+    try:
+        v, vi = engine.table_fetch(tid)
+    finally:
+        engine.table_release(tid)
+    ov, ovi = O.table(surf)
+    assert v.shape == ov.shape
+    sc = max(np.max(np.abs(ov)), np.max(np.abs(ovi)))
+    assert np.max(np.abs(v - ov)) / sc < 1e-14
+    assert np.max(np.abs(vi - ovi)) / sc < 1e-14
+
+
+def test_setup_kernels_match_oracle(engine):
+    surf = O.load_case(os.path.join(REF, "test23"), 0.68, 5)
+    p = surf.pot
+    bp = deckio.BulkP.read(os.path.join(REF, "test23", "bulkP.b"))
+    sd = deckio.SurfDeck.parse(open(os.path.join(REF, "test23", "surf.txt")).read())
+    asf, ghk = engine.asfcef(p["igh"], p["igk"], surf.ghx, surf.ghy, surf.gky, p["nabr"], p["nabi"],
+                             p["a"], p["c"], p["dth"], p["dtk"], p["nelm"])
+    nv, nelm = p["nv"], p["nelm"]
+    mask = np.zeros((nelm, 6, nv), dtype=bool)
+    mask[:, :p["nabr"] + 1, :] = True
+    A = asf.reshape(nelm, 6, nv)
+    B = p["asf"].reshape(nelm, 6, nv)
+    np.testing.assert_allclose(A[mask], B[mask], rtol=1e-13, atol=1e-300)
+    np.testing.assert_allclose(ghk, p["ghk"], rtol=1e-15)
+    gh = (-2 * np.pi / bp.nh) * p["igh"].astype(float)
+    gk = (-2 * np.pi / bp.nk) * p["igk"].astype(float)
+    nb_ = p["natmb"]
+    st0 = engine.scpot_stf(sd.nsgs, sd.msa, sd.msb, sd.nsa, sd.nsb, gh, gk, p["x"][nb_:], p["y"][nb_:],
+                           sd.dxs + bp.dx, sd.dys + bp.dy)
+    np.testing.assert_allclose(st0.reshape(-1), p["st0"], rtol=0, atol=1e-12)
+    st1 = engine.scpot_stf(bp.nsg, 1, 0, 0, 1, gh[:p["nvb"]], gk[:p["nvb"]], p["x"][:nb_], p["y"][:nb_], bp.dx, bp.dy)
+    np.testing.assert_allclose(st1.reshape(-1), p["st1"], rtol=0, atol=1e-12)

@@ -1,0 +1,399 @@
+// C ABI of libtrhepd_hip.so: context, memory accounting and the entry points that
+// replace the reference's per-structure setup, *_int_init and the angle loop of srfref.
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <new>
+
+#include "internal.hpp"
+
+namespace trhepd {
+
+int fail(trhepd_ctx *ctx, int code, const char *fmt, ...)
+{
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return code;
+}
+
+int hip_check(trhepd_ctx *ctx, hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return TRHEPD_OK;
+    return fail(ctx, e == hipErrorOutOfMemory ? TRHEPD_E_NOMEM : TRHEPD_E_HIP, "%s: %s", what,
+                hipGetErrorString(e));
+}
+
+int dev_alloc(trhepd_ctx *ctx, void **p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0) return TRHEPD_OK;
+    if (ctx->budget && ctx->held + bytes > ctx->budget)
+        return fail(ctx, TRHEPD_E_BUDGET, "allocation of %zu bytes exceeds the HBM budget (%zu of %zu held)",
+                    bytes, ctx->held, ctx->budget);
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return hip_check(ctx, e, "hipMalloc");
+    }
+    ctx->held += bytes;
+    return TRHEPD_OK;
+}
+
+void dev_free(trhepd_ctx *ctx, void *p, size_t bytes)
+{
+    if (!p) return;
+    (void)hipFree(p);
+    ctx->held -= bytes;
+}
+
+int scratch(trhepd_ctx *ctx, int slot, size_t bytes, void **p)
+{
+    if ((int)ctx->bufs.size() <= slot) ctx->bufs.resize(slot + 1);
+    DevBuf &b = ctx->bufs[slot];
+    if (b.bytes < bytes) {
+        if (b.p) {
+            TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            dev_free(ctx, b.p, b.bytes);
+            b.p = nullptr;
+            b.bytes = 0;
+        }
+        const size_t want = (bytes + 4095) & ~size_t(4095);
+        int rc = dev_alloc(ctx, &b.p, want);
+        if (rc) return rc;
+        b.bytes = want;
+    }
+    *p = b.p;
+    return TRHEPD_OK;
+}
+
+Table *find_table(trhepd_ctx *ctx, int id)
+{
+    for (Table &t : ctx->tables)
+        if (t.id == id) return &t;
+    return nullptr;
+}
+
+// surf_prkn.f90:11-52 (Blanes & Moan, JCAM 142 (2002) 313-330)
+RknCoef rkn_coef(int method)
+{
+    static const double b4[13] = {
+        0.0829844064174052, 0.245298957184271, 0.396309801498368, 0.604872665711080,
+        -0.0390563049223486, -0.350171622895351, 0.1195241940131508, -0.350171622895351,
+        -0.0390563049223486, 0.604872665711080, 0.396309801498368, 0.245298957184271,
+        0.0829844064174052};
+    static const double b6[23] = {
+        0.0414649985182624, 0.123229775946271, 0.198128671918067, 0.290553797799558,
+        -0.0400061921041533, -0.127049212625417, 0.0752539843015807, -0.246331761062075,
+        -0.0115113874206879, 0.357208872795928, 0.2366699247869311, 0.20477705429147,
+        0.2366699247869311, 0.357208872795928, -0.0115113874206879, -0.246331761062075,
+        0.0752539843015807, -0.127049212625417, -0.0400061921041533, 0.290553797799558,
+        0.198128671918067, 0.123229775946271, 0.0414649985182624};
+    RknCoef c{};
+    if (method == TRHEPD_SP4) {
+        c.s = 6;
+        std::memcpy(c.bab, b4, sizeof b4);
+    } else if (method == TRHEPD_SP6) {
+        c.s = 11;
+        std::memcpy(c.bab, b6, sizeof b6);
+    } else {   // Stormer-Verlet: one kick, one drift per slice
+        c.s = 1;
+        c.bab[0] = 1.0;
+        c.bab[1] = 1.0;
+        c.bab[2] = 1.0;
+    }
+    return c;
+}
+
+}  // namespace trhepd
+
+using namespace trhepd;
+
+extern "C" {
+
+int trhepd_ctx_create(const trhepd_opts *opts, trhepd_ctx **out)
+{
+    if (!out) return TRHEPD_E_ARG;
+    *out = nullptr;
+    trhepd_ctx *c = new (std::nothrow) trhepd_ctx();
+    if (!c) return TRHEPD_E_NOMEM;
+    if (opts) {
+        c->device = opts->device;
+        c->precision = opts->precision;
+        c->budget = opts->hbm_budget;
+    }
+    if (c->precision != TRHEPD_PREC_FP64 && c->precision != TRHEPD_PREC_MIXED) {
+        delete c;
+        return TRHEPD_E_ARG;
+    }
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0 || c->device < 0 || c->device >= ndev) {
+        delete c;
+        return e != hipSuccess ? TRHEPD_E_HIP : TRHEPD_E_ARG;
+    }
+    e = hipSetDevice(c->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e != hipSuccess) {
+        if (c->stream) hipStreamDestroy(c->stream);
+        delete c;
+        return TRHEPD_E_HIP;
+    }
+    *out = c;
+    return TRHEPD_OK;
+}
+
+int trhepd_ctx_destroy(trhepd_ctx *ctx)
+{
+    if (!ctx) return TRHEPD_E_ARG;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    for (Table &t : ctx->tables) dev_free(ctx, t.d, t.bytes);
+    for (DevBuf &b : ctx->bufs) dev_free(ctx, b.p, b.bytes);
+    hipEventDestroy(ctx->ev0);
+    hipEventDestroy(ctx->ev1);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return TRHEPD_OK;
+}
+
+const char *trhepd_last_error(const trhepd_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+size_t trhepd_ctx_bytes(const trhepd_ctx *ctx) { return ctx ? ctx->held : 0; }
+
+int trhepd_asfcef(trhepd_ctx *ctx, int nv, int nelm, const int *igh, const int *igk, double ghx,
+                  double ghy, double gky, int nabr, int nabi, const double *a, const double *c,
+                  const double *dth, const double *dtk, double *asf, double *ghk)
+{
+    if (!ctx) return TRHEPD_E_ARG;
+    if (nv < 1 || nelm < 1 || nabr < 0 || nabr + nabi > 6 || !igh || !igk || !a || !c || !dth || !dtk ||
+        !asf || !ghk)
+        return fail(ctx, TRHEPD_E_ARG, "asfcef: bad arguments");
+    TRY_HIP(ctx, hipSetDevice(ctx->device));
+    size_t off = 0;
+    auto take = [&](size_t b) { size_t o = off; off += (b + 255) & ~size_t(255); return o; };
+    const size_t oh = take(nv * 4), ok = take(nv * 4), oa = take(6 * nelm * 8), oc = take(6 * nelm * 8),
+                 o1 = take(nelm * 8), o2 = take(nelm * 8), oasf = take((size_t)nv * 6 * nelm * 8),
+                 og = take(nv * 8);
+    char *d;
+    int rc = dev_alloc(ctx, (void **)&d, off);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(d + oh, igh, nv * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (!e) e = hipMemcpyAsync(d + ok, igk, nv * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (!e) e = hipMemcpyAsync(d + oa, a, 6 * nelm * 8, hipMemcpyHostToDevice, ctx->stream);
+    if (!e) e = hipMemcpyAsync(d + oc, c, 6 * nelm * 8, hipMemcpyHostToDevice, ctx->stream);
+    if (!e) e = hipMemcpyAsync(d + o1, dth, nelm * 8, hipMemcpyHostToDevice, ctx->stream);
+    if (!e) e = hipMemcpyAsync(d + o2, dtk, nelm * 8, hipMemcpyHostToDevice, ctx->stream);
+    if (!e) {
+        rc = launch_asfcef(ctx, nv, nelm, (int *)(d + oh), (int *)(d + ok), ghx, ghy, gky, nabr, nabi,
+                           (double *)(d + oa), (double *)(d + oc), (double *)(d + o1), (double *)(d + o2),
+                           (double *)(d + oasf), (double *)(d + og));
+        if (!rc) e = hipMemcpyAsync(asf, d + oasf, (size_t)nv * 6 * nelm * 8, hipMemcpyDeviceToHost, ctx->stream);
+        if (!rc && !e) e = hipMemcpyAsync(ghk, d + og, nv * 8, hipMemcpyDeviceToHost, ctx->stream);
+    }
+    hipError_t e2 = hipStreamSynchronize(ctx->stream);
+    dev_free(ctx, d, off);
+    if (rc) return rc;
+    if (e) return hip_check(ctx, e, "asfcef");
+    return hip_check(ctx, e2, "asfcef sync");
+}
+
+int trhepd_scpot_stf(trhepd_ctx *ctx, int natm, int nv, int nsg, int ma, int mb, int na, int nb,
+                     const double *gh, const double *gk, const double *x, const double *y, double dx,
+                     double dy, double *st)
+{
+    if (!ctx) return TRHEPD_E_ARG;
+    if (natm < 0 || nv < 1 || nsg < 1 || nsg > 17 || ma * nb - mb * na == 0 || !gh || !gk || !st ||
+        (natm > 0 && (!x || !y)))
+        return fail(ctx, TRHEPD_E_ARG, "scpot_stf: bad arguments (nsg must be 1..17)");
+    if (natm == 0) return TRHEPD_OK;
+    TRY_HIP(ctx, hipSetDevice(ctx->device));
+    // orbit expansion (strfac.f90:15-65): host, <= 12 positions per atom
+    std::vector<int> ndeg(natm);
+    std::vector<double> xb(12 * (size_t)natm), yb(12 * (size_t)natm);
+    const double s = (double)(ma * nb - mb * na);
+    auto red = [](double v) { return v - std::floor(v + 1e-4); };
+    for (int j = 0; j < natm; ++j) {
+        double xs[12], ys[12];
+        const double x0 = red((nb * x[j] - na * y[j]) / s), y0 = red((ma * y[j] - mb * x[j]) / s);
+        const int n = orbit(nsg, x0, y0, xs, ys);
+        ndeg[j] = n;
+        for (int i = 0; i < n; ++i) {
+            const double u = red(xs[i]), w = red(ys[i]);
+            xb[12 * j + i] = ma * u + na * w;
+            yb[12 * j + i] = mb * u + nb * w;
+        }
+    }
+    size_t off = 0;
+    auto take = [&](size_t b) { size_t o = off; off += (b + 255) & ~size_t(255); return o; };
+    const size_t o_gh = take(nv * 8), o_gk = take(nv * 8), o_n = take(natm * 4), o_x = take(12 * natm * 8),
+                 o_y = take(12 * natm * 8), o_st = take((size_t)nv * natm * 16);
+    char *d;
+    int rc = dev_alloc(ctx, (void **)&d, off);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(d + o_gh, gh, nv * 8, hipMemcpyHostToDevice, ctx->stream);
+    if (!e) e = hipMemcpyAsync(d + o_gk, gk, nv * 8, hipMemcpyHostToDevice, ctx->stream);
+    if (!e) e = hipMemcpyAsync(d + o_n, ndeg.data(), natm * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (!e) e = hipMemcpyAsync(d + o_x, xb.data(), 12 * natm * 8, hipMemcpyHostToDevice, ctx->stream);
+    if (!e) e = hipMemcpyAsync(d + o_y, yb.data(), 12 * natm * 8, hipMemcpyHostToDevice, ctx->stream);
+    if (!e) {
+        rc = launch_strfac(ctx, natm, nv, (double *)(d + o_gh), (double *)(d + o_gk), (int *)(d + o_n),
+                           (double *)(d + o_x), (double *)(d + o_y), dx, dy, (double *)(d + o_st));
+        if (!rc) e = hipMemcpyAsync(st, d + o_st, (size_t)nv * natm * 16, hipMemcpyDeviceToHost, ctx->stream);
+    }
+    hipError_t e2 = hipStreamSynchronize(ctx->stream);
+    dev_free(ctx, d, off);
+    if (rc) return rc;
+    if (e) return hip_check(ctx, e, "scpot_stf");
+    return hip_check(ctx, e2, "scpot_stf sync");
+}
+
+int trhepd_table_build(trhepd_ctx *ctx, const trhepd_potential *pot, double dz, int ns, int method,
+                       int *table_id)
+{
+    if (!ctx) return TRHEPD_E_ARG;
+    if (!pot || !table_id || ns < 1 || !(dz > 0.0) || pot->nv < 1 || pot->nvb < 1 || pot->nvb > pot->nv ||
+        pot->nelm < 1 || pot->nabr < 0 || pot->nabr + pot->nabi > 6 || pot->natms < 0 || pot->natmb < 0 ||
+        !pot->ielm || !pot->z || !pot->dchg || !pot->ghk || !pot->ocr || !pot->b || !pot->asf ||
+        (pot->natms && !pot->st0) || (pot->natmb && (!pot->st1 || !pot->st2)))
+        return fail(ctx, TRHEPD_E_ARG, "table_build: bad potential descriptor");
+    if (method < TRHEPD_M2 || method > TRHEPD_RK4GS) return fail(ctx, TRHEPD_E_ARG, "unknown int_method %d", method);
+    for (int i = 0; i < pot->natmb + pot->natms; ++i)
+        if (pot->ielm[i] < 1 || pot->ielm[i] > pot->nelm)
+            return fail(ctx, TRHEPD_E_ARG, "ielm(%d) = %d out of 1..%d", i + 1, pot->ielm[i], pot->nelm);
+    TRY_HIP(ctx, hipSetDevice(ctx->device));
+    Table t;
+    t.method = method;
+    t.ns = ns;
+    t.dz = dz;
+    int rc = launch_table(ctx, pot, t);
+    if (rc) return rc;
+    t.id = ctx->next_table++;
+    ctx->tables.push_back(t);
+    *table_id = t.id;
+    return TRHEPD_OK;
+}
+
+int trhepd_table_shape(trhepd_ctx *ctx, int table_id, int *nv, int *ncols)
+{
+    if (!ctx) return TRHEPD_E_ARG;
+    Table *t = find_table(ctx, table_id);
+    if (!t) return fail(ctx, TRHEPD_E_NOTFOUND, "no table %d", table_id);
+    if (nv) *nv = t->nv;
+    if (ncols) *ncols = t->ncols;
+    return TRHEPD_OK;
+}
+
+int trhepd_table_fetch(trhepd_ctx *ctx, int table_id, double *v, double *vi)
+{
+    if (!ctx) return TRHEPD_E_ARG;
+    Table *t = find_table(ctx, table_id);
+    if (!t) return fail(ctx, TRHEPD_E_NOTFOUND, "no table %d", table_id);
+    if (!v || !vi) return fail(ctx, TRHEPD_E_ARG, "table_fetch: null output");
+    TRY_HIP(ctx, hipSetDevice(ctx->device));
+    std::vector<VVi> h((size_t)t->nv * t->ncols);
+    TRY_HIP(ctx, hipMemcpyAsync(h.data(), t->d, h.size() * sizeof(VVi), hipMemcpyDeviceToHost, ctx->stream));
+    TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (size_t e = 0; e < h.size(); ++e) {
+        v[2 * e] = h[e].v.x;
+        v[2 * e + 1] = h[e].v.y;
+        vi[2 * e] = h[e].vi.x;
+        vi[2 * e + 1] = h[e].vi.y;
+    }
+    return TRHEPD_OK;
+}
+
+int trhepd_table_release(trhepd_ctx *ctx, int table_id)
+{
+    if (!ctx) return TRHEPD_E_ARG;
+    for (size_t i = 0; i < ctx->tables.size(); ++i)
+        if (ctx->tables[i].id == table_id) {
+            TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            dev_free(ctx, ctx->tables[i].d, ctx->tables[i].bytes);
+            ctx->tables.erase(ctx->tables.begin() + i);
+            return TRHEPD_OK;
+        }
+    return fail(ctx, TRHEPD_E_NOTFOUND, "no table %d", table_id);
+}
+
+int trhepd_propagate_batch(trhepd_ctx *ctx, int table_id, int nb, int nang, const int *iv,
+                           const double *gamma, double *f, double *f2, int nb0, const double *wnsga2)
+{
+    if (!ctx) return TRHEPD_E_ARG;
+    Table *t = find_table(ctx, table_id);
+    if (!t) return fail(ctx, TRHEPD_E_NOTFOUND, "no table %d", table_id);
+    if (nb < 1 || nang < 0 || !iv || !gamma || !f || nb0 < 1 || nb0 > nb || (f2 && !wnsga2))
+        return fail(ctx, TRHEPD_E_ARG, "propagate_batch: bad arguments");
+    if (t->method != TRHEPD_SP4 && t->method != TRHEPD_SP6 && t->method != TRHEPD_LF)
+        return fail(ctx, TRHEPD_E_ARG, "int_method %d is not available on the device path", t->method);
+    ctx->stats = Stats();
+    if (nang == 0) return TRHEPD_OK;
+    // signed gather map (row-major): upper i<j -> +iv(j,i), lower i>j -> -iv(i,j), diagonal 0
+    std::vector<int> gm((size_t)nb * nb);
+    for (int i = 0; i < nb; ++i)
+        for (int j = 0; j < nb; ++j) {
+            int g = 0;
+            if (i < j) g = iv[j + (size_t)i * nb];
+            else if (i > j) g = -iv[i + (size_t)j * nb];
+            if ((i != j) && (g == 0 || std::abs(g) > t->nv))
+                return fail(ctx, TRHEPD_E_ARG, "iv(%d,%d) = %d outside 1..nv=%d", i > j ? i + 1 : j + 1,
+                            i > j ? j + 1 : i + 1, std::abs(g), t->nv);
+            gm[(size_t)i * nb + j] = g;
+        }
+    TRY_HIP(ctx, hipSetDevice(ctx->device));
+    const size_t nn = (size_t)nb * nb;
+    void *p_gm, *p_g, *p_f, *p_f2, *p_w, *p_cnt;
+    int rc;
+    if ((rc = scratch(ctx, 0, nn * 4, &p_gm)) || (rc = scratch(ctx, 1, (size_t)nb * nang * 16, &p_g)) ||
+        (rc = scratch(ctx, 2, nn * nang * 16, &p_f)) || (rc = scratch(ctx, 3, (size_t)nb * nang * 8, &p_f2)) ||
+        (rc = scratch(ctx, 4, (size_t)nang * 8, &p_w)) || (rc = scratch(ctx, 5, 64, &p_cnt)))
+        return rc;
+    TRY_HIP(ctx, hipMemcpyAsync(p_gm, gm.data(), nn * 4, hipMemcpyHostToDevice, ctx->stream));
+    TRY_HIP(ctx, hipMemcpyAsync(p_g, gamma, (size_t)nb * nang * 16, hipMemcpyHostToDevice, ctx->stream));
+    TRY_HIP(ctx, hipMemcpyAsync(p_f, f, nn * nang * 16, hipMemcpyHostToDevice, ctx->stream));
+    if (f2) TRY_HIP(ctx, hipMemcpyAsync(p_w, wnsga2, (size_t)nang * 8, hipMemcpyHostToDevice, ctx->stream));
+    TRY_HIP(ctx, hipMemsetAsync(p_cnt, 0, 64, ctx->stream));
+    TRY_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    if (nb <= 48)
+        rc = run_small(ctx, *t, nb, nang, (int *)p_gm, (double2 *)p_g, (double2 *)p_f, f2 ? (double *)p_f2 : nullptr,
+                       nb0, (double *)p_w, (int *)p_cnt);
+    else
+        rc = run_batched(ctx, *t, nb, nang, (int *)p_gm, (double2 *)p_g, (double2 *)p_f,
+                         f2 ? (double *)p_f2 : nullptr, nb0, (double *)p_w, (int *)p_cnt);
+    if (rc) return rc;
+    TRY_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    TRY_HIP(ctx, hipMemcpyAsync(f, p_f, nn * nang * 16, hipMemcpyDeviceToHost, ctx->stream));
+    if (f2) TRY_HIP(ctx, hipMemcpyAsync(f2, p_f2, (size_t)nb * nang * 8, hipMemcpyDeviceToHost, ctx->stream));
+    int cnt[16];
+    TRY_HIP(ctx, hipMemcpyAsync(cnt, p_cnt, 64, hipMemcpyDeviceToHost, ctx->stream));
+    TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    TRY_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    const RknCoef co = rkn_coef(t->method);
+    ctx->stats.n_renorm = cnt[0];
+    ctx->stats.n_products = (long long)nang * t->ns * co.s;
+    ctx->stats.device_ms = ms;
+    if (cnt[1]) return fail(ctx, TRHEPD_E_SINGULAR, "exact zero pivot in a renormalisation LU");
+    if (f2)
+        for (size_t e = 0; e < (size_t)nb * nang; ++e)
+            if (!std::isfinite(f2[e])) return fail(ctx, TRHEPD_E_NONFINITE, "non-finite intensity at %zu", e);
+    return TRHEPD_OK;
+}
+
+int trhepd_last_stats(const trhepd_ctx *ctx, long long *n_renorm, long long *n_products, double *device_ms)
+{
+    if (!ctx) return TRHEPD_E_ARG;
+    if (n_renorm) *n_renorm = ctx->stats.n_renorm;
+    if (n_products) *n_products = ctx->stats.n_products;
+    if (device_ms) *device_ms = ctx->stats.device_ms;
+    return TRHEPD_OK;
+}
+
+}  // extern "C"

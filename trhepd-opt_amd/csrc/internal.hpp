@@ -1,0 +1,108 @@
+// Internal declarations shared by the translation units of libtrhepd_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/trhepd.h"
+
+namespace trhepd {
+
+// Table entry: the two potential components of one Fourier coefficient at one depth.
+struct VVi {
+    double2 v;
+    double2 vi;
+};
+
+// Coefficients of the symmetric RKN schemes (surf_prkn.f90:9-52, Blanes & Moan 2002):
+// bab[2k-2] = c_b of stage k, bab[2k-1] = c_a of stage k, k = 1..s.
+struct RknCoef {
+    int s;
+    double bab[23];
+};
+
+// The integrator stages the batched path runs, flattened for one slice.
+RknCoef rkn_coef(int method);
+
+struct Table {
+    int id = 0;
+    int method = 0;
+    int nv = 0;
+    int ncols = 0;
+    int ns = 0;
+    double dz = 0.0;
+    VVi *d = nullptr;   // [ncols][nv]
+    size_t bytes = 0;
+};
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct Stats {
+    long long n_renorm = 0;
+    long long n_products = 0;
+    double device_ms = 0.0;
+};
+
+}  // namespace trhepd
+
+struct trhepd_ctx {
+    int device = 0;
+    int precision = 0;
+    size_t budget = 0;
+    size_t held = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    int next_table = 1;
+    std::vector<trhepd::Table> tables;
+    // scratch owned by the context, grown on demand (kept across calls)
+    std::vector<trhepd::DevBuf> bufs;
+    trhepd::Stats stats;
+    void *tune = nullptr;   // autotuned launch choices
+};
+
+namespace trhepd {
+
+int fail(trhepd_ctx *ctx, int code, const char *fmt, ...);
+int hip_check(trhepd_ctx *ctx, hipError_t e, const char *what);
+#define TRY_HIP(ctx, call)                                               \
+    do {                                                                 \
+        hipError_t e_ = (call);                                          \
+        if (e_ != hipSuccess) return ::trhepd::hip_check(ctx, e_, #call); \
+    } while (0)
+
+// device allocation accounted against the context budget
+int dev_alloc(trhepd_ctx *ctx, void **p, size_t bytes);
+void dev_free(trhepd_ctx *ctx, void *p, size_t bytes);
+// scratch slot `slot` of at least `bytes` (reused across calls)
+int scratch(trhepd_ctx *ctx, int slot, size_t bytes, void **p);
+
+Table *find_table(trhepd_ctx *ctx, int id);
+
+// plane-group orbit of a fractional position (strfac.f90:15-65 and its 17 group routines)
+int orbit(int nsg, double x, double y, double *xs, double *ys);
+
+// kernels launchers ---------------------------------------------------------
+int launch_asfcef(trhepd_ctx *ctx, int nv, int nelm, const int *igh, const int *igk, double ghx,
+                  double ghy, double gky, int nabr, int nabi, const double *a, const double *c,
+                  const double *dth, const double *dtk, double *asf, double *ghk);
+int launch_strfac(trhepd_ctx *ctx, int natm, int nv, const double *gh, const double *gk,
+                  const int *ndeg, const double *xb, const double *yb, double dx, double dy,
+                  double *st);
+int launch_table(trhepd_ctx *ctx, const trhepd_potential *pot, Table &t);
+
+// slice-loop drivers
+int run_small(trhepd_ctx *ctx, const Table &t, int nb, int nang, const int *d_gmap,
+              const double2 *d_gamma, double2 *d_f, double *d_f2, int nb0, const double *d_wnsga2,
+              int *d_counters);
+int run_batched(trhepd_ctx *ctx, const Table &t, int nb, int nang, const int *d_gmap,
+                const double2 *d_gamma, double2 *d_f, double *d_f2, int nb0,
+                const double *d_wnsga2, int *d_counters);
+
+}  // namespace trhepd

@@ -1,0 +1,472 @@
+// Fused slice loop for small beam counts (N <= 48): one workgroup owns one angle for
+// the whole integration.  The ratio-form state [X^H | Y^H] (here Q | P) and the slice
+// matrix M live in LDS; the stage products Q*M run on v_mfma_f64_16x16x4f64 (four real
+// products per complex block), the renormalisation is an in-LDS LU with partial pivoting.
+//
+// Reference: hp_int_spa surf_prkn.f90:278-330 (RKN), lf_int_spa surf_magnus.f90:46-110
+// (Stormer-Verlet), invstransh/stransh matcomp.f90:140-196, vvi2mate matcomp.f90:253-279,
+// cond_est matcomp.f90:498-528, matAbdivB matcomp.f90:53-69, intensity surflf.f90:143-151.
+//
+// The end-of-slice kick of the RKN schemes is shared with the first stage of the next
+// slice (psave, surf_prkn.f90:306-315).  cond_est reads only Q, which that kick does not
+// change, so the renormalisation decision is taken first and the two kicks are applied
+// as one product with coefficient 2*c_b1 when no renormalisation follows.
+#include "internal.hpp"
+
+namespace trhepd {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ d4 mfma4(double a, double b, d4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+template <int NP>
+struct SLds {
+    static constexpr int LD = NP + 1;
+    double qr[NP * LD], qi[NP * LD];   // Q = X^H, column-major
+    double pr[NP * LD], pi[NP * LD];   // P = Y^H, column-major
+    double mr[NP * LD], mi[NP * LD];   // M, row-major
+    int gm[NP * NP];                   // signed gather map, row-major
+    double g2r[NP], g2i[NP];           // conj(gamma^2)
+    double gcr[NP], gci[NP];           // conj(gamma)
+    double red0[NP], red1[NP];
+    int piv;
+    int flag;
+};
+
+struct SmallArgs {
+    const VVi *table;
+    int nv, ns, N, method;
+    double dz;
+    RknCoef co;
+    const int *gmap;        // (N*N) row-major signed map
+    const double2 *gamma;   // [nang][N]
+    double2 *f;             // [nang][N*N] column-major
+    double *f2;             // [nang][N] or null
+    int nb0;                // 1-based
+    const double *wnsga2;   // [nang]
+    int *counters;          // [0] renorm count, [1] singular flag
+};
+
+__device__ __forceinline__ double2 cdiv(double ar, double ai, double br, double bi)
+{
+    // Smith's algorithm
+    if (fabs(br) >= fabs(bi)) {
+        const double r = bi / br, den = br + bi * r;
+        return make_double2((ar + ai * r) / den, (ai - ar * r) / den);
+    }
+    const double r = br / bi, den = bi + br * r;
+    return make_double2((ar * r + ai) / den, (ai * r - ar) / den);
+}
+
+// M(k, j) of column c into LDS (vvi2mate): upper -conj(v+vi), lower -v+vi,
+// diagonal -conj(v+vi)[1] - conj(gamma_j^2).
+template <int NP>
+__device__ void gather_m(SLds<NP> &S, const VVi *__restrict__ col, int N)
+{
+    constexpr int LD = SLds<NP>::LD;
+    for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
+        const int k = e / N, j = e - k * N;
+        const int g = S.gm[e];
+        double vr, vi_;
+        if (g > 0) {
+            const VVi t = col[g - 1];
+            vr = -(t.v.x + t.vi.x);
+            vi_ = t.v.y + t.vi.y;
+        } else if (g < 0) {
+            const VVi t = col[-g - 1];
+            vr = -t.v.x + t.vi.x;
+            vi_ = -t.v.y + t.vi.y;
+        } else {
+            const VVi t = col[0];
+            vr = -(t.v.x + t.vi.x) - S.g2r[j];
+            vi_ = (t.v.y + t.vi.y) - S.g2i[j];
+        }
+        S.mr[k * LD + j] = vr;
+        S.mi[k * LD + j] = vi_;
+    }
+}
+
+// P += alpha * Q * M on the matrix cores; each wave owns output blocks blk = wave + 4t.
+template <int NP>
+__device__ void gemm_acc(SLds<NP> &S, int N, double alpha)
+{
+    constexpr int LD = SLds<NP>::LD;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int nbk = (N + 15) >> 4, ks = (N + 3) >> 2;
+    const int kq = lane >> 4, l16 = lane & 15;
+    for (int blk = wave; blk < nbk * nbk; blk += nw) {
+        const int bi = blk / nbk, bj = blk - bi * nbk;
+        const int ar = bi * 16 + l16, bc = bj * 16 + l16;
+        d4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+        for (int s = 0; s < ks; ++s) {
+            const int kk = s * 4 + kq;
+            const double a_r = S.qr[ar + kk * LD], a_i = S.qi[ar + kk * LD];
+            const double b_r = S.mr[kk * LD + bc], b_i = S.mi[kk * LD + bc];
+            cr = mfma4(a_r, b_r, cr);
+            cr = mfma4(-a_i, b_i, cr);
+            ci = mfma4(a_r, b_i, ci);
+            ci = mfma4(a_i, b_r, ci);
+        }
+        const int j = bj * 16 + l16;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = bi * 16 + 4 * kq + r;
+            if (i < N && j < N) {
+                S.pr[i + j * LD] += alpha * cr[r];
+                S.pi[i + j * LD] += alpha * ci[r];
+            }
+        }
+    }
+}
+
+// [Q | P] <- [LU | Q^{-1} P]: zgetrf(N, 2N) with |re|+|im| partial pivoting, then
+// back substitution with the upper factor (matAbdivB, matcomp.f90:53-69).
+template <int NP>
+__device__ void lu_solve(SLds<NP> &S, int N)
+{
+    constexpr int LD = SLds<NP>::LD;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nt = blockDim.x;
+    for (int k = 0; k < N; ++k) {
+        if (wave == 0) {
+            double best = -1.0;
+            int bi = N;
+            for (int i = k + lane; i < N; i += 64) {
+                const double v = fabs(S.qr[i + k * LD]) + fabs(S.qi[i + k * LD]);
+                if (v > best) { best = v; bi = i; }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ob = __shfl_xor(best, off);
+                const int oi = __shfl_xor(bi, off);
+                if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+            }
+            if (lane == 0) {
+                S.piv = bi;
+                if (best == 0.0) S.flag = 1;
+            }
+        }
+        __syncthreads();
+        const int p = S.piv;
+        if (p != k) {
+            for (int c = tid; c < 2 * N; c += nt) {
+                double *re = c < N ? S.qr : S.pr, *im = c < N ? S.qi : S.pi;
+                const int cc = c < N ? c : c - N;
+                const double tr = re[k + cc * LD], ti = im[k + cc * LD];
+                re[k + cc * LD] = re[p + cc * LD];
+                im[k + cc * LD] = im[p + cc * LD];
+                re[p + cc * LD] = tr;
+                im[p + cc * LD] = ti;
+            }
+        }
+        __syncthreads();
+        const double pkr = S.qr[k + k * LD], pki = S.qi[k + k * LD];
+        if (pkr != 0.0 || pki != 0.0) {
+            const double2 rp = cdiv(1.0, 0.0, pkr, pki);
+            for (int i = k + 1 + tid; i < N; i += nt) {
+                const double lr = S.qr[i + k * LD], li = S.qi[i + k * LD];
+                S.qr[i + k * LD] = lr * rp.x - li * rp.y;
+                S.qi[i + k * LD] = lr * rp.y + li * rp.x;
+            }
+        }
+        __syncthreads();
+        const int nr = N - 1 - k, ncq = N - 1 - k, nc = ncq + N;
+        for (int e = tid; e < nr * nc; e += nt) {
+            const int i = k + 1 + e % nr, c = e / nr;
+            const double lr = S.qr[i + k * LD], li = S.qi[i + k * LD];
+            double *re, *im;
+            int cc;
+            if (c < ncq) { re = S.qr; im = S.qi; cc = k + 1 + c; }
+            else { re = S.pr; im = S.pi; cc = c - ncq; }
+            const double ur = re[k + cc * LD], ui = im[k + cc * LD];
+            re[i + cc * LD] -= lr * ur - li * ui;
+            im[i + cc * LD] -= lr * ui + li * ur;
+        }
+        __syncthreads();
+    }
+    // back substitution, one right-hand column per thread
+    for (int j = tid; j < N; j += nt) {
+        for (int k = N - 1; k >= 0; --k) {
+            const double2 x = cdiv(S.pr[k + j * LD], S.pi[k + j * LD], S.qr[k + k * LD], S.qi[k + k * LD]);
+            S.pr[k + j * LD] = x.x;
+            S.pi[k + j * LD] = x.y;
+            for (int i = 0; i < k; ++i) {
+                const double ur = S.qr[i + k * LD], ui = S.qi[i + k * LD];
+                S.pr[i + j * LD] -= x.x * ur - x.y * ui;
+                S.pi[i + j * LD] -= x.x * ui + x.y * ur;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <int NP>
+__global__ __launch_bounds__(256) void k_small(SmallArgs A)
+{
+    __shared__ SLds<NP> S;
+    constexpr int LD = SLds<NP>::LD;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int pt = blockIdx.x;
+    const int N = A.N;
+    const double dz = A.dz;
+    for (int e = tid; e < NP * LD; e += nt) {
+        S.qr[e] = 0; S.qi[e] = 0; S.pr[e] = 0; S.pi[e] = 0; S.mr[e] = 0; S.mi[e] = 0;
+    }
+    for (int e = tid; e < N * N; e += nt) S.gm[e] = A.gmap[e];
+    for (int j = tid; j < N; j += nt) {
+        const double2 g = A.gamma[(size_t)pt * N + j];
+        S.g2r[j] = g.x * g.x - g.y * g.y;    // conj(g*g)
+        S.g2i[j] = -(g.x * g.y + g.y * g.x);
+        S.gcr[j] = g.x;
+        S.gci[j] = -g.y;
+    }
+    if (tid == 0) S.flag = 0;
+    __syncthreads();
+    // invstransh: X = I + F^H, Y = I - F^H
+    const double2 *F = A.f + (size_t)pt * N * N;
+    for (int e = tid; e < N * N; e += nt) {
+        const int i = e % N, j = e / N;
+        const double2 fji = F[j + i * N];
+        const double d = (i == j) ? 1.0 : 0.0;
+        S.qr[i + j * LD] = d + fji.x;
+        S.qi[i + j * LD] = -fji.y;
+        S.pr[i + j * LD] = d - fji.x;
+        S.pi[i + j * LD] = fji.y;
+    }
+    __syncthreads();
+    lu_solve<NP>(S, N);
+    for (int e = tid; e < N * N; e += nt) {
+        const int i = e % N, j = e / N;
+        // (0,1) * Y(i,j) * conj(g_i)
+        const double yr = -S.pi[i + j * LD], yi = S.pr[i + j * LD];
+        S.pr[i + j * LD] = yr * S.gcr[i] - yi * S.gci[i];
+        S.pi[i + j * LD] = yr * S.gci[i] + yi * S.gcr[i];
+    }
+    __syncthreads();
+
+    int xid = 1, nren = 0;
+    if (A.method == TRHEPD_LF) {   // X^H = I + (dz/2) Y^H  (surf_magnus.f90:63-67)
+        for (int e = tid; e < N * N; e += nt) {
+            const int i = e % N, j = e / N;
+            S.qr[i + j * LD] = (dz / 2) * S.pr[i + j * LD] + (i == j ? 1.0 : 0.0);
+            S.qi[i + j * LD] = (dz / 2) * S.pi[i + j * LD];
+        }
+        __syncthreads();
+        xid = 0;
+    }
+    const int s = A.co.s;
+    const double cb1 = A.co.bab[0];
+    for (int l = 1; l <= A.ns; ++l) {
+        if (A.method == TRHEPD_LF) {
+            // Stormer-Verlet: kick with M(l), drift dz (dz/2 on the last slice)
+            const VVi *col = A.table + (size_t)(l - 1) * A.nv;
+            gather_m<NP>(S, col, N);
+            __syncthreads();
+            if (xid) {
+                for (int e = tid; e < N * N; e += nt) {
+                    const int i = e % N, j = e / N;
+                    S.pr[i + j * LD] += dz * S.mr[i * LD + j];
+                    S.pi[i + j * LD] += dz * S.mi[i * LD + j];
+                }
+            } else {
+                gemm_acc<NP>(S, N, dz);
+            }
+            __syncthreads();
+            const double dz2 = (l == A.ns) ? dz / 2 : dz;
+            for (int e = tid; e < N * N; e += nt) {
+                const int i = e % N, j = e / N;
+                if (xid) {
+                    S.qr[i + j * LD] = dz2 * S.pr[i + j * LD] + (i == j ? 1.0 : 0.0);
+                    S.qi[i + j * LD] = dz2 * S.pi[i + j * LD];
+                } else {
+                    S.qr[i + j * LD] += dz2 * S.pr[i + j * LD];
+                    S.qi[i + j * LD] += dz2 * S.pi[i + j * LD];
+                }
+            }
+            __syncthreads();
+        } else {
+            for (int k = 1; k <= s; ++k) {
+                const double cb = A.co.bab[2 * k - 2], ca = A.co.bab[2 * k - 1];
+                const VVi *col = A.table + (size_t)(s * (l - 1) + k - 1) * A.nv;
+                if (k == 1) {
+                    if (xid) {
+                        gather_m<NP>(S, col, N);
+                        __syncthreads();
+                        for (int e = tid; e < N * N; e += nt) {
+                            const int i = e % N, j = e / N;
+                            const double pr = S.pr[i + j * LD] + (cb * dz) * S.mr[i * LD + j];
+                            const double pi = S.pi[i + j * LD] + (cb * dz) * S.mi[i * LD + j];
+                            S.pr[i + j * LD] = pr;
+                            S.pi[i + j * LD] = pi;
+                            S.qr[i + j * LD] = (ca * dz) * pr + (i == j ? 1.0 : 0.0);
+                            S.qi[i + j * LD] = (ca * dz) * pi;
+                        }
+                    } else {
+                        for (int e = tid; e < N * N; e += nt) {
+                            const int i = e % N, j = e / N;
+                            S.qr[i + j * LD] += (ca * dz) * S.pr[i + j * LD];
+                            S.qi[i + j * LD] += (ca * dz) * S.pi[i + j * LD];
+                        }
+                    }
+                    __syncthreads();
+                } else {
+                    gather_m<NP>(S, col, N);
+                    __syncthreads();
+                    gemm_acc<NP>(S, N, cb * dz);
+                    __syncthreads();
+                    for (int e = tid; e < N * N; e += nt) {
+                        const int i = e % N, j = e / N;
+                        S.qr[i + j * LD] += (ca * dz) * S.pr[i + j * LD];
+                        S.qi[i + j * LD] += (ca * dz) * S.pi[i + j * LD];
+                    }
+                    __syncthreads();
+                }
+            }
+        }
+        // cond_est of Q (Gershgorin, matcomp.f90:498-528)
+        for (int j = tid; j < N; j += nt) {
+            double t = 0.0;
+            for (int i = 0; i < N; ++i)
+                if (i != j) {
+                    const double re = S.qr[i + j * LD], im = S.qi[i + j * LD];
+                    t += sqrt(re * re + im * im);
+                }
+            const double dj = hypot(S.qr[j + j * LD], S.qi[j + j * LD]);
+            S.red0[j] = dj - t;
+            S.red1[j] = dj + t;
+        }
+        if (A.method != TRHEPD_LF) gather_m<NP>(S, A.table + (size_t)(s * l) * A.nv, N);
+        __syncthreads();
+        double mn = hypot(S.qr[0], S.qi[0]), mx = mn;
+        for (int j = 0; j < N; ++j) {
+            mn = fmin(mn, S.red0[j]);
+            mx = fmax(mx, S.red1[j]);
+        }
+        const double d = (mn < 0.0) ? 1e100 : mx / mn;
+        const bool renorm = (d > 1e3) || (l == A.ns);
+        if (A.method != TRHEPD_LF) {
+            gemm_acc<NP>(S, N, (renorm ? cb1 : 2.0 * cb1) * dz);
+            __syncthreads();
+        }
+        if (renorm) {
+            lu_solve<NP>(S, N);
+            xid = 1;
+            ++nren;
+        } else {
+            xid = 0;
+        }
+    }
+    // stransh: X = conj(G) - iY, Y' = conj(G) + iY, Y'' = X^{-1} Y', F = Y''^H
+    for (int e = tid; e < N * N; e += nt) {
+        const int i = e % N, j = e / N;
+        const double yr = S.pr[i + j * LD], yi = S.pi[i + j * LD];
+        const double dr = (i == j) ? S.gcr[i] : 0.0, di = (i == j) ? S.gci[i] : 0.0;
+        S.qr[i + j * LD] = dr + yi;     // conj(g) + (0,-1) y
+        S.qi[i + j * LD] = di - yr;
+        S.pr[i + j * LD] = dr - yi;     // conj(g) + (0, 1) y
+        S.pi[i + j * LD] = di + yr;
+    }
+    __syncthreads();
+    lu_solve<NP>(S, N);
+    double2 *Fo = A.f + (size_t)pt * N * N;
+    for (int e = tid; e < N * N; e += nt) {
+        const int i = e % N, j = e / N;
+        Fo[i + j * N] = make_double2(S.pr[j + i * LD], -S.pi[j + i * LD]);
+    }
+    if (A.f2) {
+        const int c0 = A.nb0 - 1;
+        const double w = A.wnsga2[pt];
+        for (int i = tid; i < N; i += nt) {
+            const double sr = S.gcr[i];   // Re(gamma_i)
+            const double fr = S.pr[c0 + i * LD], fi = -S.pi[c0 + i * LD];
+            A.f2[(size_t)pt * N + i] = (sr > 1e-10) ? (fr * fr + fi * fi) * w / sr : 0.0;
+        }
+    }
+    if (tid == 0) {
+        atomicAdd(&A.counters[0], nren);
+        if (S.flag) atomicOr(&A.counters[1], 1);
+    }
+}
+
+// hp_int_spa1d (surf_prkn.f90:485-524): the scalar N = 1 form used by int_method 6.
+__global__ void k_spa1d(SmallArgs A, int npts)
+{
+    const int pt = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pt >= npts) return;
+    const int s = A.co.s;
+    const double dz = A.dz;
+    const double2 g = A.gamma[pt];
+    const double2 gc = make_double2(g.x, -g.y);
+    const double2 g2c = make_double2(g.x * g.x - g.y * g.y, -(2.0 * g.x * g.y));
+    double2 f = A.f[pt];
+    // p = i (1 - conj f) / (1 + conj f) * conj g
+    const double2 num = make_double2(1.0 - f.x, f.y), den = make_double2(1.0 + f.x, -f.y);
+    double2 r = cdiv(num.x, num.y, den.x, den.y);
+    r = make_double2(-r.y, r.x);
+    double2 p = make_double2(r.x * gc.x - r.y * gc.y, r.x * gc.y + r.y * gc.x);
+    double2 q = make_double2(1.0, 0.0);
+    for (int l = 1; l <= A.ns; ++l) {
+        for (int k = 1; k <= s; ++k) {
+            const double cb = (l != 1 && k == 1) ? 2 * A.co.bab[0] : A.co.bab[2 * k - 2];
+            const double ca = A.co.bab[2 * k - 1];
+            const VVi t = A.table[(size_t)(s * (l - 1) + k - 1) * A.nv];
+            const double2 m = make_double2(-(t.v.x + t.vi.x) - g2c.x, (t.v.y + t.vi.y) - g2c.y);
+            const double2 mq = make_double2(m.x * q.x - m.y * q.y, m.x * q.y + m.y * q.x);
+            p = make_double2(p.x + cb * dz * mq.x, p.y + cb * dz * mq.y);
+            q = make_double2(q.x + ca * dz * p.x, q.y + ca * dz * p.y);
+        }
+        p = cdiv(p.x, p.y, q.x, q.y);
+        q = make_double2(1.0, 0.0);
+    }
+    const VVi t = A.table[(size_t)(s * A.ns) * A.nv];
+    const double2 m = make_double2(-(t.v.x + t.vi.x) - g2c.x, (t.v.y + t.vi.y) - g2c.y);
+    p = make_double2(p.x + A.co.bab[0] * dz * (m.x * q.x - m.y * q.y), p.y + A.co.bab[0] * dz * (m.x * q.y + m.y * q.x));
+    // f = (conj g + i p) / (conj g - i p)
+    const double2 a = make_double2(gc.x - p.y, gc.y + p.x), b = make_double2(gc.x + p.y, gc.y - p.x);
+    f = cdiv(a.x, a.y, b.x, b.y);
+    A.f[pt] = f;
+    if (A.f2) {
+        const double sr = g.x;
+        A.f2[pt] = (sr > 1e-10) ? (f.x * f.x + f.y * f.y) * A.wnsga2[pt] / sr : 0.0;
+    }
+}
+
+}  // namespace
+
+int run_small(trhepd_ctx *ctx, const Table &t, int nb, int nang, const int *d_gmap,
+              const double2 *d_gamma, double2 *d_f, double *d_f2, int nb0, const double *d_wnsga2,
+              int *d_counters)
+{
+    SmallArgs A;
+    A.table = t.d;
+    A.nv = t.nv;
+    A.ns = t.ns;
+    A.N = nb;
+    A.method = t.method;
+    A.dz = t.dz;
+    A.co = rkn_coef(t.method);
+    A.gmap = d_gmap;
+    A.gamma = d_gamma;
+    A.f = d_f;
+    A.f2 = d_f2;
+    A.nb0 = nb0;
+    A.wnsga2 = d_wnsga2;
+    A.counters = d_counters;
+    if (nb == 1 && t.method == TRHEPD_SP6) {
+        hipLaunchKernelGGL(k_spa1d, dim3((nang + 63) / 64), dim3(64), 0, ctx->stream, A, nang);
+    } else if (nb <= 16) {
+        hipLaunchKernelGGL(k_small<16>, dim3(nang), dim3(256), 0, ctx->stream, A);
+    } else if (nb <= 32) {
+        hipLaunchKernelGGL(k_small<32>, dim3(nang), dim3(256), 0, ctx->stream, A);
+    } else if (nb <= 48) {
+        hipLaunchKernelGGL(k_small<48>, dim3(nang), dim3(256), 0, ctx->stream, A);
+    } else {
+        return fail(ctx, TRHEPD_E_ARG, "small path supports N <= 48 (got %d)", nb);
+    }
+    TRY_HIP(ctx, hipGetLastError());
+    return TRHEPD_OK;
+}
+
+}  // namespace trhepd

@@ -1,0 +1,241 @@
+"""ctypes binding of libtrhepd_hip.so (the C ABI declared in include/trhepd.h).
+
+The library is the product: every numerical stage runs in its HIP kernels.  This module
+only marshals numpy arrays (Fortran order, complex128, 1-based index maps) across the
+ABI and turns non-zero status codes into exceptions.  If the shared library is missing
+it raises -- there is no Python fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Tuple
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)                      # trhepd-opt_amd/
+LIB_PATH = os.path.join(ROOT, "lib", "libtrhepd_hip.so")
+
+OK, E_ARG, E_HIP, E_NOMEM, E_BUDGET, E_SINGULAR, E_NONFINITE, E_NOTFOUND = range(8)
+M2, M4, M6, LF, SP4, SP6, RK4, RK4GS = range(1, 9)
+PREC_FP64, PREC_MIXED = 0, 1
+
+# exported symbols (must match include/trhepd.h)
+SYMBOLS = ("trhepd_ctx_create", "trhepd_ctx_destroy", "trhepd_last_error", "trhepd_ctx_bytes",
+           "trhepd_asfcrr", "trhepd_srfghk", "trhepd_asfcef", "trhepd_scpot_stf",
+           "trhepd_table_build", "trhepd_table_shape", "trhepd_table_fetch", "trhepd_table_release",
+           "trhepd_propagate_batch", "trhepd_last_stats")
+
+
+class TrhepdError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__("trhepd status %d: %s" % (code, msg))
+        self.code = code
+
+
+class _Opts(C.Structure):
+    _fields_ = [("device", C.c_int), ("precision", C.c_int), ("hbm_budget", C.c_size_t)]
+
+
+class Potential(C.Structure):
+    """trhepd_potential: the mod_scpot3 globals (surfio.f90:18-27)."""
+    _fields_ = [("nv", C.c_int), ("nvb", C.c_int), ("natms", C.c_int), ("natmb", C.c_int),
+                ("nelm", C.c_int), ("nabr", C.c_int), ("nabi", C.c_int),
+                ("cc", C.c_double), ("smesh", C.c_double),
+                ("ielm", C.c_void_p), ("z", C.c_void_p), ("dchg", C.c_void_p), ("ghk", C.c_void_p),
+                ("ocr", C.c_void_p), ("b", C.c_void_p), ("asf", C.c_void_p),
+                ("st0", C.c_void_p), ("st1", C.c_void_p), ("st2", C.c_void_p)]
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise TrhepdError(E_HIP, "%s is not built (run __graft_entry__.build() or make -C trhepd-opt_amd)" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    vp, i, d, sz = C.c_void_p, C.c_int, C.c_double, C.c_size_t
+    ip, dp = C.POINTER(C.c_int), C.POINTER(C.c_double)
+    L.trhepd_ctx_create.argtypes = [C.POINTER(_Opts), C.POINTER(vp)]
+    L.trhepd_ctx_destroy.argtypes = [vp]
+    L.trhepd_last_error.argtypes = [vp]
+    L.trhepd_last_error.restype = C.c_char_p
+    L.trhepd_ctx_bytes.argtypes = [vp]
+    L.trhepd_ctx_bytes.restype = sz
+    L.trhepd_asfcrr.argtypes = [d, dp, i, i, i, vp, vp, vp, vp, vp, vp, vp, vp, d, d]
+    L.trhepd_srfghk.argtypes = [i, i, vp, vp, i, vp, vp, vp, ip]
+    L.trhepd_asfcef.argtypes = [vp, i, i, vp, vp, d, d, d, i, i, vp, vp, vp, vp, vp, vp]
+    L.trhepd_scpot_stf.argtypes = [vp, i, i, i, i, i, i, i, vp, vp, vp, vp, d, d, vp]
+    L.trhepd_table_build.argtypes = [vp, C.POINTER(Potential), d, i, i, ip]
+    L.trhepd_table_shape.argtypes = [vp, i, ip, ip]
+    L.trhepd_table_fetch.argtypes = [vp, i, vp, vp]
+    L.trhepd_table_release.argtypes = [vp, i]
+    L.trhepd_propagate_batch.argtypes = [vp, i, i, i, vp, vp, vp, vp, i, vp]
+    L.trhepd_last_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), dp]
+    for name in SYMBOLS:
+        getattr(L, name).restype = getattr(L, name).restype or C.c_int
+    L.trhepd_last_error.restype = C.c_char_p
+    L.trhepd_ctx_bytes.restype = sz
+    _lib = L
+    return L
+
+
+def _p(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a) -> np.ndarray:
+    return np.require(np.asarray(a, dtype=np.float64), requirements=["C", "A", "W"]).copy()
+
+
+def asfcrr(be: float, nelm: int, nabr: int, nabi: int, a, b, c, dchg, dth, dtk, dtz, elm, tem: float,
+           smesh: float):
+    """asfcrr (scpot.f90:11-58) on (nelm, 6)-shaped C arrays == Fortran (6, nelm); returns wn
+    and the updated arrays."""
+    L = load()
+    a, b, c, dchg, dth, dtk = (_f64(x) for x in (a, b, c, dchg, dth, dtk))
+    dtz, elm = _f64(dtz), _f64(elm)
+    wn = C.c_double(0.0)
+    rc = L.trhepd_asfcrr(be, C.byref(wn), nelm, nabr, nabi, _p(a), _p(b), _p(c), _p(dchg), _p(dth), _p(dtk),
+                         _p(dtz), _p(elm), tem, smesh)
+    if rc:
+        raise TrhepdError(rc, "asfcrr: bad arguments")
+    return wn.value, a, b, c, dchg, dth, dtk
+
+
+def srfghk(nvm: int, igh_b, igk_b, ih, ik) -> Tuple[int, np.ndarray, np.ndarray, np.ndarray]:
+    """srfghk (surfsub.f90:9-41): returns nv, igh, igk (length nv) and iv (nb, nb) Fortran order."""
+    L = load()
+    nvb = len(igh_b)
+    nb = len(ih)
+    igh = np.zeros(max(nvm, nvb), dtype=np.int32)
+    igk = np.zeros(max(nvm, nvb), dtype=np.int32)
+    igh[:nvb], igk[:nvb] = igh_b, igk_b
+    ihh = np.ascontiguousarray(ih, dtype=np.int32)
+    ikk = np.ascontiguousarray(ik, dtype=np.int32)
+    iv = np.zeros((nb, nb), dtype=np.int32, order="F")
+    nv = C.c_int(0)
+    rc = L.trhepd_srfghk(nvm, nvb, _p(igh), _p(igk), nb, _p(ihh), _p(ikk), _p(iv), C.byref(nv))
+    if rc:
+        raise TrhepdError(rc, "srfghk: nv exceeds nvm or bad arguments")
+    return nv.value, igh[:nv.value].copy(), igk[:nv.value].copy(), iv
+
+
+class Engine:
+    """A device context (trhepd_ctx): owns HBM buffers, a HIP stream and the tables."""
+
+    def __init__(self, device: int = 0, precision: int = PREC_FP64, hbm_budget: int = 0):
+        L = load()
+        self._L = L
+        self._h = C.c_void_p()
+        rc = L.trhepd_ctx_create(C.byref(_Opts(device, precision, hbm_budget)), C.byref(self._h))
+        if rc:
+            raise TrhepdError(rc, "ctx_create failed (device %d)" % device)
+        self.precision = precision
+
+    def close(self):
+        if self._h:
+            self._L.trhepd_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc: int):
+        if rc:
+            raise TrhepdError(rc, self._L.trhepd_last_error(self._h).decode())
+
+    @property
+    def bytes_held(self) -> int:
+        return int(self._L.trhepd_ctx_bytes(self._h))
+
+    # -- per-structure setup -------------------------------------------------
+    def asfcef(self, igh, igk, ghx, ghy, gky, nabr, nabi, a, c, dth, dtk, nelm):
+        nv = len(igh)
+        igh = np.ascontiguousarray(igh, dtype=np.int32)
+        igk = np.ascontiguousarray(igk, dtype=np.int32)
+        a, c, dth, dtk = (np.ascontiguousarray(x, dtype=np.float64) for x in (a, c, dth, dtk))
+        asf = np.zeros(nv * 6 * nelm)
+        ghk = np.zeros(nv)
+        self._chk(self._L.trhepd_asfcef(self._h, nv, nelm, _p(igh), _p(igk), ghx, ghy, gky, nabr, nabi,
+                                        _p(a), _p(c), _p(dth), _p(dtk), _p(asf), _p(ghk)))
+        return asf, ghk
+
+    def scpot_stf(self, nsg, ma, mb, na, nb, gh, gk, x, y, dx, dy, nv=None):
+        natm = len(x)
+        gh = np.ascontiguousarray(gh, dtype=np.float64)
+        gk = np.ascontiguousarray(gk, dtype=np.float64)
+        nv = len(gh) if nv is None else nv
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        st = np.zeros((natm, nv), dtype=np.complex128)   # == Fortran (nv, natm)
+        if natm:
+            self._chk(self._L.trhepd_scpot_stf(self._h, natm, nv, nsg, ma, mb, na, nb, _p(gh), _p(gk),
+                                               _p(x), _p(y), dx, dy, _p(st)))
+        return st
+
+    # -- table ---------------------------------------------------------------
+    def table_build(self, pot: dict, dz: float, ns: int, method: int) -> int:
+        keep = {}
+
+        def arr(k, dt):
+            keep[k] = np.ascontiguousarray(pot[k], dtype=dt)
+            return _p(keep[k])
+
+        P = Potential(int(pot["nv"]), int(pot["nvb"]), int(pot["natms"]), int(pot["natmb"]), int(pot["nelm"]),
+                      int(pot["nabr"]), int(pot["nabi"]), float(pot["cc"]), float(pot["smesh"]),
+                      arr("ielm", np.int32), arr("z", np.float64), arr("dchg", np.float64),
+                      arr("ghk", np.float64), arr("ocr", np.float64), arr("b", np.float64),
+                      arr("asf", np.float64), arr("st0", np.complex128), arr("st1", np.complex128),
+                      arr("st2", np.complex128))
+        tid = C.c_int(0)
+        self._chk(self._L.trhepd_table_build(self._h, C.byref(P), dz, ns, method, C.byref(tid)))
+        return tid.value
+
+    def table_shape(self, tid: int):
+        nv, nc = C.c_int(), C.c_int()
+        self._chk(self._L.trhepd_table_shape(self._h, tid, C.byref(nv), C.byref(nc)))
+        return nv.value, nc.value
+
+    def table_fetch(self, tid: int):
+        nv, nc = self.table_shape(tid)
+        v = np.zeros((nc, nv), dtype=np.complex128)
+        vi = np.zeros((nc, nv), dtype=np.complex128)
+        self._chk(self._L.trhepd_table_fetch(self._h, tid, _p(v), _p(vi)))
+        return v, vi
+
+    def table_release(self, tid: int):
+        self._chk(self._L.trhepd_table_release(self._h, tid))
+
+    # -- slice loop ----------------------------------------------------------
+    def propagate_batch(self, tid: int, iv: np.ndarray, gamma: np.ndarray, f: np.ndarray, nb0: int,
+                        wnsga2: Optional[np.ndarray] = None):
+        """gamma (nang, nb) complex; f (nang, nb, nb) complex with f[a][i, j] = F_a(i, j).
+        Returns (f_out, f2): f_out in the same indexing, f2 (nang, nb) or None."""
+        nang, nb = gamma.shape
+        ivf = np.asfortranarray(iv, dtype=np.int32)
+        g = np.ascontiguousarray(gamma, dtype=np.complex128)
+        ff = np.ascontiguousarray(np.asarray(f, dtype=np.complex128).transpose(0, 2, 1))
+        f2 = np.zeros((nang, nb)) if wnsga2 is not None else None
+        w = None if wnsga2 is None else np.ascontiguousarray(wnsga2, dtype=np.float64)
+        self._chk(self._L.trhepd_propagate_batch(self._h, tid, nb, nang, _p(ivf), _p(g), _p(ff), _p(f2), nb0,
+                                                 _p(w)))
+        return ff.transpose(0, 2, 1), f2
+
+    def last_stats(self):
+        r, p, ms = C.c_longlong(), C.c_longlong(), C.c_double()
+        self._chk(self._L.trhepd_last_stats(self._h, C.byref(r), C.byref(p), C.byref(ms)))
+        return {"n_renorm": r.value, "n_products": p.value, "device_ms": ms.value}
