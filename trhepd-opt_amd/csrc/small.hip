@@ -114,7 +114,7 @@ __device__ void gemm_acc(SLds<NP> &S, int N, double alpha)
         const int j = bj * 16 + l16;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int i = bi * 16 + 4 * kq + r;
+            const int i = bi * 16 + kq + 4 * r;
             if (i < N && j < N) {
                 S.pr[i + j * LD] += alpha * cr[r];
                 S.pi[i + j * LD] += alpha * ci[r];
