@@ -1,0 +1,134 @@
+"""Rocking-curve throughput of the slice loop on one MI355X.
+
+Workload (the headline configuration): a seeded synthetic Si-surface structure with N = 521
+beams, 12 surface atoms of Z = 14, 8, 1, 40 glancing angles 0.5-4.4 deg, sp4 (int_method 5)
+at dz = 0.02 A.  One step = one full rocking curve of that structure = one batch_run over
+the 40 angles with inputs (bulk reflection matrices, gamma, potential table) resident in
+HBM.  value = points (angles) per second.
+
+Also reported: the dominant kernel's roofline (stage-product GEMM, FP64 matrix peak
+78.6 TFLOP/s; algorithmic flops per launch = 8 N^3 x angles) measured with HIP events on
+the library's stream during the timed steps, and the CPU oracle timed on a bounded sample
+of the same workload on this host.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "trhepd-opt_amd"))
+
+METRIC = "rocking-curve points/s at N=521 beams, mixed vs FP64 GPU @1e-6 rel; p99 lat"
+PEAK_FP64 = 78.6   # TFLOP/s, MI355X dense FP64 matrix
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(st, blocks, method, dz2, threads, sample_slices):
+    """The oracle on a bounded sample: `sample_slices` slices of one angle of the same
+    deck (same N, same per-slice cost), scaled to a full point."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    import numpy as np
+    import oracle as O
+    bp = st.bp
+    bp.domains[0].blocks = blocks
+    surf = O.setup_surface(bp, st.surf, dz2, method)
+    ns_full = surf.ns
+    surf.ns = sample_slices
+    v, vi = O.table(surf)
+    r1, r2, ga, az = surf.angles()[0]
+    g = O.gamma(surf, ga, az)
+    f = surf.scatter_f(r1, r2)
+    t0 = time.perf_counter()
+    O.integrate(surf, f, g, v, vi)
+    dt = time.perf_counter() - t0
+    per_point = dt * ns_full / sample_slices
+    return {"value": 1.0 / per_point, "unit": "points/s", "cores": threads, "kind": "port",
+            "sample": "oracle (C, OpenMP %d threads), 1 angle x %d of %d slices of the same N=521 deck, "
+                      "%.1f s measured, scaled by %d/%d slices" % (threads, sample_slices, ns_full, dt, ns_full,
+                                                                   sample_slices)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-slices", type=int, default=8)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the slice loop shards by structure and angle with no "
+              "exchange, so the multi-GPU figure is N x the one-GPU value; run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from trhepd import driver, lib, synth
+
+    method, dz2 = lib.SP4, 0.02
+    st = synth.config(3, seed=args.seed)
+    eng = lib.Engine(0, precision=lib.PREC_FP64)
+    t0 = time.perf_counter()
+    blocks = synth.bulk_reflection(eng, st, layers=2, dz2=dz2, method=method)
+    log("bulk reflection (2 bulk layers, device): %.1f s" % (time.perf_counter() - t0))
+    t0 = time.perf_counter()
+    job = driver.SurfaceJob(eng, st.bp, st.surf, dz2, method)
+    G, F, W, A = job.inputs(0, blocks)
+    nang, N = G.shape
+    bid = eng.batch_create(job.table_id, job.iv, G, F, job.nb0, W)
+    log("setup: N=%d nv=%d ns=%d dz=%.5f angles=%d (%.1f s)" % (N, job.nv, job.ns, job.dz, nang,
+                                                               time.perf_counter() - t0))
+    for _ in range(args.warmup):
+        eng.batch_run(bid)
+        log("warmup step: device %.1f ms, renorm %d" % (eng.last_stats()["device_ms"], eng.last_stats()["n_renorm"]))
+    eng.profile(True)
+    t0 = time.perf_counter()
+    ren = 0
+    for k in range(args.steps):
+        eng.batch_run(bid)
+        ren += eng.last_stats()["n_renorm"]
+        log("step %d done" % k)
+    elapsed = time.perf_counter() - t0
+    launches, kms = eng.kernel_stats()
+    eng.profile(False)
+    _, f2 = eng.batch_fetch(bid, want_f=False)
+    assert np.all(np.isfinite(f2))
+    value = nang * args.steps / elapsed
+    flops_launch = 8.0 * N ** 3 * nang
+    avg_ms = kms / max(launches, 1)
+    achieved = flops_launch / (avg_ms * 1e-3) / 1e12
+    co = 6
+    out = {
+        "metric": METRIC, "value": value, "unit": "points/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "synthetic Si surface, N=%d beams, %d angles 0.5-4.4 deg, 12 surface atoms "
+                               "(Z=14,8,1), sp4 dz=%.4f ns=%d, FP64 path" % (N, nang, job.dz, job.ns),
+                   "path": "fp64", "angles": nang, "beams": N, "slices": job.ns,
+                   "renorm_per_point": ren / (nang * args.steps)},
+        "roofline": {"bound": "mfma", "kernel": "k_stage_gemm (stage product, 8 N^3 flops per angle)",
+                     "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
+                     "traffic": None, "launches": launches, "avg_launch_ms": avg_ms},
+    }
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(st, blocks, method, dz2, args.cpu_threads, args.cpu_slices)
+        except Exception as e:   # the CPU leg is reported, never fatal to the GPU figure
+            out["cpu_baseline"] = {"value": None, "error": repr(e)}
+    eng.batch_release(bid)
+    job.release()
+    eng.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -131,10 +131,28 @@ int trhepd_propagate_batch(trhepd_ctx *ctx, int table_id, int nb, int nang, cons
                            const double *gamma, double *f, double *f2, int nb0,
                            const double *wnsga2);
 
-/* statistics of the last propagate call: renormalisations (LU solves) summed over the
- * batch, stage products executed, and the device time in milliseconds. */
+/* The same slice loop with device-resident inputs, for callers that run a batch more
+ * than once or keep several batches on the device (serving).  batch_create uploads the
+ * inputs (same arguments as trhepd_propagate_batch); batch_run integrates all angles,
+ * leaving F and f2 on the device (the uploaded F is kept, so a batch can be re-run);
+ * batch_fetch copies the results out (either pointer may be NULL). */
+int trhepd_batch_create(trhepd_ctx *ctx, int table_id, int nb, int nang, const int *iv,
+                        const double *gamma, const double *f, int nb0, const double *wnsga2,
+                        int *batch_id);
+int trhepd_batch_run(trhepd_ctx *ctx, int batch_id);
+int trhepd_batch_fetch(trhepd_ctx *ctx, int batch_id, double *f, double *f2);
+int trhepd_batch_release(trhepd_ctx *ctx, int batch_id);
+
+/* statistics of the last propagate / batch_run call: renormalisations (LU solves) summed
+ * over the batch, stage products executed, and the device time in milliseconds. */
 int trhepd_last_stats(const trhepd_ctx *ctx, long long *n_renorm, long long *n_products,
                       double *device_ms);
+
+/* Kernel timing: when enabled, every stage-product kernel launch is bracketed by HIP
+ * events on the context's stream; trhepd_kernel_stats returns the number of such
+ * launches and their summed duration (ms) since the last enable. */
+int trhepd_profile(trhepd_ctx *ctx, int enable);
+int trhepd_kernel_stats(trhepd_ctx *ctx, long long *launches, double *total_ms);
 
 #ifdef __cplusplus
 }

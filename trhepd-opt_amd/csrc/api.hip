@@ -72,6 +72,32 @@ int scratch(trhepd_ctx *ctx, int slot, size_t bytes, void **p)
     return TRHEPD_OK;
 }
 
+int prof_mark(trhepd_ctx *ctx)
+{
+    Prof &P = ctx->prof;
+    if (!P.on) return TRHEPD_OK;
+    if (P.used == (int)P.ev.size()) {
+        hipEvent_t e;
+        TRY_HIP(ctx, hipEventCreate(&e));
+        P.ev.push_back(e);
+    }
+    TRY_HIP(ctx, hipEventRecord(P.ev[P.used++], ctx->stream));
+    return TRHEPD_OK;
+}
+
+int prof_collect(trhepd_ctx *ctx)
+{
+    Prof &P = ctx->prof;
+    for (int i = 0; i + 1 < P.used; i += 2) {
+        float ms = 0;
+        TRY_HIP(ctx, hipEventElapsedTime(&ms, P.ev[i], P.ev[i + 1]));
+        P.ms += ms;
+        P.launches++;
+    }
+    P.used = 0;
+    return TRHEPD_OK;
+}
+
 Table *find_table(trhepd_ctx *ctx, int id)
 {
     for (Table &t : ctx->tables)
@@ -156,6 +182,8 @@ int trhepd_ctx_destroy(trhepd_ctx *ctx)
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
     for (Table &t : ctx->tables) dev_free(ctx, t.d, t.bytes);
+    for (Batch &b : ctx->batches) dev_free(ctx, b.d, b.bytes);
+    for (hipEvent_t e : ctx->prof.ev) (void)hipEventDestroy(e);
     for (DevBuf &b : ctx->bufs) dev_free(ctx, b.p, b.bytes);
     hipEventDestroy(ctx->ev0);
     hipEventDestroy(ctx->ev1);
@@ -323,18 +351,25 @@ int trhepd_table_release(trhepd_ctx *ctx, int table_id)
     return fail(ctx, TRHEPD_E_NOTFOUND, "no table %d", table_id);
 }
 
-int trhepd_propagate_batch(trhepd_ctx *ctx, int table_id, int nb, int nang, const int *iv,
-                           const double *gamma, double *f, double *f2, int nb0, const double *wnsga2)
+static Batch *find_batch(trhepd_ctx *ctx, int id)
+{
+    for (Batch &b : ctx->batches)
+        if (b.id == id) return &b;
+    return nullptr;
+}
+
+int trhepd_batch_create(trhepd_ctx *ctx, int table_id, int nb, int nang, const int *iv, const double *gamma,
+                        const double *f, int nb0, const double *wnsga2, int *batch_id)
 {
     if (!ctx) return TRHEPD_E_ARG;
     Table *t = find_table(ctx, table_id);
     if (!t) return fail(ctx, TRHEPD_E_NOTFOUND, "no table %d", table_id);
-    if (nb < 1 || nang < 0 || !iv || !gamma || !f || nb0 < 1 || nb0 > nb || (f2 && !wnsga2))
-        return fail(ctx, TRHEPD_E_ARG, "propagate_batch: bad arguments");
+    if (nb < 1 || nang < 1 || !iv || !gamma || !f || nb0 < 1 || nb0 > nb || !batch_id)
+        return fail(ctx, TRHEPD_E_ARG, "batch_create: bad arguments");
     if (t->method != TRHEPD_SP4 && t->method != TRHEPD_SP6 && t->method != TRHEPD_LF)
         return fail(ctx, TRHEPD_E_ARG, "int_method %d is not available on the device path", t->method);
-    ctx->stats = Stats();
-    if (nang == 0) return TRHEPD_OK;
+    if (nb > 48 && t->method == TRHEPD_LF)
+        return fail(ctx, TRHEPD_E_ARG, "int_method 4 runs on the device for N <= 48 only (N = %d)", nb);
     // signed gather map (row-major): upper i<j -> +iv(j,i), lower i>j -> -iv(i,j), diagonal 0
     std::vector<int> gm((size_t)nb * nb);
     for (int i = 0; i < nb; ++i)
@@ -349,41 +384,143 @@ int trhepd_propagate_batch(trhepd_ctx *ctx, int table_id, int nb, int nang, cons
         }
     TRY_HIP(ctx, hipSetDevice(ctx->device));
     const size_t nn = (size_t)nb * nb;
-    void *p_gm, *p_g, *p_f, *p_f2, *p_w, *p_cnt;
-    int rc;
-    if ((rc = scratch(ctx, 0, nn * 4, &p_gm)) || (rc = scratch(ctx, 1, (size_t)nb * nang * 16, &p_g)) ||
-        (rc = scratch(ctx, 2, nn * nang * 16, &p_f)) || (rc = scratch(ctx, 3, (size_t)nb * nang * 8, &p_f2)) ||
-        (rc = scratch(ctx, 4, (size_t)nang * 8, &p_w)) || (rc = scratch(ctx, 5, 64, &p_cnt)))
-        return rc;
-    TRY_HIP(ctx, hipMemcpyAsync(p_gm, gm.data(), nn * 4, hipMemcpyHostToDevice, ctx->stream));
-    TRY_HIP(ctx, hipMemcpyAsync(p_g, gamma, (size_t)nb * nang * 16, hipMemcpyHostToDevice, ctx->stream));
-    TRY_HIP(ctx, hipMemcpyAsync(p_f, f, nn * nang * 16, hipMemcpyHostToDevice, ctx->stream));
-    if (f2) TRY_HIP(ctx, hipMemcpyAsync(p_w, wnsga2, (size_t)nang * 8, hipMemcpyHostToDevice, ctx->stream));
-    TRY_HIP(ctx, hipMemsetAsync(p_cnt, 0, 64, ctx->stream));
+    Batch b;
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~size_t(255); return o; };
+    const size_t o_gm = take(nn * 4), o_g = take((size_t)nb * nang * 16), o_fi = take(nn * nang * 16),
+                 o_fo = take(nn * nang * 16), o_f2 = take((size_t)nb * nang * 8), o_w = take((size_t)nang * 8),
+                 o_c = take(64);
+    int rc = dev_alloc(ctx, (void **)&b.d, off);
+    if (rc) return rc;
+    b.bytes = off;
+    b.gmap = (int *)(b.d + o_gm);
+    b.gamma = (double2 *)(b.d + o_g);
+    b.fin = (double2 *)(b.d + o_fi);
+    b.fout = (double2 *)(b.d + o_fo);
+    b.f2 = (double *)(b.d + o_f2);
+    b.w = (double *)(b.d + o_w);
+    b.cnt = (int *)(b.d + o_c);
+    b.table_id = table_id;
+    b.nb = nb;
+    b.nang = nang;
+    b.nb0 = nb0;
+    b.has_f2 = wnsga2 != nullptr;
+    hipError_t e = hipMemcpyAsync(b.gmap, gm.data(), nn * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (!e) e = hipMemcpyAsync(b.gamma, gamma, (size_t)nb * nang * 16, hipMemcpyHostToDevice, ctx->stream);
+    if (!e) e = hipMemcpyAsync(b.fin, f, nn * nang * 16, hipMemcpyHostToDevice, ctx->stream);
+    if (!e && wnsga2) e = hipMemcpyAsync(b.w, wnsga2, (size_t)nang * 8, hipMemcpyHostToDevice, ctx->stream);
+    if (!e) e = hipStreamSynchronize(ctx->stream);   // gm is a host temporary
+    if (e) {
+        dev_free(ctx, b.d, b.bytes);
+        return hip_check(ctx, e, "batch upload");
+    }
+    b.id = ctx->next_batch++;
+    ctx->batches.push_back(b);
+    *batch_id = b.id;
+    return TRHEPD_OK;
+}
+
+int trhepd_batch_run(trhepd_ctx *ctx, int batch_id)
+{
+    if (!ctx) return TRHEPD_E_ARG;
+    Batch *b = find_batch(ctx, batch_id);
+    if (!b) return fail(ctx, TRHEPD_E_NOTFOUND, "no batch %d", batch_id);
+    Table *t = find_table(ctx, b->table_id);
+    if (!t) return fail(ctx, TRHEPD_E_NOTFOUND, "table %d of batch %d was released", b->table_id, batch_id);
+    TRY_HIP(ctx, hipSetDevice(ctx->device));
+    ctx->stats = Stats();
+    const size_t nn = (size_t)b->nb * b->nb;
+    TRY_HIP(ctx, hipMemsetAsync(b->cnt, 0, 64, ctx->stream));
     TRY_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    if (nb <= 48)
-        rc = run_small(ctx, *t, nb, nang, (int *)p_gm, (double2 *)p_g, (double2 *)p_f, f2 ? (double *)p_f2 : nullptr,
-                       nb0, (double *)p_w, (int *)p_cnt);
+    TRY_HIP(ctx, hipMemcpyAsync(b->fout, b->fin, nn * b->nang * 16, hipMemcpyDeviceToDevice, ctx->stream));
+    int rc;
+    double *f2 = b->has_f2 ? b->f2 : nullptr;
+    if (b->nb <= 48)
+        rc = run_small(ctx, *t, b->nb, b->nang, b->gmap, b->gamma, b->fout, f2, b->nb0, b->w, b->cnt);
     else
-        rc = run_batched(ctx, *t, nb, nang, (int *)p_gm, (double2 *)p_g, (double2 *)p_f,
-                         f2 ? (double *)p_f2 : nullptr, nb0, (double *)p_w, (int *)p_cnt);
+        rc = run_batched(ctx, *t, b->nb, b->nang, b->gmap, b->gamma, b->fout, f2, b->nb0, b->w, b->cnt);
     if (rc) return rc;
     TRY_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-    TRY_HIP(ctx, hipMemcpyAsync(f, p_f, nn * nang * 16, hipMemcpyDeviceToHost, ctx->stream));
-    if (f2) TRY_HIP(ctx, hipMemcpyAsync(f2, p_f2, (size_t)nb * nang * 8, hipMemcpyDeviceToHost, ctx->stream));
     int cnt[16];
-    TRY_HIP(ctx, hipMemcpyAsync(cnt, p_cnt, 64, hipMemcpyDeviceToHost, ctx->stream));
+    TRY_HIP(ctx, hipMemcpyAsync(cnt, b->cnt, 64, hipMemcpyDeviceToHost, ctx->stream));
     TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if ((rc = prof_collect(ctx))) return rc;
     float ms = 0;
     TRY_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     const RknCoef co = rkn_coef(t->method);
     ctx->stats.n_renorm = cnt[0];
-    ctx->stats.n_products = (long long)nang * t->ns * co.s;
+    ctx->stats.n_products = (long long)b->nang * t->ns * co.s;
     ctx->stats.device_ms = ms;
     if (cnt[1]) return fail(ctx, TRHEPD_E_SINGULAR, "exact zero pivot in a renormalisation LU");
+    return TRHEPD_OK;
+}
+
+int trhepd_batch_fetch(trhepd_ctx *ctx, int batch_id, double *f, double *f2)
+{
+    if (!ctx) return TRHEPD_E_ARG;
+    Batch *b = find_batch(ctx, batch_id);
+    if (!b) return fail(ctx, TRHEPD_E_NOTFOUND, "no batch %d", batch_id);
+    if (f2 && !b->has_f2) return fail(ctx, TRHEPD_E_ARG, "batch %d was created without wnsga2", batch_id);
+    TRY_HIP(ctx, hipSetDevice(ctx->device));
+    const size_t nn = (size_t)b->nb * b->nb;
+    if (f) TRY_HIP(ctx, hipMemcpyAsync(f, b->fout, nn * b->nang * 16, hipMemcpyDeviceToHost, ctx->stream));
+    if (f2) TRY_HIP(ctx, hipMemcpyAsync(f2, b->f2, (size_t)b->nb * b->nang * 8, hipMemcpyDeviceToHost, ctx->stream));
+    TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (f2)
-        for (size_t e = 0; e < (size_t)nb * nang; ++e)
+        for (size_t e = 0; e < (size_t)b->nb * b->nang; ++e)
             if (!std::isfinite(f2[e])) return fail(ctx, TRHEPD_E_NONFINITE, "non-finite intensity at %zu", e);
+    if (f)
+        for (size_t e = 0; e < 2 * nn * b->nang; ++e)
+            if (!std::isfinite(f[e])) return fail(ctx, TRHEPD_E_NONFINITE, "non-finite reflection at %zu", e / 2);
+    return TRHEPD_OK;
+}
+
+int trhepd_batch_release(trhepd_ctx *ctx, int batch_id)
+{
+    if (!ctx) return TRHEPD_E_ARG;
+    for (size_t i = 0; i < ctx->batches.size(); ++i)
+        if (ctx->batches[i].id == batch_id) {
+            TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            dev_free(ctx, ctx->batches[i].d, ctx->batches[i].bytes);
+            ctx->batches.erase(ctx->batches.begin() + i);
+            return TRHEPD_OK;
+        }
+    return fail(ctx, TRHEPD_E_NOTFOUND, "no batch %d", batch_id);
+}
+
+int trhepd_propagate_batch(trhepd_ctx *ctx, int table_id, int nb, int nang, const int *iv, const double *gamma,
+                           double *f, double *f2, int nb0, const double *wnsga2)
+{
+    if (!ctx) return TRHEPD_E_ARG;
+    if (f2 && !wnsga2) return fail(ctx, TRHEPD_E_ARG, "propagate_batch: f2 needs wnsga2");
+    if (nang == 0 && nb >= 1) {
+        ctx->stats = Stats();
+        return TRHEPD_OK;
+    }
+    int id = 0;
+    int rc = trhepd_batch_create(ctx, table_id, nb, nang, iv, gamma, f, nb0, f2 ? wnsga2 : nullptr, &id);
+    if (rc) return rc;
+    rc = trhepd_batch_run(ctx, id);
+    if (!rc) rc = trhepd_batch_fetch(ctx, id, f, f2);
+    const int rc2 = trhepd_batch_release(ctx, id);
+    return rc ? rc : rc2;
+}
+
+int trhepd_profile(trhepd_ctx *ctx, int enable)
+{
+    if (!ctx) return TRHEPD_E_ARG;
+    ctx->prof.on = enable != 0;
+    ctx->prof.used = 0;
+    ctx->prof.launches = 0;
+    ctx->prof.ms = 0.0;
+    return TRHEPD_OK;
+}
+
+int trhepd_kernel_stats(trhepd_ctx *ctx, long long *launches, double *total_ms)
+{
+    if (!ctx) return TRHEPD_E_ARG;
+    if (launches) *launches = ctx->prof.launches;
+    if (total_ms) *total_ms = ctx->prof.ms;
     return TRHEPD_OK;
 }
 

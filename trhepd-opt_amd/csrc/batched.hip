@@ -663,7 +663,9 @@ int run_batched(trhepd_ctx *ctx, const Table &t, int nb, int nang, const int *d_
         S.beta = beta;
         S.coef = coef;
         S.mode = mode;
+        prof_mark(ctx);
         launch_stage<48>(ctx, S);
+        prof_mark(ctx);
         if (mode != 2) std::swap(Q, Qn);
     };
     for (int l = 1; l <= t.ns; ++l) {

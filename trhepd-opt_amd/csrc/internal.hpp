@@ -49,6 +49,27 @@ struct Stats {
     double device_ms = 0.0;
 };
 
+// a batch of angles with device-resident inputs and outputs (one allocation)
+struct Batch {
+    int id = 0, table_id = 0, nb = 0, nang = 0, nb0 = 1;
+    bool has_f2 = false;
+    char *d = nullptr;
+    size_t bytes = 0;
+    int *gmap = nullptr;
+    double2 *gamma = nullptr, *fin = nullptr, *fout = nullptr;
+    double *f2 = nullptr, *w = nullptr;
+    int *cnt = nullptr;
+};
+
+// event pairs bracketing stage-product launches when profiling is on
+struct Prof {
+    bool on = false;
+    std::vector<hipEvent_t> ev;
+    int used = 0;
+    long long launches = 0;
+    double ms = 0.0;
+};
+
 }  // namespace trhepd
 
 struct trhepd_ctx {
@@ -64,6 +85,9 @@ struct trhepd_ctx {
     // scratch owned by the context, grown on demand (kept across calls)
     std::vector<trhepd::DevBuf> bufs;
     trhepd::Stats stats;
+    int next_batch = 1;
+    std::vector<trhepd::Batch> batches;
+    trhepd::Prof prof;
     void *tune = nullptr;   // autotuned launch choices
 };
 
@@ -84,6 +108,11 @@ void dev_free(trhepd_ctx *ctx, void *p, size_t bytes);
 int scratch(trhepd_ctx *ctx, int slot, size_t bytes, void **p);
 
 Table *find_table(trhepd_ctx *ctx, int id);
+
+// profiling: record an event before / after a timed launch (no-ops when profiling is off)
+int prof_mark(trhepd_ctx *ctx);
+// accumulate the elapsed times of the recorded pairs (call after the stream is idle)
+int prof_collect(trhepd_ctx *ctx);
 
 // plane-group orbit of a fractional position (strfac.f90:15-65 and its 17 group routines)
 int orbit(int nsg, double x, double y, double *xs, double *ys);

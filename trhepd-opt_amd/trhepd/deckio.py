@@ -443,7 +443,9 @@ def fortran_e(x: float, w: int, d: int) -> str:
 
 
 def format_s_row(angle_deg: float, f2: Sequence[float]) -> str:
-    """One angle row of ``(E12.4,200(",",E23.15))`` with format reversion every 200 values."""
+    """One angle row of ``(E12.4,200(",",E23.15))`` with format reversion every 200 values.
+    Output stops at the first data edit descriptor without a list item, so a record that
+    ends inside the 200-group carries the group's leading "," at its end."""
     lines = []
     cur = fortran_e(angle_deg, 12, 4)
     for i, v in enumerate(f2):
@@ -451,6 +453,8 @@ def format_s_row(angle_deg: float, f2: Sequence[float]) -> str:
             lines.append(cur)
             cur = ""
         cur += "," + fortran_e(float(v), 23, 15)
+    if len(f2) % 200 != 0 or len(f2) == 0:
+        cur += ","
     lines.append(cur)
     return "\n".join(lines)
 
@@ -466,7 +470,7 @@ def write_s(path: str, naz: int, ng: int, ih: Sequence[int], ik: Sequence[int],
     for block in rows:
         for ang, f2 in block:
             out.append(format_s_row(ang, f2))
-        out.append(" ")
+        out.append("")
     with open(path, "w") as fh:
         fh.write("\n".join(out) + "\n")
 

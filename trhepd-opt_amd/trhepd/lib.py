@@ -25,7 +25,8 @@ PREC_FP64, PREC_MIXED = 0, 1
 SYMBOLS = ("trhepd_ctx_create", "trhepd_ctx_destroy", "trhepd_last_error", "trhepd_ctx_bytes",
            "trhepd_asfcrr", "trhepd_srfghk", "trhepd_asfcef", "trhepd_scpot_stf",
            "trhepd_table_build", "trhepd_table_shape", "trhepd_table_fetch", "trhepd_table_release",
-           "trhepd_propagate_batch", "trhepd_last_stats")
+           "trhepd_propagate_batch", "trhepd_batch_create", "trhepd_batch_run", "trhepd_batch_fetch",
+           "trhepd_batch_release", "trhepd_last_stats", "trhepd_profile", "trhepd_kernel_stats")
 
 
 class TrhepdError(RuntimeError):
@@ -76,6 +77,12 @@ def load() -> C.CDLL:
     L.trhepd_table_release.argtypes = [vp, i]
     L.trhepd_propagate_batch.argtypes = [vp, i, i, i, vp, vp, vp, vp, i, vp]
     L.trhepd_last_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), dp]
+    L.trhepd_batch_create.argtypes = [vp, i, i, i, vp, vp, vp, i, vp, ip]
+    L.trhepd_batch_run.argtypes = [vp, i]
+    L.trhepd_batch_fetch.argtypes = [vp, i, vp, vp]
+    L.trhepd_batch_release.argtypes = [vp, i]
+    L.trhepd_profile.argtypes = [vp, i]
+    L.trhepd_kernel_stats.argtypes = [vp, C.POINTER(C.c_longlong), dp]
     for name in SYMBOLS:
         getattr(L, name).restype = getattr(L, name).restype or C.c_int
     L.trhepd_last_error.restype = C.c_char_p
@@ -234,6 +241,43 @@ class Engine:
         self._chk(self._L.trhepd_propagate_batch(self._h, tid, nb, nang, _p(ivf), _p(g), _p(ff), _p(f2), nb0,
                                                  _p(w)))
         return ff.transpose(0, 2, 1), f2
+
+    # -- device-resident batches ----------------------------------------------
+    def batch_create(self, tid: int, iv, gamma, f, nb0: int, wnsga2=None) -> int:
+        """Upload one batch (same conventions as propagate_batch); returns its id."""
+        nang, nb = gamma.shape
+        self._keep_shape = getattr(self, "_keep_shape", {})
+        ivf = np.asfortranarray(iv, dtype=np.int32)
+        g = np.ascontiguousarray(gamma, dtype=np.complex128)
+        ff = np.ascontiguousarray(np.asarray(f, dtype=np.complex128).transpose(0, 2, 1))
+        w = None if wnsga2 is None else np.ascontiguousarray(wnsga2, dtype=np.float64)
+        bid = C.c_int(0)
+        self._chk(self._L.trhepd_batch_create(self._h, tid, nb, nang, _p(ivf), _p(g), _p(ff), nb0, _p(w),
+                                              C.byref(bid)))
+        self._keep_shape[bid.value] = (nang, nb, w is not None)
+        return bid.value
+
+    def batch_run(self, bid: int):
+        self._chk(self._L.trhepd_batch_run(self._h, bid))
+
+    def batch_fetch(self, bid: int, want_f: bool = True):
+        nang, nb, has_f2 = self._keep_shape[bid]
+        ff = np.zeros((nang, nb, nb), dtype=np.complex128) if want_f else None
+        f2 = np.zeros((nang, nb)) if has_f2 else None
+        self._chk(self._L.trhepd_batch_fetch(self._h, bid, _p(ff), _p(f2)))
+        return (None if ff is None else ff.transpose(0, 2, 1)), f2
+
+    def batch_release(self, bid: int):
+        self._chk(self._L.trhepd_batch_release(self._h, bid))
+        self._keep_shape.pop(bid, None)
+
+    def profile(self, enable: bool):
+        self._chk(self._L.trhepd_profile(self._h, 1 if enable else 0))
+
+    def kernel_stats(self):
+        n, ms = C.c_longlong(), C.c_double()
+        self._chk(self._L.trhepd_kernel_stats(self._h, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
 
     def last_stats(self):
         r, p, ms = C.c_longlong(), C.c_longlong(), C.c_double()
