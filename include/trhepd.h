@@ -140,6 +140,9 @@ int trhepd_batch_create(trhepd_ctx *ctx, int table_id, int nb, int nang, const i
                         const double *gamma, const double *f, int nb0, const double *wnsga2,
                         int *batch_id);
 int trhepd_batch_run(trhepd_ctx *ctx, int batch_id);
+/* Runs n batches concurrently, each on its own HIP stream; their slices interleave so that
+ * one batch's renormalisation solves overlap the others' stage products. */
+int trhepd_batch_run_many(trhepd_ctx *ctx, int n, const int *batch_ids);
 int trhepd_batch_fetch(trhepd_ctx *ctx, int batch_id, double *f, double *f2);
 int trhepd_batch_release(trhepd_ctx *ctx, int batch_id);
 

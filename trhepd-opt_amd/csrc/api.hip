@@ -72,7 +72,7 @@ int scratch(trhepd_ctx *ctx, int slot, size_t bytes, void **p)
     return TRHEPD_OK;
 }
 
-int prof_mark(trhepd_ctx *ctx)
+int prof_mark(trhepd_ctx *ctx, hipStream_t stream)
 {
     Prof &P = ctx->prof;
     if (!P.on) return TRHEPD_OK;
@@ -81,7 +81,7 @@ int prof_mark(trhepd_ctx *ctx)
         TRY_HIP(ctx, hipEventCreate(&e));
         P.ev.push_back(e);
     }
-    TRY_HIP(ctx, hipEventRecord(P.ev[P.used++], ctx->stream));
+    TRY_HIP(ctx, hipEventRecord(P.ev[P.used++], stream));
     return TRHEPD_OK;
 }
 
@@ -182,7 +182,13 @@ int trhepd_ctx_destroy(trhepd_ctx *ctx)
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
     for (Table &t : ctx->tables) dev_free(ctx, t.d, t.bytes);
-    for (Batch &b : ctx->batches) dev_free(ctx, b.d, b.bytes);
+    for (Batch &b : ctx->batches) {
+        runner_destroy(ctx, b);
+        dev_free(ctx, b.d, b.bytes);
+        (void)hipEventDestroy(b.ev0);
+        (void)hipEventDestroy(b.ev1);
+        (void)hipStreamDestroy(b.stream);
+    }
     for (hipEvent_t e : ctx->prof.ev) (void)hipEventDestroy(e);
     for (DevBuf &b : ctx->bufs) dev_free(ctx, b.p, b.bytes);
     hipEventDestroy(ctx->ev0);
@@ -405,14 +411,22 @@ int trhepd_batch_create(trhepd_ctx *ctx, int table_id, int nb, int nang, const i
     b.nang = nang;
     b.nb0 = nb0;
     b.has_f2 = wnsga2 != nullptr;
-    hipError_t e = hipMemcpyAsync(b.gmap, gm.data(), nn * 4, hipMemcpyHostToDevice, ctx->stream);
-    if (!e) e = hipMemcpyAsync(b.gamma, gamma, (size_t)nb * nang * 16, hipMemcpyHostToDevice, ctx->stream);
-    if (!e) e = hipMemcpyAsync(b.fin, f, nn * nang * 16, hipMemcpyHostToDevice, ctx->stream);
-    if (!e && wnsga2) e = hipMemcpyAsync(b.w, wnsga2, (size_t)nang * 8, hipMemcpyHostToDevice, ctx->stream);
-    if (!e) e = hipStreamSynchronize(ctx->stream);   // gm is a host temporary
-    if (e) {
+    hipError_t e = hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking);
+    if (!e) e = hipEventCreate(&b.ev0);
+    if (!e) e = hipEventCreate(&b.ev1);
+    if (!e) e = hipMemcpyAsync(b.gmap, gm.data(), nn * 4, hipMemcpyHostToDevice, b.stream);
+    if (!e) e = hipMemcpyAsync(b.gamma, gamma, (size_t)nb * nang * 16, hipMemcpyHostToDevice, b.stream);
+    if (!e) e = hipMemcpyAsync(b.fin, f, nn * nang * 16, hipMemcpyHostToDevice, b.stream);
+    if (!e && wnsga2) e = hipMemcpyAsync(b.w, wnsga2, (size_t)nang * 8, hipMemcpyHostToDevice, b.stream);
+    if (!e) e = hipStreamSynchronize(b.stream);   // gm is a host temporary
+    int rc2 = e ? hip_check(ctx, e, "batch upload") : TRHEPD_OK;
+    if (!rc2 && nb > 48) rc2 = runner_create(ctx, b, *t);
+    if (rc2) {
         dev_free(ctx, b.d, b.bytes);
-        return hip_check(ctx, e, "batch upload");
+        if (b.ev0) (void)hipEventDestroy(b.ev0);
+        if (b.ev1) (void)hipEventDestroy(b.ev1);
+        if (b.stream) (void)hipStreamDestroy(b.stream);
+        return rc2;
     }
     b.id = ctx->next_batch++;
     ctx->batches.push_back(b);
@@ -420,40 +434,85 @@ int trhepd_batch_create(trhepd_ctx *ctx, int table_id, int nb, int nang, const i
     return TRHEPD_OK;
 }
 
-int trhepd_batch_run(trhepd_ctx *ctx, int batch_id)
+int trhepd_batch_run_many(trhepd_ctx *ctx, int n, const int *batch_ids)
 {
     if (!ctx) return TRHEPD_E_ARG;
-    Batch *b = find_batch(ctx, batch_id);
-    if (!b) return fail(ctx, TRHEPD_E_NOTFOUND, "no batch %d", batch_id);
-    Table *t = find_table(ctx, b->table_id);
-    if (!t) return fail(ctx, TRHEPD_E_NOTFOUND, "table %d of batch %d was released", b->table_id, batch_id);
+    if (n < 1 || !batch_ids) return fail(ctx, TRHEPD_E_ARG, "batch_run_many: no batches");
+    std::vector<Batch *> bs;
+    std::vector<const Table *> ts;
+    for (int i = 0; i < n; ++i) {
+        Batch *b = find_batch(ctx, batch_ids[i]);
+        if (!b) return fail(ctx, TRHEPD_E_NOTFOUND, "no batch %d", batch_ids[i]);
+        for (Batch *o : bs)
+            if (o == b) return fail(ctx, TRHEPD_E_ARG, "batch %d listed twice", batch_ids[i]);
+        Table *t = find_table(ctx, b->table_id);
+        if (!t) return fail(ctx, TRHEPD_E_NOTFOUND, "table %d of batch %d was released", b->table_id, b->id);
+        bs.push_back(b);
+        ts.push_back(t);
+    }
     TRY_HIP(ctx, hipSetDevice(ctx->device));
     ctx->stats = Stats();
-    const size_t nn = (size_t)b->nb * b->nb;
-    TRY_HIP(ctx, hipMemsetAsync(b->cnt, 0, 64, ctx->stream));
-    TRY_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    TRY_HIP(ctx, hipMemcpyAsync(b->fout, b->fin, nn * b->nang * 16, hipMemcpyDeviceToDevice, ctx->stream));
-    int rc;
-    double *f2 = b->has_f2 ? b->f2 : nullptr;
-    if (b->nb <= 48)
-        rc = run_small(ctx, *t, b->nb, b->nang, b->gmap, b->gamma, b->fout, f2, b->nb0, b->w, b->cnt);
-    else
-        rc = run_batched(ctx, *t, b->nb, b->nang, b->gmap, b->gamma, b->fout, f2, b->nb0, b->w, b->cnt);
+    int rc = TRHEPD_OK, L = 0;
+    for (int i = 0; i < n && !rc; ++i) {
+        Batch &b = *bs[i];
+        const size_t nn = (size_t)b.nb * b.nb;
+        b.stats = Stats();
+        TRY_HIP(ctx, hipMemsetAsync(b.cnt, 0, 64, b.stream));
+        TRY_HIP(ctx, hipEventRecord(b.ev0, b.stream));
+        TRY_HIP(ctx, hipMemcpyAsync(b.fout, b.fin, nn * b.nang * 16, hipMemcpyDeviceToDevice, b.stream));
+        if (b.run) {
+            rc = runner_begin(ctx, b, *ts[i]);
+            L = std::max(L, ts[i]->ns);
+        } else {
+            rc = run_small(ctx, b.stream, *ts[i], b.nb, b.nang, b.gmap, b.gamma, b.fout, b.has_f2 ? b.f2 : nullptr,
+                           b.nb0, b.w, b.cnt);
+        }
+    }
+    // slices of all large-N batches interleaved: while the host waits for one batch's
+    // renormalisation flags the others' kernels are already queued on their streams
+    for (int l = 1; l <= L && !rc; ++l) {
+        for (int i = 0; i < n && !rc; ++i)
+            if (bs[i]->run && l <= ts[i]->ns) rc = runner_slice_enqueue(ctx, *bs[i], *ts[i], l);
+        for (int i = 0; i < n && !rc; ++i)
+            if (bs[i]->run && l <= ts[i]->ns) rc = runner_slice_finish(ctx, *bs[i], *ts[i]);
+    }
+    for (int i = 0; i < n && !rc; ++i)
+        if (bs[i]->run) rc = runner_end(ctx, *bs[i], *ts[i]);
+    for (int i = 0; i < n; ++i) {
+        if (!rc) rc = hip_check(ctx, hipEventRecord(bs[i]->ev1, bs[i]->stream), "event");
+    }
+    int singular = 0;
+    for (int i = 0; i < n; ++i) {
+        Batch &b = *bs[i];
+        const hipError_t e = hipStreamSynchronize(b.stream);
+        if (!rc && e) rc = hip_check(ctx, e, "batch stream");
+        if (rc) continue;
+        int cnt[16];
+        TRY_HIP(ctx, hipMemcpy(cnt, b.cnt, 64, hipMemcpyDeviceToHost));
+        float ms = 0;
+        TRY_HIP(ctx, hipEventElapsedTime(&ms, b.ev0, b.ev1));
+        b.stats.n_renorm = cnt[0];
+        b.stats.n_products = (long long)b.nang * ts[i]->ns * rkn_coef(ts[i]->method).s;
+        b.stats.device_ms = ms;
+        singular |= cnt[1];
+        ctx->stats.n_renorm += b.stats.n_renorm;
+        ctx->stats.n_products += b.stats.n_products;
+    }
     if (rc) return rc;
-    TRY_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-    int cnt[16];
-    TRY_HIP(ctx, hipMemcpyAsync(cnt, b->cnt, 64, hipMemcpyDeviceToHost, ctx->stream));
-    TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    // device span: first start to last end
+    float span = 0;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, bs[i]->ev0, bs[j]->ev1) == hipSuccess) span = std::max(span, ms);
+        }
+    ctx->stats.device_ms = span;
     if ((rc = prof_collect(ctx))) return rc;
-    float ms = 0;
-    TRY_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-    const RknCoef co = rkn_coef(t->method);
-    ctx->stats.n_renorm = cnt[0];
-    ctx->stats.n_products = (long long)b->nang * t->ns * co.s;
-    ctx->stats.device_ms = ms;
-    if (cnt[1]) return fail(ctx, TRHEPD_E_SINGULAR, "exact zero pivot in a renormalisation LU");
+    if (singular) return fail(ctx, TRHEPD_E_SINGULAR, "exact zero pivot in a renormalisation LU");
     return TRHEPD_OK;
 }
+
+int trhepd_batch_run(trhepd_ctx *ctx, int batch_id) { return trhepd_batch_run_many(ctx, 1, &batch_id); }
 
 int trhepd_batch_fetch(trhepd_ctx *ctx, int batch_id, double *f, double *f2)
 {
@@ -463,9 +522,9 @@ int trhepd_batch_fetch(trhepd_ctx *ctx, int batch_id, double *f, double *f2)
     if (f2 && !b->has_f2) return fail(ctx, TRHEPD_E_ARG, "batch %d was created without wnsga2", batch_id);
     TRY_HIP(ctx, hipSetDevice(ctx->device));
     const size_t nn = (size_t)b->nb * b->nb;
-    if (f) TRY_HIP(ctx, hipMemcpyAsync(f, b->fout, nn * b->nang * 16, hipMemcpyDeviceToHost, ctx->stream));
-    if (f2) TRY_HIP(ctx, hipMemcpyAsync(f2, b->f2, (size_t)b->nb * b->nang * 8, hipMemcpyDeviceToHost, ctx->stream));
-    TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (f) TRY_HIP(ctx, hipMemcpyAsync(f, b->fout, nn * b->nang * 16, hipMemcpyDeviceToHost, b->stream));
+    if (f2) TRY_HIP(ctx, hipMemcpyAsync(f2, b->f2, (size_t)b->nb * b->nang * 8, hipMemcpyDeviceToHost, b->stream));
+    TRY_HIP(ctx, hipStreamSynchronize(b->stream));
     if (f2)
         for (size_t e = 0; e < (size_t)b->nb * b->nang; ++e)
             if (!std::isfinite(f2[e])) return fail(ctx, TRHEPD_E_NONFINITE, "non-finite intensity at %zu", e);
@@ -480,8 +539,13 @@ int trhepd_batch_release(trhepd_ctx *ctx, int batch_id)
     if (!ctx) return TRHEPD_E_ARG;
     for (size_t i = 0; i < ctx->batches.size(); ++i)
         if (ctx->batches[i].id == batch_id) {
-            TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-            dev_free(ctx, ctx->batches[i].d, ctx->batches[i].bytes);
+            Batch &b = ctx->batches[i];
+            TRY_HIP(ctx, hipStreamSynchronize(b.stream));
+            runner_destroy(ctx, b);
+            dev_free(ctx, b.d, b.bytes);
+            (void)hipEventDestroy(b.ev0);
+            (void)hipEventDestroy(b.ev1);
+            (void)hipStreamDestroy(b.stream);
             ctx->batches.erase(ctx->batches.begin() + i);
             return TRHEPD_OK;
         }

@@ -234,25 +234,48 @@ __global__ __launch_bounds__(256) void k_stage_gemm(StageArgs S)
     const int ksteps = (g.N + 3) >> 2;
     const double2 *__restrict__ Qp = S.Q;
     const double2 *__restrict__ Mp = S.M0;
+    int jb[NB];
+    bool jv[NB];
+#pragma unroll
+    for (int cb = 0; cb < NB; ++cb) {
+        jb[cb] = c0 + cb * 16 + l16;
+        jv[cb] = jb[cb] < g.NP;
+    }
+    // operands of k-step s+1 are loaded while the matrix cores work on step s; the four
+    // real products of each complex block are issued term by term so that consecutive
+    // MFMAs never depend on each other
+    double2 av[2], bv[NB];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) av[rb] = Qp[abase[rb] + (size_t)kq * g.NP];
+#pragma unroll
+    for (int cb = 0; cb < NB; ++cb) bv[cb] = jv[cb] ? Mp[(size_t)kq * g.NP + jb[cb]] : make_double2(0, 0);
     for (int s = 0; s < ksteps; ++s) {
-        const int k = s * 4 + kq;
-        double2 av[2], bv[NB];
+        double2 an[2], bn[NB];
+        const int kn = (s + 1 < ksteps ? s + 1 : s) * 4 + kq;
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb) av[rb] = Qp[abase[rb] + (size_t)k * g.NP];
+        for (int rb = 0; rb < 2; ++rb) an[rb] = Qp[abase[rb] + (size_t)kn * g.NP];
 #pragma unroll
-        for (int cb = 0; cb < NB; ++cb) {
-            const int j = c0 + cb * 16 + l16;
-            bv[cb] = (j < g.NP) ? Mp[(size_t)k * g.NP + j] : make_double2(0, 0);
-        }
+        for (int cb = 0; cb < NB; ++cb) bn[cb] = jv[cb] ? Mp[(size_t)kn * g.NP + jb[cb]] : make_double2(0, 0);
 #pragma unroll
         for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
-            for (int cb = 0; cb < NB; ++cb) {
-                cr[rb][cb] = mfma4(av[rb].x, bv[cb].x, cr[rb][cb]);
-                cr[rb][cb] = mfma4(-av[rb].y, bv[cb].y, cr[rb][cb]);
-                ci[rb][cb] = mfma4(av[rb].x, bv[cb].y, ci[rb][cb]);
-                ci[rb][cb] = mfma4(av[rb].y, bv[cb].x, ci[rb][cb]);
-            }
+            for (int cb = 0; cb < NB; ++cb) cr[rb][cb] = mfma4(av[rb].x, bv[cb].x, cr[rb][cb]);
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < NB; ++cb) ci[rb][cb] = mfma4(av[rb].x, bv[cb].y, ci[rb][cb]);
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < NB; ++cb) cr[rb][cb] = mfma4(-av[rb].y, bv[cb].y, cr[rb][cb]);
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < NB; ++cb) ci[rb][cb] = mfma4(av[rb].y, bv[cb].x, ci[rb][cb]);
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) av[rb] = an[rb];
+#pragma unroll
+        for (int cb = 0; cb < NB; ++cb) bv[cb] = bn[cb];
     }
     const double2 w0 = w_upper(S.col[0]);
 #pragma unroll
@@ -346,6 +369,7 @@ struct LuArgs {
     int *piv;          // [A][NP]
     const int *list;
     int *flag;         // singular pivot flag
+    double2 *dinv;     // [npts][nblk][16*16] inverses of U's diagonal blocks
 };
 
 __device__ __forceinline__ double2 *colp(const LuArgs &L, int a, int c)
@@ -354,25 +378,34 @@ __device__ __forceinline__ double2 *colp(const LuArgs &L, int a, int c)
                        : L.P + (size_t)a * L.g.mat + (size_t)(c - L.g.N) * L.g.NP;
 }
 
-// factor the panel of columns [k0, k0+kw) in LDS
-__global__ __launch_bounds__(256) void k_lu_panel(LuArgs L, int k0)
+// Factor the panel of columns [k0, k0+kw) (rows k0..N-1) in LDS `pan`, stored
+// column-major with an odd row pitch so that a wave reading consecutive rows, or 16
+// threads reading one row across the columns, hit distinct banks.  Per column: one pivot
+// search (|re|+|im|, first maximum), the row swap inside the panel, then the scaling and
+// rank-1 update of the remaining panel columns, each thread owning whole rows.  Writes the
+// factored panel and the pivots back.  All 256 threads call it.
+__device__ void factor_panel(const LuArgs &L, int a, int k0, double2 *pan, double *rbest, int *ribest)
 {
-    extern __shared__ double2 pan[];   // [rows][16]
-    __shared__ double rbest[4];
-    __shared__ int ribest[4];
-    const int a = L.list[blockIdx.x];
     const int N = L.g.N, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int kw = min(16, N - k0), rows = N - k0;
-    for (int e = tid; e < rows * 16; e += 256) {
-        const int r = e >> 4, c = e & 15;
-        pan[e] = (c < kw) ? colp(L, a, k0 + c)[k0 + r] : make_double2(0, 0);
+    const int kw = min(16, N - k0), rows = N - k0, ld = rows | 1;
+    // all loads of a 256-row chunk are issued before any LDS store
+    for (int r0 = 0; r0 < rows; r0 += 256) {
+        const int r = r0 + tid;
+        double2 buf[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c)
+            buf[c] = (c < kw && r < rows) ? colp(L, a, k0 + c)[k0 + r] : make_double2(0, 0);
+        if (r < rows)
+#pragma unroll
+            for (int c = 0; c < 16; ++c)
+                if (c < kw) pan[c * ld + r] = buf[c];
     }
     __syncthreads();
     for (int c = 0; c < kw; ++c) {
         double best = -1.0;
         int bi = rows;
         for (int r = c + tid; r < rows; r += 256) {
-            const double2 v = pan[r * 16 + c];
+            const double2 v = pan[c * ld + r];
             const double m = fabs(v.x) + fabs(v.y);
             if (m > best) { best = m; bi = r; }
         }
@@ -391,116 +424,225 @@ __global__ __launch_bounds__(256) void k_lu_panel(LuArgs L, int k0)
             L.piv[(size_t)a * L.g.NP + k0 + c] = k0 + bi;
             if (best == 0.0) *L.flag = 1;
         }
-        if (bi != c && tid < 16) {
-            const double2 t0 = pan[c * 16 + tid];
-            pan[c * 16 + tid] = pan[bi * 16 + tid];
-            pan[bi * 16 + tid] = t0;
+        if (bi != c && tid < kw) {
+            const double2 t0 = pan[tid * ld + c];
+            pan[tid * ld + c] = pan[tid * ld + bi];
+            pan[tid * ld + bi] = t0;
         }
         __syncthreads();
-        const double2 pk = pan[c * 16 + c];
-        if (pk.x != 0.0 || pk.y != 0.0) {
-            const double2 rp = cdiv(make_double2(1, 0), pk);
-            for (int r = c + 1 + tid; r < rows; r += 256) pan[r * 16 + c] = cmul(pan[r * 16 + c], rp);
-        }
-        __syncthreads();
-        const int nc = kw - 1 - c;
-        if (nc > 0)
-            for (int e = tid; e < (rows - c - 1) * nc; e += 256) {
-                const int r = c + 1 + e / nc, cc = c + 1 + e % nc;
-                const double2 l = pan[r * 16 + c], u = pan[c * 16 + cc];
-                double2 v = pan[r * 16 + cc];
+        const double2 pk = pan[c * ld + c];
+        const bool nz = (pk.x != 0.0 || pk.y != 0.0);
+        const double2 rp = nz ? cdiv(make_double2(1, 0), pk) : make_double2(0, 0);
+        for (int r = c + 1 + tid; r < rows; r += 256) {
+            double2 l = pan[c * ld + r];
+            if (nz) {
+                l = cmul(l, rp);
+                pan[c * ld + r] = l;
+            }
+            for (int cc = c + 1; cc < kw; ++cc) {
+                const double2 u = pan[cc * ld + c];
+                double2 v = pan[cc * ld + r];
                 v.x -= l.x * u.x - l.y * u.y;
                 v.y -= l.x * u.y + l.y * u.x;
-                pan[r * 16 + cc] = v;
+                pan[cc * ld + r] = v;
             }
+        }
         __syncthreads();
     }
-    for (int e = tid; e < rows * 16; e += 256) {
-        const int r = e >> 4, c = e & 15;
-        if (c < kw) colp(L, a, k0 + c)[k0 + r] = pan[e];
+    for (int c = 0; c < kw; ++c) {
+        double2 *dst = colp(L, a, k0 + c) + k0;
+        for (int r = tid; r < rows; r += 256) dst[r] = pan[c * ld + r];
     }
 }
 
-// for one 16-column tile right of the panel: row swaps, U12 = L11^{-1} A12, A22 -= L21 U12
-__global__ __launch_bounds__(256) void k_lu_update(LuArgs L, int k0, int nqt)
+__global__ __launch_bounds__(256) void k_lu_panel0(LuArgs L)
 {
-    __shared__ double2 U[16][16];   // [k][j]
+    extern __shared__ double2 dyn[];
+    __shared__ double rbest[4];
+    __shared__ int ribest[4];
+    factor_panel(L, L.list[blockIdx.x], 0, dyn, rbest, ribest);
+}
+
+// One elimination step for a 64-column tile right of panel [k0, k0+kw): the panel's row
+// interchanges (gathered as one permutation of the <= 32 rows they touch), U12 =
+// L11^{-1} A12, A22 -= L21 U12 on the matrix cores.  The tile that holds the next panel's
+// columns then factors that panel, so each step is one launch.
+constexpr int LU_TILE = 64;
+
+__global__ __launch_bounds__(256) void k_lu_step(LuArgs L, int k0)
+{
+    extern __shared__ double2 dyn[];
+    __shared__ double2 L11[16][16];
+    __shared__ double2 U[16][LU_TILE];
+    __shared__ int trow[32], tsrc[32], ntouch;
+    __shared__ double rbest[4];
+    __shared__ int ribest[4];
     const int a = L.list[blockIdx.y];
     const int N = L.g.N, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int kw = min(16, N - k0);
-    // tile columns: Q columns [k0+kw, N) in tiles of 16, then P columns [0, N)
-    const int tix = blockIdx.x;
-    int cstart, cw;
-    if (tix < nqt) {
-        cstart = k0 + kw + tix * 16;
-        cw = min(16, N - cstart);
-    } else {
-        cstart = N + (tix - nqt) * 16;
-        cw = min(16, 2 * N - cstart);
+    const int kw = min(16, N - k0), first = k0 + kw;
+    const int c0 = first + blockIdx.x * LU_TILE;
+    const int cw = min(LU_TILE, 2 * N - c0);
+    double2 *swp = dyn;   // [32][LU_TILE]
+    for (int e = tid; e < 256; e += 256) L11[e >> 4][e & 15] = make_double2(0, 0);
+    __syncthreads();
+    for (int e = tid; e < kw * kw; e += 256) {
+        const int r = e / kw, c = e % kw;
+        L11[r][c] = colp(L, a, k0 + c)[k0 + r];
     }
-    if (tid < 16) {
+    if (tid == 0) {
+        // the panel rows k0..k0+kw-1 take slots 0..kw-1, rows they displace follow
+        int n = 0;
+        for (int c = 0; c < kw; ++c) {
+            trow[n] = k0 + c;
+            tsrc[n] = k0 + c;
+            ++n;
+        }
+        auto idx = [&](int r) {
+            for (int i = 0; i < n; ++i)
+                if (trow[i] == r) return i;
+            trow[n] = r;
+            tsrc[n] = r;
+            return n++;
+        };
+        for (int c = 0; c < kw; ++c) {
+            const int i1 = idx(k0 + c), i2 = idx(L.piv[(size_t)a * L.g.NP + k0 + c]);
+            const int t0 = tsrc[i1];
+            tsrc[i1] = tsrc[i2];
+            tsrc[i2] = t0;
+        }
+        ntouch = n;
+    }
+    __syncthreads();
+    const int nt = ntouch;
+    {
+        // up to 32 x 64 elements: 8 per thread, loads issued together
+        double2 buf[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = tid + q * 256, i = e / LU_TILE, c = e % LU_TILE;
+            buf[q] = (i < nt && c < cw) ? colp(L, a, c0 + c)[tsrc[i]] : make_double2(0, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = tid + q * 256, i = e / LU_TILE;
+            if (i < nt) swp[e] = buf[q];
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < nt * LU_TILE; e += 256) {
+        const int i = e / LU_TILE, c = e % LU_TILE;
+        if (c < cw && trow[i] >= first) colp(L, a, c0 + c)[trow[i]] = swp[i * LU_TILE + c];
+    }
+    // U12: forward substitution with unit-lower L11, one column per thread (rows k0.. are
+    // the first kw touched rows, in order)
+    if (tid < LU_TILE) {
         for (int r = 0; r < 16; ++r) U[r][tid] = make_double2(0, 0);
         if (tid < cw) {
-            double2 *col = colp(L, a, cstart + tid);
-            for (int c = 0; c < kw; ++c) {
-                const int p = L.piv[(size_t)a * L.g.NP + k0 + c];
-                if (p != k0 + c) {
-                    const double2 t0 = col[k0 + c];
-                    col[k0 + c] = col[p];
-                    col[p] = t0;
-                }
-            }
-            double2 u[16];
+            double2 *col = colp(L, a, c0 + tid);
             for (int r = 0; r < kw; ++r) {
-                double2 x = col[k0 + r];
+                double2 v = swp[r * LU_TILE + tid];
                 for (int rr = 0; rr < r; ++rr) {
-                    const double2 lv = colp(L, a, k0 + rr)[k0 + r];
-                    x.x -= lv.x * u[rr].x - lv.y * u[rr].y;
-                    x.y -= lv.x * u[rr].y + lv.y * u[rr].x;
+                    const double2 l = L11[r][rr], x = U[rr][tid];
+                    v.x -= l.x * x.x - l.y * x.y;
+                    v.y -= l.x * x.y + l.y * x.x;
                 }
-                u[r] = x;
-                col[k0 + r] = x;
-                U[r][tid] = x;
+                U[r][tid] = v;
+                col[k0 + r] = v;
             }
         }
     }
     __syncthreads();
     const int l16 = lane & 15, kq = lane >> 4;
-    const int first = k0 + kw;
     const int nrb = (N - first + 15) / 16;
     const int ks = (kw + 3) >> 2;
+    const int ncb = (cw + 15) / 16;
     for (int rb = wave; rb < nrb; rb += 4) {
-        const int i0 = first + rb * 16;
-        const int ia = i0 + l16;
-        d4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
-        for (int s = 0; s < ks; ++s) {
-            const int k = s * 4 + kq;
-            double2 av = make_double2(0, 0);
-            if (k < kw && ia < N) av = colp(L, a, k0 + k)[ia];
-            const double2 bv = U[k][l16];
-            cr = mfma4(av.x, bv.x, cr);
-            cr = mfma4(-av.y, bv.y, cr);
-            ci = mfma4(av.x, bv.y, ci);
-            ci = mfma4(av.y, bv.x, ci);
-        }
-        if (l16 < cw) {
-            double2 *col = colp(L, a, cstart + l16);
+        const int i0 = first + rb * 16, ia = i0 + l16;
+        d4 cr[4], ci[4];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int i = i0 + kq + 4 * r;
-                if (i < N) {
-                    double2 v = col[i];
-                    v.x -= cr[r];
-                    v.y -= ci[r];
-                    col[i] = v;
+        for (int cb = 0; cb < 4; ++cb) { cr[cb] = d4{0, 0, 0, 0}; ci[cb] = d4{0, 0, 0, 0}; }
+        double2 avs[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int k = s * 4 + kq;
+            avs[s] = (s < ks && k < kw && ia < N) ? colp(L, a, k0 + k)[ia] : make_double2(0, 0);
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            if (s >= ks) break;
+            const int k = s * 4 + kq;
+            const double2 av = avs[s];
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) {
+                if (cb < ncb) {
+                    const double2 bv = U[k][cb * 16 + l16];
+                    cr[cb] = mfma4(av.x, bv.x, cr[cb]);
+                    ci[cb] = mfma4(av.x, bv.y, ci[cb]);
+                    cr[cb] = mfma4(-av.y, bv.y, cr[cb]);
+                    ci[cb] = mfma4(av.y, bv.x, ci[cb]);
+                }
+            }
+        }
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) {
+            const int c = cb * 16 + l16;
+            if (cb < ncb && c < cw) {
+                double2 *col = colp(L, a, c0 + c);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int i = i0 + kq + 4 * r;
+                    if (i < N) {
+                        double2 v = col[i];
+                        v.x -= cr[cb][r];
+                        v.y -= ci[cb][r];
+                        col[i] = v;
+                    }
                 }
             }
         }
     }
+    if (blockIdx.x == 0 && k0 + 16 < N) {
+        __threadfence();
+        __syncthreads();
+        factor_panel(L, a, k0 + 16, dyn, rbest, ribest);
+    }
 }
 
-// Y <- U^{-1} Y for one 16-column tile of Y, all block rows (independent per tile)
-__global__ __launch_bounds__(256) void k_lu_back(LuArgs L)
+// Inverses of the 16x16 diagonal blocks of U, all blocks of all listed angles in
+// parallel: one column of the inverse per thread (back substitution in LDS).
+__global__ __launch_bounds__(64) void k_lu_dinv(LuArgs L, double2 *__restrict__ dinv)
+{
+    __shared__ double2 Ukk[16][17];
+    const int a = L.list[blockIdx.y], kb = blockIdx.x;
+    const int N = L.g.N, tid = threadIdx.x;
+    const int r0 = kb * 16, rw = min(16, N - r0);
+    for (int e = tid; e < 256; e += 64) {
+        const int r = e >> 4, c = e & 15;
+        Ukk[r][c] = (r < rw && c < rw) ? colp(L, a, r0 + c)[r0 + r] : make_double2(0, 0);
+    }
+    __syncthreads();
+    double2 *out = dinv + ((size_t)blockIdx.y * gridDim.x + kb) * 256;   // [col][row]
+    if (tid < 16) {
+        double2 x[16];
+#pragma unroll
+        for (int r = 15; r >= 0; --r) {
+            double2 v = make_double2(r == tid ? 1.0 : 0.0, 0.0);
+#pragma unroll
+            for (int rr = r + 1; rr < 16; ++rr) {
+                const double2 u = Ukk[r][rr];
+                v.x -= u.x * x[rr].x - u.y * x[rr].y;
+                v.y -= u.x * x[rr].y + u.y * x[rr].x;
+            }
+            x[r] = (r < rw && tid < rw) ? cdiv(v, Ukk[r][r]) : make_double2(0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) out[tid * 16 + r] = x[r];
+    }
+}
+
+// Y <- U^{-1} Y for one 16-column tile of Y, block rows from the bottom: X_kb =
+// inv(U_kb,kb) Y_kb and Y_above -= U_above,kb X_kb, both on the matrix cores.
+__global__ __launch_bounds__(256) void k_lu_back(LuArgs L, const double2 *__restrict__ dinv)
 {
     __shared__ double2 X[16][16];
     const int a = L.list[blockIdx.y];
@@ -511,48 +653,56 @@ __global__ __launch_bounds__(256) void k_lu_back(LuArgs L)
     const int l16 = lane & 15, kq = lane >> 4;
     for (int kb = nblk - 1; kb >= 0; --kb) {
         const int r0 = kb * 16, rw = min(16, N - r0);
-        if (tid < 16) {
-            for (int r = 0; r < 16; ++r) X[r][tid] = make_double2(0, 0);
-            if (tid < cw) {
-                double2 *y = colp(L, a, N + j0 + tid);
-                double2 x[16];
-                for (int r = rw - 1; r >= 0; --r) {
-                    double2 v = y[r0 + r];
-                    for (int rr = r + 1; rr < rw; ++rr) {
-                        const double2 u = colp(L, a, r0 + rr)[r0 + r];
-                        v.x -= u.x * x[rr].x - u.y * x[rr].y;
-                        v.y -= u.x * x[rr].y + u.y * x[rr].x;
-                    }
-                    x[r] = cdiv(v, colp(L, a, r0 + r)[r0 + r]);
-                    y[r0 + r] = x[r];
-                    X[r][tid] = x[r];
-                }
-            }
-        }
-        __syncthreads();
-        const int ks = (rw + 3) >> 2;
-        for (int rb = wave; rb < kb; rb += 4) {
-            const int i0 = rb * 16, ia = i0 + l16;
+        if (wave == 0) {
+            const double2 *inv = dinv + ((size_t)blockIdx.y * nblk + kb) * 256;
+            const double2 *ycol = colp(L, a, N + j0 + min(l16, cw - 1));
             d4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
-            for (int s = 0; s < ks; ++s) {
+            for (int s = 0; s < 4; ++s) {
                 const int k = s * 4 + kq;
-                double2 av = make_double2(0, 0);
-                if (k < rw) av = colp(L, a, r0 + k)[ia];
-                const double2 bv = X[k][l16];
+                const double2 av = inv[k * 16 + l16];                       // inv(l16, k)
+                const double2 bv = (k < rw && l16 < cw) ? ycol[r0 + k] : make_double2(0, 0);
                 cr = mfma4(av.x, bv.x, cr);
                 cr = mfma4(-av.y, bv.y, cr);
                 ci = mfma4(av.x, bv.y, ci);
                 ci = mfma4(av.y, bv.x, ci);
             }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) X[kq + 4 * r][l16] = make_double2(cr[r], ci[r]);
+        }
+        __syncthreads();
+        if (tid < cw)
+            for (int r = 0; r < rw; ++r) colp(L, a, N + j0 + tid)[r0 + r] = X[r][tid];
+        const int ks = (rw + 3) >> 2;
+        for (int rb = wave; rb < kb; rb += 4) {
+            const int i0 = rb * 16, ia = i0 + l16;
+            d4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+            double2 avs[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int k = s * 4 + kq;
+                avs[s] = (s < ks && k < rw) ? colp(L, a, r0 + k)[ia] : make_double2(0, 0);
+            }
+            double2 yv[4];
+            double2 *y = colp(L, a, N + j0 + min(l16, cw - 1));
+#pragma unroll
+            for (int r = 0; r < 4; ++r) yv[r] = y[i0 + kq + 4 * r];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                if (s >= ks) break;
+                const int k = s * 4 + kq;
+                const double2 av = avs[s], bv = X[k][l16];
+                cr = mfma4(av.x, bv.x, cr);
+                ci = mfma4(av.x, bv.y, ci);
+                cr = mfma4(-av.y, bv.y, cr);
+                ci = mfma4(av.y, bv.x, ci);
+            }
             if (l16 < cw) {
-                double2 *y = colp(L, a, N + j0 + l16);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int i = i0 + kq + 4 * r;
-                    double2 v = y[i];
+                    double2 v = yv[r];
                     v.x -= cr[r];
                     v.y -= ci[r];
-                    y[i] = v;
+                    y[i0 + kq + 4 * r] = v;
                 }
             }
         }
@@ -560,147 +710,228 @@ __global__ __launch_bounds__(256) void k_lu_back(LuArgs L)
     }
 }
 
-int lu_solve(trhepd_ctx *ctx, LuArgs &L, int npts)
+}  // namespace
+
+struct Runner {
+    Geo g;
+    char *d = nullptr;
+    size_t bytes = 0;
+    double2 *Q0, *Q1, *P, *M, *g2c, *gc, *dinv;
+    double *colsum, *diagabs, *coef;
+    int *xid, *ren, *piv, *list, *all;
+    int *h = nullptr;           // pinned: [A] flags, [A] list
+    hipEvent_t evf = nullptr;
+    double2 *Q, *Qn;            // current / next Q buffer
+    LuArgs lu(const int *lst) const { return LuArgs{g, Q, P, piv, lst, nullptr, dinv}; }
+};
+
+static int lds_bytes(int N)
+{
+    return (int)std::max((size_t)(N | 1) * 16 * sizeof(double2), (size_t)32 * LU_TILE * sizeof(double2));
+}
+
+int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
+{
+    if (t.method != TRHEPD_SP4 && t.method != TRHEPD_SP6)
+        return fail(ctx, TRHEPD_E_ARG, "large-N device path supports int_method 5 and 6 (got %d)", t.method);
+    if (b.nb > 600) return fail(ctx, TRHEPD_E_ARG, "N = %d exceeds the device path limit of 600 beams", b.nb);
+    Runner *R = new Runner();
+    Geo &g = R->g;
+    g.A = b.nang;
+    g.N = b.nb;
+    g.NP = (b.nb + 15) / 16 * 16;
+    g.mat = (size_t)g.NP * g.NP;
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~size_t(255); return o; };
+    const size_t mb = g.mat * g.A * sizeof(double2);
+    const size_t oQ0 = take(mb), oQ1 = take(mb), oP = take(mb), oM = take(g.mat * sizeof(double2)),
+                 og2 = take((size_t)g.A * g.NP * 16), ogc = take((size_t)g.A * g.NP * 16),
+                 odi = take((size_t)g.A * (g.NP / 16) * 256 * sizeof(double2)),
+                 ocs = take((size_t)g.A * (g.NP / 16) * g.NP * 8), oda = take((size_t)g.A * g.NP * 8),
+                 oco = take((size_t)g.A * 8), oxi = take((size_t)g.A * 4), ore = take((size_t)g.A * 4),
+                 opv = take((size_t)g.A * g.NP * 4), oli = take((size_t)g.A * 4), oal = take((size_t)g.A * 4);
+    int rc = dev_alloc(ctx, (void **)&R->d, off);
+    if (rc) {
+        delete R;
+        return rc;
+    }
+    R->bytes = off;
+    char *d = R->d;
+    R->Q0 = (double2 *)(d + oQ0);
+    R->Q1 = (double2 *)(d + oQ1);
+    R->P = (double2 *)(d + oP);
+    R->M = (double2 *)(d + oM);
+    R->g2c = (double2 *)(d + og2);
+    R->gc = (double2 *)(d + ogc);
+    R->dinv = (double2 *)(d + odi);
+    R->colsum = (double *)(d + ocs);
+    R->diagabs = (double *)(d + oda);
+    R->coef = (double *)(d + oco);
+    R->xid = (int *)(d + oxi);
+    R->ren = (int *)(d + ore);
+    R->piv = (int *)(d + opv);
+    R->list = (int *)(d + oli);
+    R->all = (int *)(d + oal);
+    hipError_t e = hipHostMalloc((void **)&R->h, (size_t)g.A * 8 + 64, hipHostMallocDefault);
+    if (!e) e = hipEventCreateWithFlags(&R->evf, hipEventDisableTiming);
+    if (e) {
+        dev_free(ctx, R->d, R->bytes);
+        if (R->h) (void)hipHostFree(R->h);
+        delete R;
+        return hip_check(ctx, e, "runner host buffers");
+    }
+    std::vector<int> all(g.A);
+    for (int i = 0; i < g.A; ++i) all[i] = i;
+    TRY_HIP(ctx, hipMemcpyAsync(R->all, all.data(), g.A * 4, hipMemcpyHostToDevice, b.stream));
+    TRY_HIP(ctx, hipStreamSynchronize(b.stream));
+    static bool attr = false;
+    if (!attr) {
+        TRY_HIP(ctx, hipFuncSetAttribute((const void *)k_lu_panel0, hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes(600)));
+        TRY_HIP(ctx, hipFuncSetAttribute((const void *)k_lu_step, hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes(600)));
+        attr = true;
+    }
+    b.run = R;
+    return TRHEPD_OK;
+}
+
+void runner_destroy(trhepd_ctx *ctx, Batch &b)
+{
+    Runner *R = b.run;
+    if (!R) return;
+    dev_free(ctx, R->d, R->bytes);
+    (void)hipHostFree(R->h);
+    (void)hipEventDestroy(R->evf);
+    delete R;
+    b.run = nullptr;
+}
+
+static int lu_on(trhepd_ctx *ctx, hipStream_t st, LuArgs L, int npts)
 {
     if (npts == 0) return TRHEPD_OK;
     const int N = L.g.N;
+    const size_t lds = lds_bytes(N);
+    hipLaunchKernelGGL(k_lu_panel0, dim3(npts), dim3(256), lds, st, L);
     for (int k0 = 0; k0 < N; k0 += 16) {
         const int kw = std::min(16, N - k0);
-        const size_t lds = (size_t)(N - k0) * 16 * sizeof(double2);
-        hipLaunchKernelGGL(k_lu_panel, dim3(npts), dim3(256), lds, ctx->stream, L, k0);
-        const int nqt = (N - k0 - kw + 15) / 16, npt = (N + 15) / 16;
-        hipLaunchKernelGGL(k_lu_update, dim3(nqt + npt, npts), dim3(256), 0, ctx->stream, L, k0, nqt);
+        const int ntile = (2 * N - k0 - kw + LU_TILE - 1) / LU_TILE;
+        hipLaunchKernelGGL(k_lu_step, dim3(ntile, npts), dim3(256), lds, st, L, k0);
     }
-    hipLaunchKernelGGL(k_lu_back, dim3((N + 15) / 16, npts), dim3(256), 0, ctx->stream, L);
+    const int nblk = (N + 15) / 16;
+    hipLaunchKernelGGL(k_lu_dinv, dim3(nblk, npts), dim3(64), 0, st, L, L.dinv);
+    hipLaunchKernelGGL(k_lu_back, dim3(nblk, npts), dim3(256), 0, st, L, (const double2 *)L.dinv);
     TRY_HIP(ctx, hipGetLastError());
     return TRHEPD_OK;
 }
 
-template <int BN>
-void launch_stage(trhepd_ctx *ctx, StageArgs &S)
+int runner_begin(trhepd_ctx *ctx, Batch &b, const Table &t)
 {
-    const int rowt = (S.g.A * S.g.NP + 127) / 128;
-    S.ncolt = (S.g.NP + BN - 1) / BN;
-    S.ntiles = rowt * S.ncolt;
-    S.tiles_per_xcd = (S.ntiles + 7) / 8;
-    hipLaunchKernelGGL(k_stage_gemm<BN>, dim3(8 * S.tiles_per_xcd), dim3(256), 0, ctx->stream, S);
-}
-
-}  // namespace
-
-int run_batched(trhepd_ctx *ctx, const Table &t, int nb, int nang, const int *d_gmap, const double2 *d_gamma,
-                double2 *d_f, double *d_f2, int nb0, const double *d_wnsga2, int *d_counters)
-{
-    if (t.method != TRHEPD_SP4 && t.method != TRHEPD_SP6)
-        return fail(ctx, TRHEPD_E_ARG, "large-N device path supports int_method 5 and 6 (got %d)", t.method);
-    if (nb > 600) return fail(ctx, TRHEPD_E_ARG, "N = %d exceeds the device path limit of 600 beams", nb);
-    Geo g;
-    g.A = nang;
-    g.N = nb;
-    g.NP = (nb + 15) / 16 * 16;
-    g.mat = (size_t)g.NP * g.NP;
-    const size_t mb = g.mat * g.A * sizeof(double2);
-    void *pQ0, *pQ1, *pP, *pM, *pg2, *pgc, *pcs, *pda, *pco, *pfl, *ppv, *plist, *pall;
-    int rc;
-    if ((rc = scratch(ctx, 10, mb, &pQ0)) || (rc = scratch(ctx, 11, mb, &pQ1)) || (rc = scratch(ctx, 12, mb, &pP)) ||
-        (rc = scratch(ctx, 13, g.mat * sizeof(double2), &pM)) ||
-        (rc = scratch(ctx, 14, (size_t)g.A * g.NP * sizeof(double2), &pg2)) ||
-        (rc = scratch(ctx, 15, (size_t)g.A * g.NP * sizeof(double2), &pgc)) ||
-        (rc = scratch(ctx, 16, (size_t)g.A * (g.NP / 16) * g.NP * 8, &pcs)) ||
-        (rc = scratch(ctx, 17, (size_t)g.A * g.NP * 8, &pda)) || (rc = scratch(ctx, 18, (size_t)g.A * 8, &pco)) ||
-        (rc = scratch(ctx, 19, (size_t)g.A * 8, &pfl)) || (rc = scratch(ctx, 20, (size_t)g.A * g.NP * 4, &ppv)) ||
-        (rc = scratch(ctx, 21, (size_t)g.A * 4, &plist)) || (rc = scratch(ctx, 22, (size_t)g.A * 4, &pall)))
-        return rc;
-    int *d_xid = (int *)pfl, *d_ren = (int *)pfl + g.A;
-    std::vector<int> all(g.A);
-    for (int i = 0; i < g.A; ++i) all[i] = i;
-    TRY_HIP(ctx, hipMemcpyAsync(pall, all.data(), g.A * 4, hipMemcpyHostToDevice, ctx->stream));
-    int *h_ren = nullptr;
-    TRY_HIP(ctx, hipHostMalloc((void **)&h_ren, (size_t)g.A * 8 + 64, hipHostMallocDefault));
-    struct Pin {
-        int *p;
-        ~Pin() { (void)hipHostFree(p); }
-    } pin{h_ren};
-    hipEvent_t evf;
-    TRY_HIP(ctx, hipEventCreateWithFlags(&evf, hipEventDisableTiming));
-    struct Ev {
-        hipEvent_t e;
-        ~Ev() { (void)hipEventDestroy(e); }
-    } evguard{evf};
-
+    Runner &R = *b.run;
+    const Geo &g = R.g;
     const size_t ntot = g.mat * g.A;
     const dim3 eb(256), eg((unsigned)((ntot + 255) / 256));
-    hipLaunchKernelGGL(k_prep, dim3((g.NP + 255) / 256, g.A), dim3(256), 0, ctx->stream, g, d_gamma,
-                       (double2 *)pg2, (double2 *)pgc);
-    double2 *Q = (double2 *)pQ0, *Qn = (double2 *)pQ1, *P = (double2 *)pP;
-    hipLaunchKernelGGL(k_inv_setup, eg, eb, 0, ctx->stream, g, (const double2 *)d_f, Q, P);
-    TRY_HIP(ctx, hipMemsetAsync(pM, 0, g.mat * sizeof(double2), ctx->stream));
-    LuArgs L{g, Q, P, (int *)ppv, (const int *)pall, d_counters + 1};
-    // k_lu_panel needs up to N*16*16 bytes of dynamic LDS
-    TRY_HIP(ctx, hipFuncSetAttribute((const void *)k_lu_panel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)(g.N * 16 * sizeof(double2))));
-    if ((rc = lu_solve(ctx, L, g.A))) return rc;
-    hipLaunchKernelGGL(k_inv_scale, eg, eb, 0, ctx->stream, g, (const double2 *)pgc, P);
-    std::vector<int> ones(g.A, 1);
-    TRY_HIP(ctx, hipMemcpyAsync(d_xid, ones.data(), g.A * 4, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_prep, dim3((g.NP + 255) / 256, g.A), dim3(256), 0, b.stream, g, (const double2 *)b.gamma,
+                       R.g2c, R.gc);
+    R.Q = R.Q0;
+    R.Qn = R.Q1;
+    hipLaunchKernelGGL(k_inv_setup, eg, eb, 0, b.stream, g, (const double2 *)b.fout, R.Q, R.P);
+    TRY_HIP(ctx, hipMemsetAsync(R.M, 0, g.mat * sizeof(double2), b.stream));
+    LuArgs L = R.lu(R.all);
+    L.flag = b.cnt + 1;
+    int rc = lu_on(ctx, b.stream, L, g.A);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_inv_scale, eg, eb, 0, b.stream, g, (const double2 *)R.gc, R.P);
+    for (int i = 0; i < g.A; ++i) R.h[g.A + i] = 1;
+    TRY_HIP(ctx, hipMemcpyAsync(R.xid, R.h + g.A, g.A * 4, hipMemcpyHostToDevice, b.stream));
+    // the staging area is rewritten only after the first slice's flag event
+    return TRHEPD_OK;
+}
 
-    const RknCoef co = rkn_coef(t.method);
-    const int s = co.s;
+static void stage_launch(trhepd_ctx *ctx, Batch &b, const Table &t, int mode, const VVi *col, double alpha,
+                         double beta, const double *coef)
+{
+    Runner &R = *b.run;
+    const Geo &g = R.g;
+    const int ggrid = (int)((g.mat + 255) / 256);
+    hipLaunchKernelGGL(k_gather, dim3(ggrid), dim3(256), 0, b.stream, g, b.gmap, col, R.M);
     StageArgs S;
     S.g = g;
-    S.M0 = (const double2 *)pM;
-    S.g2c = (const double2 *)pg2;
-    S.colsum = (double *)pcs;
-    S.diagabs = (double *)pda;
-    const int gthreads = 256, ggrid = (int)((g.mat + gthreads - 1) / gthreads);
-    auto stage = [&](int mode, const VVi *col, double alpha, double beta, const double *coef) {
-        hipLaunchKernelGGL(k_gather, dim3(ggrid), dim3(gthreads), 0, ctx->stream, g, d_gmap, col, (double2 *)pM);
-        S.Q = Q;
-        S.Qn = Qn;
-        S.P = P;
-        S.col = col;
-        S.alpha = alpha;
-        S.beta = beta;
-        S.coef = coef;
-        S.mode = mode;
-        prof_mark(ctx);
-        launch_stage<48>(ctx, S);
-        prof_mark(ctx);
-        if (mode != 2) std::swap(Q, Qn);
-    };
-    for (int l = 1; l <= t.ns; ++l) {
-        const VVi *col1 = t.d + (size_t)(s * (l - 1)) * t.nv;
-        hipLaunchKernelGGL(k_stage1, eg, eb, 0, ctx->stream, g, d_gmap, col1, (const double2 *)pg2,
-                           (const int *)d_xid, co.bab[0] * t.dz, co.bab[1] * t.dz, Q, P);
-        for (int k = 2; k <= s; ++k) {
-            const VVi *col = t.d + (size_t)(s * (l - 1) + k - 1) * t.nv;
-            stage(k == s ? 1 : 0, col, co.bab[2 * k - 2] * t.dz, co.bab[2 * k - 1] * t.dz, nullptr);
-        }
-        hipLaunchKernelGGL(k_cond, dim3(g.A), dim3(256), 0, ctx->stream, g, (const double *)pcs, (const double *)pda,
-                           l == t.ns ? 1 : 0, co.bab[0] * t.dz, (double *)pco, d_xid, d_ren, d_counters);
-        TRY_HIP(ctx, hipMemcpyAsync(h_ren, d_ren, g.A * 4, hipMemcpyDeviceToHost, ctx->stream));
-        TRY_HIP(ctx, hipEventRecord(evf, ctx->stream));
-        stage(2, t.d + (size_t)(s * l) * t.nv, 0.0, 0.0, (const double *)pco);
-        TRY_HIP(ctx, hipGetLastError());
-        TRY_HIP(ctx, hipEventSynchronize(evf));
-        // compact list of the angles to renormalise, staged in pinned memory: the copy is
-        // queued behind this slice's kernels and the next slice rewrites the staging area
-        // only after waiting on its own event, which follows this copy in stream order
-        int *lst = h_ren + g.A, nl = 0;
-        for (int i = 0; i < g.A; ++i)
-            if (h_ren[i]) lst[nl++] = i;
-        if (nl) {
-            TRY_HIP(ctx, hipMemcpyAsync(plist, lst, (size_t)nl * 4, hipMemcpyHostToDevice, ctx->stream));
-            LuArgs Lr{g, Q, P, (int *)ppv, (const int *)plist, d_counters + 1};
-            if ((rc = lu_solve(ctx, Lr, nl))) return rc;
-        }
-    }
-    hipLaunchKernelGGL(k_st_setup, eg, eb, 0, ctx->stream, g, (const double2 *)pgc, Q, P);
-    LuArgs Lf{g, Q, P, (int *)ppv, (const int *)pall, d_counters + 1};
-    if ((rc = lu_solve(ctx, Lf, g.A))) return rc;
+    S.Q = R.Q;
+    S.Qn = R.Qn;
+    S.P = R.P;
+    S.M0 = R.M;
+    S.g2c = R.g2c;
+    S.col = col;
+    S.alpha = alpha;
+    S.beta = beta;
+    S.coef = coef;
+    S.mode = mode;
+    S.colsum = R.colsum;
+    S.diagabs = R.diagabs;
+    prof_mark(ctx, b.stream);
+    const int rowt = (g.A * g.NP + 127) / 128;
+    S.ncolt = (g.NP + 47) / 48;
+    S.ntiles = rowt * S.ncolt;
+    S.tiles_per_xcd = (S.ntiles + 7) / 8;
+    hipLaunchKernelGGL(k_stage_gemm<48>, dim3(8 * S.tiles_per_xcd), dim3(256), 0, b.stream, S);
+    prof_mark(ctx, b.stream);
+    if (mode != 2) std::swap(R.Q, R.Qn);
+}
+
+int runner_slice_enqueue(trhepd_ctx *ctx, Batch &b, const Table &t, int l)
+{
+    Runner &R = *b.run;
+    const Geo &g = R.g;
+    const RknCoef co = rkn_coef(t.method);
+    const int s = co.s;
+    const size_t ntot = g.mat * g.A;
+    const dim3 eb(256), eg((unsigned)((ntot + 255) / 256));
+    const VVi *col1 = t.d + (size_t)(s * (l - 1)) * t.nv;
+    hipLaunchKernelGGL(k_stage1, eg, eb, 0, b.stream, g, b.gmap, col1, (const double2 *)R.g2c, (const int *)R.xid,
+                       co.bab[0] * t.dz, co.bab[1] * t.dz, R.Q, R.P);
+    for (int k = 2; k <= s; ++k)
+        stage_launch(ctx, b, t, k == s ? 1 : 0, t.d + (size_t)(s * (l - 1) + k - 1) * t.nv, co.bab[2 * k - 2] * t.dz,
+                     co.bab[2 * k - 1] * t.dz, nullptr);
+    hipLaunchKernelGGL(k_cond, dim3(g.A), dim3(256), 0, b.stream, g, (const double *)R.colsum,
+                       (const double *)R.diagabs, l == t.ns ? 1 : 0, co.bab[0] * t.dz, R.coef, R.xid, R.ren, b.cnt);
+    TRY_HIP(ctx, hipMemcpyAsync(R.h, R.ren, g.A * 4, hipMemcpyDeviceToHost, b.stream));
+    TRY_HIP(ctx, hipEventRecord(R.evf, b.stream));
+    stage_launch(ctx, b, t, 2, t.d + (size_t)(s * l) * t.nv, 0.0, 0.0, R.coef);
+    TRY_HIP(ctx, hipGetLastError());
+    return TRHEPD_OK;
+}
+
+int runner_slice_finish(trhepd_ctx *ctx, Batch &b, const Table &t)
+{
+    Runner &R = *b.run;
+    const Geo &g = R.g;
+    TRY_HIP(ctx, hipEventSynchronize(R.evf));
+    // compact list of the angles to renormalise, staged in pinned memory: its copy is
+    // queued behind this slice's kernels, and the next slice rewrites the staging area only
+    // after waiting on its own event, which follows this copy in stream order
+    int *lst = R.h + g.A, nl = 0;
+    for (int i = 0; i < g.A; ++i)
+        if (R.h[i]) lst[nl++] = i;
+    if (!nl) return TRHEPD_OK;
+    TRY_HIP(ctx, hipMemcpyAsync(R.list, lst, (size_t)nl * 4, hipMemcpyHostToDevice, b.stream));
+    LuArgs L = R.lu(R.list);
+    L.flag = b.cnt + 1;
+    return lu_on(ctx, b.stream, L, nl);
+}
+
+int runner_end(trhepd_ctx *ctx, Batch &b, const Table &t)
+{
+    Runner &R = *b.run;
+    const Geo &g = R.g;
+    const size_t ntot = g.mat * g.A;
+    const dim3 eb(256), eg((unsigned)((ntot + 255) / 256));
+    hipLaunchKernelGGL(k_st_setup, eg, eb, 0, b.stream, g, (const double2 *)R.gc, R.Q, R.P);
+    LuArgs L = R.lu(R.all);
+    L.flag = b.cnt + 1;
+    int rc = lu_on(ctx, b.stream, L, g.A);
+    if (rc) return rc;
     const size_t nout = (size_t)g.N * g.N * g.A;
-    hipLaunchKernelGGL(k_out, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, ctx->stream, g, (const double2 *)P,
-                       (const double2 *)pgc, d_f, d_f2, nb0, d_wnsga2);
+    hipLaunchKernelGGL(k_out, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, b.stream, g, (const double2 *)R.P,
+                       (const double2 *)R.gc, b.fout, b.has_f2 ? b.f2 : nullptr, b.nb0, (const double *)b.w);
     TRY_HIP(ctx, hipGetLastError());
     return TRHEPD_OK;
 }

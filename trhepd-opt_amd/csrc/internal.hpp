@@ -49,7 +49,10 @@ struct Stats {
     double device_ms = 0.0;
 };
 
-// a batch of angles with device-resident inputs and outputs (one allocation)
+struct Runner;   // working state of the batched large-N path (batched.hip)
+
+// a batch of angles with device-resident inputs, outputs and working memory; each batch
+// runs on its own stream so that several batches overlap on the device
 struct Batch {
     int id = 0, table_id = 0, nb = 0, nang = 0, nb0 = 1;
     bool has_f2 = false;
@@ -59,6 +62,10 @@ struct Batch {
     double2 *gamma = nullptr, *fin = nullptr, *fout = nullptr;
     double *f2 = nullptr, *w = nullptr;
     int *cnt = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    Runner *run = nullptr;
+    Stats stats;
 };
 
 // event pairs bracketing stage-product launches when profiling is on
@@ -109,8 +116,8 @@ int scratch(trhepd_ctx *ctx, int slot, size_t bytes, void **p);
 
 Table *find_table(trhepd_ctx *ctx, int id);
 
-// profiling: record an event before / after a timed launch (no-ops when profiling is off)
-int prof_mark(trhepd_ctx *ctx);
+// profiling: record an event on `stream` before / after a timed launch (no-op when off)
+int prof_mark(trhepd_ctx *ctx, hipStream_t stream);
 // accumulate the elapsed times of the recorded pairs (call after the stream is idle)
 int prof_collect(trhepd_ctx *ctx);
 
@@ -127,11 +134,19 @@ int launch_strfac(trhepd_ctx *ctx, int natm, int nv, const double *gh, const dou
 int launch_table(trhepd_ctx *ctx, const trhepd_potential *pot, Table &t);
 
 // slice-loop drivers
-int run_small(trhepd_ctx *ctx, const Table &t, int nb, int nang, const int *d_gmap,
+int run_small(trhepd_ctx *ctx, hipStream_t stream, const Table &t, int nb, int nang, const int *d_gmap,
               const double2 *d_gamma, double2 *d_f, double *d_f2, int nb0, const double *d_wnsga2,
               int *d_counters);
-int run_batched(trhepd_ctx *ctx, const Table &t, int nb, int nang, const int *d_gmap,
-                const double2 *d_gamma, double2 *d_f, double *d_f2, int nb0,
-                const double *d_wnsga2, int *d_counters);
+// batched large-N path, split into phases so that several batches interleave:
+// runner_create allocates the batch's working memory; begin enqueues the boundary
+// transform; slice_enqueue(l) the stages of slice l up to the end kick (the renormalisation
+// flags are copied to the host behind an event); slice_finish waits for the flags and
+// enqueues the LU solves; end enqueues the back transform and intensities.
+int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t);
+void runner_destroy(trhepd_ctx *ctx, Batch &b);
+int runner_begin(trhepd_ctx *ctx, Batch &b, const Table &t);
+int runner_slice_enqueue(trhepd_ctx *ctx, Batch &b, const Table &t, int l);
+int runner_slice_finish(trhepd_ctx *ctx, Batch &b, const Table &t);
+int runner_end(trhepd_ctx *ctx, Batch &b, const Table &t);
 
 }  // namespace trhepd

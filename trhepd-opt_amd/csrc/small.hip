@@ -435,7 +435,7 @@ __global__ void k_spa1d(SmallArgs A, int npts)
 
 }  // namespace
 
-int run_small(trhepd_ctx *ctx, const Table &t, int nb, int nang, const int *d_gmap,
+int run_small(trhepd_ctx *ctx, hipStream_t stream, const Table &t, int nb, int nang, const int *d_gmap,
               const double2 *d_gamma, double2 *d_f, double *d_f2, int nb0, const double *d_wnsga2,
               int *d_counters)
 {
@@ -455,13 +455,13 @@ int run_small(trhepd_ctx *ctx, const Table &t, int nb, int nang, const int *d_gm
     A.wnsga2 = d_wnsga2;
     A.counters = d_counters;
     if (nb == 1 && t.method == TRHEPD_SP6) {
-        hipLaunchKernelGGL(k_spa1d, dim3((nang + 63) / 64), dim3(64), 0, ctx->stream, A, nang);
+        hipLaunchKernelGGL(k_spa1d, dim3((nang + 63) / 64), dim3(64), 0, stream, A, nang);
     } else if (nb <= 16) {
-        hipLaunchKernelGGL(k_small<16>, dim3(nang), dim3(256), 0, ctx->stream, A);
+        hipLaunchKernelGGL(k_small<16>, dim3(nang), dim3(256), 0, stream, A);
     } else if (nb <= 32) {
-        hipLaunchKernelGGL(k_small<32>, dim3(nang), dim3(256), 0, ctx->stream, A);
+        hipLaunchKernelGGL(k_small<32>, dim3(nang), dim3(256), 0, stream, A);
     } else if (nb <= 48) {
-        hipLaunchKernelGGL(k_small<48>, dim3(nang), dim3(256), 0, ctx->stream, A);
+        hipLaunchKernelGGL(k_small<48>, dim3(nang), dim3(256), 0, stream, A);
     } else {
         return fail(ctx, TRHEPD_E_ARG, "small path supports N <= 48 (got %d)", nb);
     }

@@ -25,7 +25,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 SYMBOLS = ("trhepd_ctx_create", "trhepd_ctx_destroy", "trhepd_last_error", "trhepd_ctx_bytes",
            "trhepd_asfcrr", "trhepd_srfghk", "trhepd_asfcef", "trhepd_scpot_stf",
            "trhepd_table_build", "trhepd_table_shape", "trhepd_table_fetch", "trhepd_table_release",
-           "trhepd_propagate_batch", "trhepd_batch_create", "trhepd_batch_run", "trhepd_batch_fetch",
+           "trhepd_propagate_batch", "trhepd_batch_create", "trhepd_batch_run", "trhepd_batch_run_many", "trhepd_batch_fetch",
            "trhepd_batch_release", "trhepd_last_stats", "trhepd_profile", "trhepd_kernel_stats")
 
 
@@ -79,6 +79,7 @@ def load() -> C.CDLL:
     L.trhepd_last_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), dp]
     L.trhepd_batch_create.argtypes = [vp, i, i, i, vp, vp, vp, i, vp, ip]
     L.trhepd_batch_run.argtypes = [vp, i]
+    L.trhepd_batch_run_many.argtypes = [vp, i, vp]
     L.trhepd_batch_fetch.argtypes = [vp, i, vp, vp]
     L.trhepd_batch_release.argtypes = [vp, i]
     L.trhepd_profile.argtypes = [vp, i]
@@ -259,6 +260,10 @@ class Engine:
 
     def batch_run(self, bid: int):
         self._chk(self._L.trhepd_batch_run(self._h, bid))
+
+    def batch_run_many(self, bids):
+        ids = np.ascontiguousarray(bids, dtype=np.int32)
+        self._chk(self._L.trhepd_batch_run_many(self._h, len(ids), _p(ids)))
 
     def batch_fetch(self, bid: int, want_f: bool = True):
         nang, nb, has_f2 = self._keep_shape[bid]
