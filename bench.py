@@ -64,7 +64,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-slices", type=int, default=8)
-    ap.add_argument("--split", type=int, default=4,
+    ap.add_argument("--renorm-period", type=int, default=0,
+                    help="0: the reference's Gershgorin renormalisation rule; K: every K slices")
+    ap.add_argument("--split", type=int, default=2,
                     help="angle groups run concurrently on separate streams (overlaps the LU chains)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -81,6 +83,7 @@ def main():
     eng = lib.Engine(0, precision=lib.PREC_FP64)
     t0 = time.perf_counter()
     blocks = synth.bulk_reflection(eng, st, layers=2, dz2=dz2, method=method)
+    eng.set_renorm_period(args.renorm_period)
     log("bulk reflection (2 bulk layers, device): %.1f s" % (time.perf_counter() - t0))
     t0 = time.perf_counter()
     job = driver.SurfaceJob(eng, st.bp, st.surf, dz2, method)
@@ -118,7 +121,9 @@ def main():
         "config": {"workload": "synthetic Si surface, N=%d beams, %d angles 0.5-4.4 deg, 12 surface atoms "
                                "(Z=14,8,1), sp4 dz=%.4f ns=%d, FP64 path" % (N, nang, job.dz, job.ns),
                    "path": "fp64", "angles": nang, "beams": N, "slices": job.ns,
-                   "renorm_per_point": ren / (nang * args.steps), "concurrent_batches": len(bids)},
+                   "renorm_per_point": ren / (nang * args.steps), "concurrent_batches": len(bids),
+                   "renorm_rule": ("every %d slices" % args.renorm_period) if args.renorm_period
+                   else "Gershgorin cond_est > 1e3 (reference)"},
         "roofline": {"bound": "mfma", "kernel": "k_stage_gemm (stage product, 8 N^3 flops per angle; "
                                "launches of concurrent batches overlap, so the per-launch duration is an "
                                "upper bound on the kernel's own time)",

@@ -151,6 +151,14 @@ int trhepd_batch_release(trhepd_ctx *ctx, int batch_id);
 int trhepd_last_stats(const trhepd_ctx *ctx, long long *n_renorm, long long *n_products,
                       double *device_ms);
 
+/* Renormalisation schedule of the ratio-form integrators.  The reference renormalises an
+ * angle when the Gershgorin estimate of cond(X^H) exceeds 1e3 (surf_prkn.f90:317-324);
+ * that rule is the default (period = 0).  A period K > 0 renormalises every angle after
+ * every K-th slice instead (and after the last): the change of variables is exact, so
+ * results agree to rounding, and with no data-dependent branch the whole batch is
+ * enqueued without host round trips.  Applies to later batch runs of the context. */
+int trhepd_set_renorm_period(trhepd_ctx *ctx, int period);
+
 /* Kernel timing: when enabled, every stage-product kernel launch is bracketed by HIP
  * events on the context's stream; trhepd_kernel_stats returns the number of such
  * launches and their summed duration (ms) since the last enable. */

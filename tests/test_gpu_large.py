@@ -82,3 +82,18 @@ def test_concurrent_batches_match_single_batch(engine):
     finally:
         engine.table_release(tid)
     assert np.array_equal(f2w, f2p)
+
+
+def test_periodic_renormalisation_n521(engine):
+    surf = O.load_case(os.path.join(REF, "test521"), 0.47, 5)
+    angs, G, F, W = _inputs(surf)
+    tid = engine.table_build(surf.pot, surf.dz, surf.ns, 5)
+    engine.set_renorm_period(2)
+    try:
+        _, f2 = engine.propagate_batch(tid, surf.iv, G, F, surf.nb0, W)
+    finally:
+        engine.set_renorm_period(0)
+        engine.table_release(tid)
+    ref = deckio.read_s(os.path.join(REF, "test521", "perf-5-0.47.txt"))
+    l2 = np.linalg.norm(f2[0] - ref.f2[0]) / np.linalg.norm(ref.f2[0])
+    assert O.rel_err(f2[0], ref.f2[0]) < 1e-8 and l2 < 1e-10

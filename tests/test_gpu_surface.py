@@ -96,3 +96,24 @@ def test_setup_kernels_match_oracle(engine):
     np.testing.assert_allclose(st0.reshape(-1), p["st0"], rtol=0, atol=1e-12)
     st1 = engine.scpot_stf(bp.nsg, 1, 0, 0, 1, gh[:p["nvb"]], gk[:p["nvb"]], p["x"][:nb_], p["y"][:nb_], bp.dx, bp.dy)
     np.testing.assert_allclose(st1.reshape(-1), p["st1"], rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("case,dz,period", [("test23", "0.1", 4), ("test47", "0.1", 6)])
+def test_periodic_renormalisation_matches_reference(engine, case, dz, period):
+    """The renormalisation is an exact change of variables: a fixed schedule (every K
+    slices, all angles) reproduces the reference's FP64 curves to the same 1e-10 bar."""
+    surf = O.load_case(os.path.join(REF, case), float(dz), 5)
+    angs, G, F, W = _inputs(surf)
+    tid = engine.table_build(surf.pot, surf.dz, surf.ns, 5)
+    engine.set_renorm_period(period)
+    try:
+        _, f2 = engine.propagate_batch(tid, surf.iv, G, F, surf.nb0, W)
+        st = engine.last_stats()
+    finally:
+        engine.set_renorm_period(0)
+        engine.table_release(tid)
+    ref = deckio.read_s(os.path.join(REF, case, "perf-5-%s.txt" % dz))
+    e_ref = max(O.rel_err(f2[p], ref.f2[p]) for p in range(len(angs)))
+    print("periodic", period, "renorms", st["n_renorm"], "rel vs reference %.3e" % e_ref)
+    assert st["n_renorm"] == len(angs) * (surf.ns // period + (1 if surf.ns % period else 0))
+    assert e_ref < 1e-10

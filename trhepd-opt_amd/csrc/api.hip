@@ -464,7 +464,7 @@ int trhepd_batch_run_many(trhepd_ctx *ctx, int n, const int *batch_ids)
             rc = runner_begin(ctx, b, *ts[i]);
             L = std::max(L, ts[i]->ns);
         } else {
-            rc = run_small(ctx, b.stream, *ts[i], b.nb, b.nang, b.gmap, b.gamma, b.fout, b.has_f2 ? b.f2 : nullptr,
+            rc = run_small(ctx, b.stream, ctx->renorm_period, *ts[i], b.nb, b.nang, b.gmap, b.gamma, b.fout, b.has_f2 ? b.f2 : nullptr,
                            b.nb0, b.w, b.cnt);
         }
     }
@@ -568,6 +568,14 @@ int trhepd_propagate_batch(trhepd_ctx *ctx, int table_id, int nb, int nang, cons
     if (!rc) rc = trhepd_batch_fetch(ctx, id, f, f2);
     const int rc2 = trhepd_batch_release(ctx, id);
     return rc ? rc : rc2;
+}
+
+int trhepd_set_renorm_period(trhepd_ctx *ctx, int period)
+{
+    if (!ctx) return TRHEPD_E_ARG;
+    if (period < 0) return fail(ctx, TRHEPD_E_ARG, "renormalisation period must be >= 0 (got %d)", period);
+    ctx->renorm_period = period;
+    return TRHEPD_OK;
 }
 
 int trhepd_profile(trhepd_ctx *ctx, int enable)

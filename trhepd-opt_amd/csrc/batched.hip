@@ -324,8 +324,8 @@ __global__ __launch_bounds__(256) void k_stage_gemm(StageArgs S)
 
 // Gershgorin estimate per angle; renormalise when > 1e3 or on the last slice.
 __global__ void k_cond(Geo g, const double *__restrict__ colsum, const double *__restrict__ diagabs, int last,
-                       double cb1dz, double *__restrict__ coef, int *__restrict__ xid, int *__restrict__ ren,
-                       int *__restrict__ counters)
+                       int forced, double cb1dz, double *__restrict__ coef, int *__restrict__ xid,
+                       int *__restrict__ ren, int *__restrict__ counters)
 {
     __shared__ double smn[256], smx[256];
     const int a = blockIdx.x, tid = threadIdx.x;
@@ -352,7 +352,7 @@ __global__ void k_cond(Geo g, const double *__restrict__ colsum, const double *_
         const double d00 = diagabs[(size_t)a * g.NP];
         const double m0 = fmin(d00, smn[0]), m1 = fmax(d00, smx[0]);
         const double d = (m0 < 0.0) ? 1e100 : m1 / m0;
-        const int r = (d > 1e3) || last;
+        const int r = (forced >= 0 ? forced : (d > 1e3)) || last;
         ren[a] = r;
         xid[a] = r;
         coef[a] = r ? cb1dz : 2.0 * cb1dz;
@@ -471,9 +471,8 @@ constexpr int LU_TILE = 64;
 
 __global__ __launch_bounds__(256) void k_lu_step(LuArgs L, int k0)
 {
-    extern __shared__ double2 dyn[];
+    extern __shared__ double2 dyn[];   // [swp 32 x LU_TILE][U 16 x LU_TILE]; later the next panel
     __shared__ double2 L11[16][16];
-    __shared__ double2 U[16][LU_TILE];
     __shared__ int trow[32], tsrc[32], ntouch;
     __shared__ double rbest[4];
     __shared__ int ribest[4];
@@ -482,7 +481,8 @@ __global__ __launch_bounds__(256) void k_lu_step(LuArgs L, int k0)
     const int kw = min(16, N - k0), first = k0 + kw;
     const int c0 = first + blockIdx.x * LU_TILE;
     const int cw = min(LU_TILE, 2 * N - c0);
-    double2 *swp = dyn;   // [32][LU_TILE]
+    double2 *swp = dyn;                                           // [32][LU_TILE]
+    double2 (*U)[LU_TILE] = reinterpret_cast<double2 (*)[LU_TILE]>(dyn + 32 * LU_TILE);   // [16][LU_TILE]
     for (int e = tid; e < 256; e += 256) L11[e >> 4][e & 15] = make_double2(0, 0);
     __syncthreads();
     for (int e = tid; e < kw * kw; e += 256) {
@@ -722,12 +722,13 @@ struct Runner {
     int *h = nullptr;           // pinned: [A] flags, [A] list
     hipEvent_t evf = nullptr;
     double2 *Q, *Qn;            // current / next Q buffer
+    int forced = -1;            // this slice's renormalisation: -1 device decides, 0 no, 1 all
     LuArgs lu(const int *lst) const { return LuArgs{g, Q, P, piv, lst, nullptr, dinv}; }
 };
 
 static int lds_bytes(int N)
 {
-    return (int)std::max((size_t)(N | 1) * 16 * sizeof(double2), (size_t)32 * LU_TILE * sizeof(double2));
+    return (int)std::max((size_t)(N | 1) * 16 * sizeof(double2), (size_t)48 * LU_TILE * sizeof(double2));
 }
 
 int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
@@ -784,11 +785,12 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
     for (int i = 0; i < g.A; ++i) all[i] = i;
     TRY_HIP(ctx, hipMemcpyAsync(R->all, all.data(), g.A * 4, hipMemcpyHostToDevice, b.stream));
     TRY_HIP(ctx, hipStreamSynchronize(b.stream));
-    static bool attr = false;
-    if (!attr) {
-        TRY_HIP(ctx, hipFuncSetAttribute((const void *)k_lu_panel0, hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes(600)));
-        TRY_HIP(ctx, hipFuncSetAttribute((const void *)k_lu_step, hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes(600)));
-        attr = true;
+    // dynamic LDS of the LU kernels grows with N (the panel is N x 16 complex)
+    static int attr_bytes = 0;
+    if (lds_bytes(g.N) > attr_bytes) {
+        TRY_HIP(ctx, hipFuncSetAttribute((const void *)k_lu_panel0, hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes(g.N)));
+        TRY_HIP(ctx, hipFuncSetAttribute((const void *)k_lu_step, hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes(g.N)));
+        attr_bytes = lds_bytes(g.N);
     }
     b.run = R;
     return TRHEPD_OK;
@@ -891,10 +893,16 @@ int runner_slice_enqueue(trhepd_ctx *ctx, Batch &b, const Table &t, int l)
     for (int k = 2; k <= s; ++k)
         stage_launch(ctx, b, t, k == s ? 1 : 0, t.d + (size_t)(s * (l - 1) + k - 1) * t.nv, co.bab[2 * k - 2] * t.dz,
                      co.bab[2 * k - 1] * t.dz, nullptr);
+    // periodic schedule: the decision is known on the host, no flag round trip
+    const int period = ctx->renorm_period;
+    R.forced = period > 0 ? ((l % period == 0) || l == t.ns ? 1 : 0) : -1;
     hipLaunchKernelGGL(k_cond, dim3(g.A), dim3(256), 0, b.stream, g, (const double *)R.colsum,
-                       (const double *)R.diagabs, l == t.ns ? 1 : 0, co.bab[0] * t.dz, R.coef, R.xid, R.ren, b.cnt);
-    TRY_HIP(ctx, hipMemcpyAsync(R.h, R.ren, g.A * 4, hipMemcpyDeviceToHost, b.stream));
-    TRY_HIP(ctx, hipEventRecord(R.evf, b.stream));
+                       (const double *)R.diagabs, l == t.ns ? 1 : 0, R.forced, co.bab[0] * t.dz, R.coef, R.xid,
+                       R.ren, b.cnt);
+    if (R.forced < 0) {
+        TRY_HIP(ctx, hipMemcpyAsync(R.h, R.ren, g.A * 4, hipMemcpyDeviceToHost, b.stream));
+        TRY_HIP(ctx, hipEventRecord(R.evf, b.stream));
+    }
     stage_launch(ctx, b, t, 2, t.d + (size_t)(s * l) * t.nv, 0.0, 0.0, R.coef);
     TRY_HIP(ctx, hipGetLastError());
     return TRHEPD_OK;
@@ -904,6 +912,12 @@ int runner_slice_finish(trhepd_ctx *ctx, Batch &b, const Table &t)
 {
     Runner &R = *b.run;
     const Geo &g = R.g;
+    if (R.forced >= 0) {
+        if (!R.forced) return TRHEPD_OK;
+        LuArgs L = R.lu(R.all);
+        L.flag = b.cnt + 1;
+        return lu_on(ctx, b.stream, L, g.A);
+    }
     TRY_HIP(ctx, hipEventSynchronize(R.evf));
     // compact list of the angles to renormalise, staged in pinned memory: its copy is
     // queued behind this slice's kernels, and the next slice rewrites the staging area only

@@ -93,6 +93,7 @@ struct trhepd_ctx {
     std::vector<trhepd::DevBuf> bufs;
     trhepd::Stats stats;
     int next_batch = 1;
+    int renorm_period = 0;   // 0: reference rule (cond_est > 1e3); K > 0: every K slices
     std::vector<trhepd::Batch> batches;
     trhepd::Prof prof;
     void *tune = nullptr;   // autotuned launch choices
@@ -134,7 +135,7 @@ int launch_strfac(trhepd_ctx *ctx, int natm, int nv, const double *gh, const dou
 int launch_table(trhepd_ctx *ctx, const trhepd_potential *pot, Table &t);
 
 // slice-loop drivers
-int run_small(trhepd_ctx *ctx, hipStream_t stream, const Table &t, int nb, int nang, const int *d_gmap,
+int run_small(trhepd_ctx *ctx, hipStream_t stream, int period, const Table &t, int nb, int nang, const int *d_gmap,
               const double2 *d_gamma, double2 *d_f, double *d_f2, int nb0, const double *d_wnsga2,
               int *d_counters);
 // batched large-N path, split into phases so that several batches interleave:

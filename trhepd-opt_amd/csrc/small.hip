@@ -49,6 +49,7 @@ struct SmallArgs {
     int nb0;                // 1-based
     const double *wnsga2;   // [nang]
     int *counters;          // [0] renorm count, [1] singular flag
+    int period;             // renormalisation period (0: Gershgorin rule)
 };
 
 __device__ __forceinline__ double2 cdiv(double ar, double ai, double br, double bi)
@@ -345,7 +346,7 @@ __global__ __launch_bounds__(256) void k_small(SmallArgs A)
             mx = fmax(mx, S.red1[j]);
         }
         const double d = (mn < 0.0) ? 1e100 : mx / mn;
-        const bool renorm = (d > 1e3) || (l == A.ns);
+        const bool renorm = (A.period > 0 ? (l % A.period == 0) : (d > 1e3)) || (l == A.ns);
         if (A.method != TRHEPD_LF) {
             gemm_acc<NP>(S, N, (renorm ? cb1 : 2.0 * cb1) * dz);
             __syncthreads();
@@ -435,7 +436,7 @@ __global__ void k_spa1d(SmallArgs A, int npts)
 
 }  // namespace
 
-int run_small(trhepd_ctx *ctx, hipStream_t stream, const Table &t, int nb, int nang, const int *d_gmap,
+int run_small(trhepd_ctx *ctx, hipStream_t stream, int period, const Table &t, int nb, int nang, const int *d_gmap,
               const double2 *d_gamma, double2 *d_f, double *d_f2, int nb0, const double *d_wnsga2,
               int *d_counters)
 {
@@ -454,6 +455,7 @@ int run_small(trhepd_ctx *ctx, hipStream_t stream, const Table &t, int nb, int n
     A.nb0 = nb0;
     A.wnsga2 = d_wnsga2;
     A.counters = d_counters;
+    A.period = period;
     if (nb == 1 && t.method == TRHEPD_SP6) {
         hipLaunchKernelGGL(k_spa1d, dim3((nang + 63) / 64), dim3(64), 0, stream, A, nang);
     } else if (nb <= 16) {

@@ -26,7 +26,7 @@ SYMBOLS = ("trhepd_ctx_create", "trhepd_ctx_destroy", "trhepd_last_error", "trhe
            "trhepd_asfcrr", "trhepd_srfghk", "trhepd_asfcef", "trhepd_scpot_stf",
            "trhepd_table_build", "trhepd_table_shape", "trhepd_table_fetch", "trhepd_table_release",
            "trhepd_propagate_batch", "trhepd_batch_create", "trhepd_batch_run", "trhepd_batch_run_many", "trhepd_batch_fetch",
-           "trhepd_batch_release", "trhepd_last_stats", "trhepd_profile", "trhepd_kernel_stats")
+           "trhepd_batch_release", "trhepd_last_stats", "trhepd_set_renorm_period", "trhepd_profile", "trhepd_kernel_stats")
 
 
 class TrhepdError(RuntimeError):
@@ -83,6 +83,7 @@ def load() -> C.CDLL:
     L.trhepd_batch_fetch.argtypes = [vp, i, vp, vp]
     L.trhepd_batch_release.argtypes = [vp, i]
     L.trhepd_profile.argtypes = [vp, i]
+    L.trhepd_set_renorm_period.argtypes = [vp, i]
     L.trhepd_kernel_stats.argtypes = [vp, C.POINTER(C.c_longlong), dp]
     for name in SYMBOLS:
         getattr(L, name).restype = getattr(L, name).restype or C.c_int
@@ -275,6 +276,10 @@ class Engine:
     def batch_release(self, bid: int):
         self._chk(self._L.trhepd_batch_release(self._h, bid))
         self._keep_shape.pop(bid, None)
+
+    def set_renorm_period(self, period: int):
+        """0: the reference's Gershgorin rule; K > 0: every angle after every K-th slice."""
+        self._chk(self._L.trhepd_set_renorm_period(self._h, int(period)))
 
     def profile(self, enable: bool):
         self._chk(self._L.trhepd_profile(self._h, 1 if enable else 0))
