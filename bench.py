@@ -64,8 +64,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-slices", type=int, default=8)
-    ap.add_argument("--renorm-period", type=int, default=0,
-                    help="0: the reference's Gershgorin renormalisation rule; K: every K slices")
+    ap.add_argument("--renorm-period", type=int, default=-1,
+                    help="0: the reference's Gershgorin rule; K: every K slices; -1: K from the evanescent growth rate")
     ap.add_argument("--split", type=int, default=2,
                     help="angle groups run concurrently on separate streams (overlaps the LU chains)")
     args = ap.parse_args()
@@ -122,8 +122,9 @@ def main():
                                "(Z=14,8,1), sp4 dz=%.4f ns=%d, FP64 path" % (N, nang, job.dz, job.ns),
                    "path": "fp64", "angles": nang, "beams": N, "slices": job.ns,
                    "renorm_per_point": ren / (nang * args.steps), "concurrent_batches": len(bids),
-                   "renorm_rule": ("every %d slices" % args.renorm_period) if args.renorm_period
-                   else "Gershgorin cond_est > 1e3 (reference)"},
+                   "renorm_rule": {0: "Gershgorin cond_est > 1e3 (reference)",
+                                   -1: "periodic, K = floor(ln 1e3 / (max|Im gamma| dz))"}.get(
+                       args.renorm_period, "every %d slices" % args.renorm_period)},
         "roofline": {"bound": "mfma", "kernel": "k_stage_gemm (stage product, 8 N^3 flops per angle; "
                                "launches of concurrent batches overlap, so the per-launch duration is an "
                                "upper bound on the kernel's own time)",

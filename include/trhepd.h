@@ -156,7 +156,10 @@ int trhepd_last_stats(const trhepd_ctx *ctx, long long *n_renorm, long long *n_p
  * that rule is the default (period = 0).  A period K > 0 renormalises every angle after
  * every K-th slice instead (and after the last): the change of variables is exact, so
  * results agree to rounding, and with no data-dependent branch the whole batch is
- * enqueued without host round trips.  Applies to later batch runs of the context. */
+ * enqueued without host round trips.  period = -1 derives K per batch from the fastest
+ * evanescent growth: K = max(1, floor(ln(1e3) / (max |Im gamma| * dz))), the number of
+ * slices over which the most strongly growing mode alone gains a factor 1e3.  Applies to
+ * later batch runs of the context. */
 int trhepd_set_renorm_period(trhepd_ctx *ctx, int period);
 
 /* Kernel timing: when enabled, every stage-product kernel launch is bracketed by HIP

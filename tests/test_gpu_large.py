@@ -84,11 +84,13 @@ def test_concurrent_batches_match_single_batch(engine):
     assert np.array_equal(f2w, f2p)
 
 
-def test_periodic_renormalisation_n521(engine):
+def test_auto_periodic_renormalisation_n521(engine):
+    """Automatic period at a coarse dz: the evanescent growth per slice is so fast that the
+    derived period is one slice, as the reference's rule does on this deck."""
     surf = O.load_case(os.path.join(REF, "test521"), 0.47, 5)
     angs, G, F, W = _inputs(surf)
     tid = engine.table_build(surf.pot, surf.dz, surf.ns, 5)
-    engine.set_renorm_period(2)
+    engine.set_renorm_period(-1)
     try:
         _, f2 = engine.propagate_batch(tid, surf.iv, G, F, surf.nb0, W)
     finally:

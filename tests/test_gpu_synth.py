@@ -50,3 +50,26 @@ def test_resident_batch_rerun_is_deterministic(engine):
     finally:
         job.release()
     assert np.array_equal(f2_a, f2_b) and np.array_equal(f_a, f_b)
+
+
+def test_c3_periodic_schedule_vs_reference_rule(engine):
+    """The bench configuration (N = 521, synthetic deck): the automatic periodic
+    renormalisation agrees with the reference's Gershgorin rule to the FP64 bar."""
+    st = synth.config(3)
+    job = driver.SurfaceJob(engine, st.bp, st.surf, 0.02, lib.SP4)
+    try:
+        G, F, W, A = job.inputs(0, None)
+        sel = [0, 13, 26, 39]
+        out = {}
+        for period in (0, -1):
+            engine.set_renorm_period(period)
+            _, f2 = engine.propagate_batch(job.table_id, job.iv, G[sel], F[sel], job.nb0, W[sel])
+            out[period] = (f2, engine.last_stats()["n_renorm"])
+    finally:
+        engine.set_renorm_period(0)
+        job.release()
+    import oracle as O
+    err = max(O.rel_err(out[-1][0][i], out[0][0][i]) for i in range(len(sel)))
+    l2 = max(np.linalg.norm(out[-1][0][i] - out[0][0][i]) / np.linalg.norm(out[0][0][i]) for i in range(len(sel)))
+    print("renorms reference rule %d, periodic %d; rel %.3e L2 %.3e" % (out[0][1], out[-1][1], err, l2))
+    assert l2 < 1e-10 and err < 1e-8

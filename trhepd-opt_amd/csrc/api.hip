@@ -98,6 +98,14 @@ int prof_collect(trhepd_ctx *ctx)
     return TRHEPD_OK;
 }
 
+int batch_period(const trhepd_ctx *ctx, const Batch &b, const Table &t)
+{
+    if (ctx->renorm_period >= 0) return ctx->renorm_period;
+    const double rate = b.gim_max * t.dz;
+    if (!(rate > 0.0)) return t.ns;
+    return std::max(1, (int)std::floor(std::log(1e3) / rate));
+}
+
 Table *find_table(trhepd_ctx *ctx, int id)
 {
     for (Table &t : ctx->tables)
@@ -411,6 +419,7 @@ int trhepd_batch_create(trhepd_ctx *ctx, int table_id, int nb, int nang, const i
     b.nang = nang;
     b.nb0 = nb0;
     b.has_f2 = wnsga2 != nullptr;
+    for (size_t e = 0; e < (size_t)nb * nang; ++e) b.gim_max = std::max(b.gim_max, std::fabs(gamma[2 * e + 1]));
     hipError_t e = hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking);
     if (!e) e = hipEventCreate(&b.ev0);
     if (!e) e = hipEventCreate(&b.ev1);
@@ -464,7 +473,7 @@ int trhepd_batch_run_many(trhepd_ctx *ctx, int n, const int *batch_ids)
             rc = runner_begin(ctx, b, *ts[i]);
             L = std::max(L, ts[i]->ns);
         } else {
-            rc = run_small(ctx, b.stream, ctx->renorm_period, *ts[i], b.nb, b.nang, b.gmap, b.gamma, b.fout, b.has_f2 ? b.f2 : nullptr,
+            rc = run_small(ctx, b.stream, batch_period(ctx, b, *ts[i]), *ts[i], b.nb, b.nang, b.gmap, b.gamma, b.fout, b.has_f2 ? b.f2 : nullptr,
                            b.nb0, b.w, b.cnt);
         }
     }
@@ -573,7 +582,7 @@ int trhepd_propagate_batch(trhepd_ctx *ctx, int table_id, int nb, int nang, cons
 int trhepd_set_renorm_period(trhepd_ctx *ctx, int period)
 {
     if (!ctx) return TRHEPD_E_ARG;
-    if (period < 0) return fail(ctx, TRHEPD_E_ARG, "renormalisation period must be >= 0 (got %d)", period);
+    if (period < -1) return fail(ctx, TRHEPD_E_ARG, "renormalisation period must be >= -1 (got %d)", period);
     ctx->renorm_period = period;
     return TRHEPD_OK;
 }

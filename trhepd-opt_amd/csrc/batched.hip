@@ -894,7 +894,7 @@ int runner_slice_enqueue(trhepd_ctx *ctx, Batch &b, const Table &t, int l)
         stage_launch(ctx, b, t, k == s ? 1 : 0, t.d + (size_t)(s * (l - 1) + k - 1) * t.nv, co.bab[2 * k - 2] * t.dz,
                      co.bab[2 * k - 1] * t.dz, nullptr);
     // periodic schedule: the decision is known on the host, no flag round trip
-    const int period = ctx->renorm_period;
+    const int period = batch_period(ctx, b, t);
     R.forced = period > 0 ? ((l % period == 0) || l == t.ns ? 1 : 0) : -1;
     hipLaunchKernelGGL(k_cond, dim3(g.A), dim3(256), 0, b.stream, g, (const double *)R.colsum,
                        (const double *)R.diagabs, l == t.ns ? 1 : 0, R.forced, co.bab[0] * t.dz, R.coef, R.xid,

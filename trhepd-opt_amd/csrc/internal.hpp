@@ -66,7 +66,11 @@ struct Batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     Runner *run = nullptr;
     Stats stats;
+    double gim_max = 0.0;   // max |Im gamma| over beams and angles (evanescent growth rate)
 };
+
+// renormalisation period of a batch under the context's schedule (0: Gershgorin rule)
+int batch_period(const trhepd_ctx *ctx, const Batch &b, const Table &t);
 
 // event pairs bracketing stage-product launches when profiling is on
 struct Prof {
