@@ -23,6 +23,7 @@ sys.path.insert(0, os.path.join(REPO, "trhepd-opt_amd"))
 
 METRIC = "rocking-curve points/s at N=521 beams, mixed vs FP64 GPU @1e-6 rel; p99 lat"
 PEAK_FP64 = 78.6   # TFLOP/s, MI355X dense FP64 matrix
+PEAK_INT8 = 5000.0   # TOP/s, MI355X dense int8 matrix
 
 
 def log(*a):
@@ -55,36 +56,12 @@ def cpu_baseline(st, blocks, method, dz2, threads, sample_slices):
                                                                    sample_slices)}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--seed", type=int, default=1234)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--cpu-slices", type=int, default=8)
-    ap.add_argument("--renorm-period", type=int, default=-1,
-                    help="0: the reference's Gershgorin rule; K: every K slices; -1: K from the evanescent growth rate")
-    ap.add_argument("--split", type=int, default=2,
-                    help="angle groups run concurrently on separate streams (overlaps the LU chains)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: the slice loop shards by structure and angle with no "
-              "exchange, so the multi-GPU figure is N x the one-GPU value; run with --gpus 1",
-              file=sys.stderr)
-        sys.exit(2)
-
+def run_path(eng, st, blocks, args, method, dz2):
+    """Times `args.steps` full curves of one precision path; returns the figures and the
+    intensities of the last step."""
     import numpy as np
-    from trhepd import driver, lib, synth
-
-    method, dz2 = lib.SP4, 0.02
-    st = synth.config(3, seed=args.seed)
-    eng = lib.Engine(0, precision=lib.PREC_FP64)
-    t0 = time.perf_counter()
-    blocks = synth.bulk_reflection(eng, st, layers=2, dz2=dz2, method=method)
+    from trhepd import driver
     eng.set_renorm_period(args.renorm_period)
-    log("bulk reflection (2 bulk layers, device): %.1f s" % (time.perf_counter() - t0))
     t0 = time.perf_counter()
     job = driver.SurfaceJob(eng, st.bp, st.surf, dz2, method)
     G, F, W, A = job.inputs(0, blocks)
@@ -106,40 +83,105 @@ def main():
     elapsed = time.perf_counter() - t0
     launches, kms = eng.kernel_stats()
     eng.profile(False)
+    f2 = np.concatenate([eng.batch_fetch(b, want_f=False)[1] for b in bids])
+    assert np.all(np.isfinite(f2))
     for b in bids:
-        _, f2 = eng.batch_fetch(b, want_f=False)
-        assert np.all(np.isfinite(f2))
-    value = nang * args.steps / elapsed
-    flops_launch = 8.0 * N ** 3 * nang / len(bids)
-    avg_ms = kms / max(launches, 1)
-    achieved = flops_launch / (avg_ms * 1e-3) / 1e12
-    co = 6
+        eng.batch_release(b)
+    out = dict(value=nang * args.steps / elapsed, ms=1e3 * elapsed / args.steps, launches=launches, kms=kms,
+               f2=f2, ren=ren / (nang * args.steps), N=N, nang=nang, ns=job.ns, dz=job.dz, nb0=job.nb0,
+               per_launch=nang / len(bids), nbatch=len(bids))
+    job.release()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-slices", type=int, default=8)
+    ap.add_argument("--renorm-period", type=int, default=-1,
+                    help="0: the reference's Gershgorin rule; K: every K slices; -1: K from the evanescent growth rate")
+    ap.add_argument("--split", type=int, default=2,
+                    help="angle groups run concurrently on separate streams (overlaps the LU chains)")
+    ap.add_argument("--paths", default="fp64,mixed", help="precision paths to time (fp64, mixed)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the slice loop shards by structure and angle with no "
+              "exchange, so the multi-GPU figure is N x the one-GPU value; run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from trhepd import lib, synth
+
+    method, dz2 = lib.SP4, 0.02
+    st = synth.config(3, seed=args.seed)
+    eng64 = lib.Engine(0, precision=lib.PREC_FP64)
+    t0 = time.perf_counter()
+    blocks = synth.bulk_reflection(eng64, st, layers=2, dz2=dz2, method=method)
+    log("bulk reflection (2 bulk layers, device, FP64): %.1f s" % (time.perf_counter() - t0))
+    paths = [p.strip() for p in args.paths.split(",") if p.strip()]
+    res = {}
+    for p in paths:
+        eng = eng64 if p == "fp64" else lib.Engine(0, precision=lib.PREC_MIXED)
+        log("path %s" % p)
+        res[p] = run_path(eng, st, blocks, args, method, dz2)
+        if eng is not eng64:
+            eng.close()
+    head = res.get("mixed", res.get("fp64"))
+    N, nang = head["N"], head["nang"]
+    rule = {0: "Gershgorin cond_est > 1e3 (reference)",
+            -1: "periodic, K = floor(ln 1e3 / (max|Im gamma| dz))"}.get(args.renorm_period,
+                                                                      "every %d slices" % args.renorm_period)
+    cfg = {"workload": "synthetic Si surface, N=%d beams, %d angles 0.5-4.4 deg, 12 surface atoms (Z=14,8,1), "
+                       "sp4 dz=%.4f ns=%d" % (N, nang, head["dz"], head["ns"]),
+           "angles": nang, "beams": N, "slices": head["ns"], "renorm_rule": rule,
+           "concurrent_batches": head["nbatch"], "paths": {}}
+    for p, r in res.items():
+        cfg["paths"][p] = {"points_per_s": r["value"], "ms_per_step": r["ms"], "renorm_per_point": r["ren"]}
+    if "mixed" in res and "fp64" in res:
+        a, b = res["mixed"]["f2"], res["fp64"]["f2"]
+        mx = np.max(np.abs(b), axis=1, keepdims=True)
+        rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-12 * mx)
+        cfg["mixed_over_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
+        cfg["mixed_max_rel_err_vs_fp64"] = float(rel.max())
+        cfg["mixed_l2_err_vs_fp64"] = float(np.max(np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)))
+
+    def roofline(r, mixed):
+        avg_ms = r["kms"] / max(r["launches"], 1)
+        if mixed:
+            # 15 digit pairs x 3 real products x 2 N^3 int8 ops per angle
+            ops = 90.0 * N ** 3 * r["per_launch"]
+            peak, unit, kern = PEAK_INT8, "TOP/s", "k_stage_i8 (int8 digit-sliced stage product, 90 N^3 int8 ops per angle)"
+        else:
+            ops = 8.0 * N ** 3 * r["per_launch"]
+            peak, unit, kern = PEAK_FP64, "TFLOP/s", "k_stage_gemm (FP64 stage product, 8 N^3 flops per angle)"
+        ach = ops / (avg_ms * 1e-3) / 1e12
+        return {"bound": "mfma", "kernel": kern + "; concurrent batches overlap, so the per-launch duration is an "
+                                                  "upper bound on the kernel's own time",
+                "achieved": ach, "peak": peak, "unit": unit, "frac": ach / peak, "traffic": None,
+                "launches": r["launches"], "avg_launch_ms": avg_ms}
+
     out = {
-        "metric": METRIC, "value": value, "unit": "points/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "synthetic Si surface, N=%d beams, %d angles 0.5-4.4 deg, 12 surface atoms "
-                               "(Z=14,8,1), sp4 dz=%.4f ns=%d, FP64 path" % (N, nang, job.dz, job.ns),
-                   "path": "fp64", "angles": nang, "beams": N, "slices": job.ns,
-                   "renorm_per_point": ren / (nang * args.steps), "concurrent_batches": len(bids),
-                   "renorm_rule": {0: "Gershgorin cond_est > 1e3 (reference)",
-                                   -1: "periodic, K = floor(ln 1e3 / (max|Im gamma| dz))"}.get(
-                       args.renorm_period, "every %d slices" % args.renorm_period)},
-        "roofline": {"bound": "mfma", "kernel": "k_stage_gemm (stage product, 8 N^3 flops per angle; "
-                               "launches of concurrent batches overlap, so the per-launch duration is an "
-                               "upper bound on the kernel's own time)",
-                     "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
-                     "traffic": None, "launches": launches, "avg_launch_ms": avg_ms},
+        "metric": METRIC, "value": head["value"], "unit": "points/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": head["ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "i8 digit-sliced products + f64" if "mixed" in res else "f64", "data": "synthetic",
+        "config": cfg,
+        "roofline": roofline(head, "mixed" in res),
     }
+    if "mixed" in res and "fp64" in res:
+        out["roofline_fp64"] = roofline(res["fp64"], False)
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(st, blocks, method, dz2, args.cpu_threads, args.cpu_slices)
         except Exception as e:   # the CPU leg is reported, never fatal to the GPU figure
             out["cpu_baseline"] = {"value": None, "error": repr(e)}
-    for b in bids:
-        eng.batch_release(b)
-    job.release()
-    eng.close()
+    eng64.close()
     print(json.dumps(out))
 
 

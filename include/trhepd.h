@@ -42,8 +42,10 @@ enum {
 /* arithmetic of the propagator products */
 enum {
     TRHEPD_PREC_FP64 = 0,    /* all-FP64 complex */
-    TRHEPD_PREC_MIXED = 1    /* stage products as split BF16 MFMA with FP32 accumulate,
-                                FP64 state, FP64 table / LU / boundary transforms */
+    TRHEPD_PREC_MIXED = 1    /* large-N stage products as exact int8 digit-sliced MFMA
+                                (five base-2^7 digits per operand, int32 accumulation,
+                                FP64 recombination); FP64 state, table, LU and boundary
+                                transforms.  N <= 48 runs the FP64 path. */
 };
 
 typedef struct trhepd_ctx trhepd_ctx;
@@ -161,6 +163,12 @@ int trhepd_last_stats(const trhepd_ctx *ctx, long long *n_renorm, long long *n_p
  * slices over which the most strongly growing mode alone gains a factor 1e3.  Applies to
  * later batch runs of the context. */
 int trhepd_set_renorm_period(trhepd_ctx *ctx, int period);
+
+/* Digits per operand of the mixed-precision stage products (5, 6 or 7; default 7).  Each
+ * digit adds 7 bits: the normwise error of one stage product is about 2^(1-7 digits) and
+ * the int8 matrix work grows as digits (digits + 1) / 2.  Applies to batches created
+ * afterwards on a TRHEPD_PREC_MIXED context. */
+int trhepd_set_mixed_digits(trhepd_ctx *ctx, int digits);
 
 /* Kernel timing: when enabled, every stage-product kernel launch is bracketed by HIP
  * events on the context's stream; trhepd_kernel_stats returns the number of such

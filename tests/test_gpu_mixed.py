@@ -1,0 +1,80 @@
+"""The mixed-precision path (int8 digit-sliced stage products, FP64 state / LU / table)
+against the FP64 path and the reference's FP64 output.
+
+Gate: per-intensity relative error <= 1e-6, floored at 1e-12 of the per-angle maximum
+(the accuracy metric of the rocking-curve comparison), on the reference's N = 521 deck and
+on the full N = 521 synthetic bench curve (40 angles)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import REF
+from trhepd import deckio, driver, lib, synth
+from test_gpu_surface import _inputs
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(eng, surf, method, G, F, W, period=0):
+    tid = eng.table_build(surf.pot, surf.dz, surf.ns, method)
+    eng.set_renorm_period(period)
+    try:
+        _, f2 = eng.propagate_batch(tid, surf.iv, G, F, surf.nb0, W)
+        st = eng.last_stats()
+    finally:
+        eng.set_renorm_period(0)
+        eng.table_release(tid)
+    return f2, st
+
+
+@pytest.mark.parametrize("dz,method", [("0.68", 5), ("0.1", 5), ("0.68", 6)])
+def test_mixed_n521_vs_fp64_and_reference(engine, mixed_engine, dz, method):
+    print("case test521", dz, method, flush=True)
+    surf = O.load_case(os.path.join(REF, "test521"), float(dz), method)
+    angs, G, F, W = _inputs(surf)
+    f64, _ = _run(engine, surf, method, G, F, W)
+    fmx, st = _run(mixed_engine, surf, method, G, F, W)
+    ref = deckio.read_s(os.path.join(REF, "test521", "perf-%d-%s.txt" % (method, dz)))
+    e64 = max(O.rel_err(fmx[p], f64[p]) for p in range(len(angs)))
+    eref = max(O.rel_err(fmx[p], ref.f2[p]) for p in range(len(angs)))
+    l2 = np.linalg.norm(fmx - f64) / np.linalg.norm(f64)
+    print("mixed vs fp64 %.3e (L2 %.3e)  vs reference %.3e  stats %s" % (e64, l2, eref, st))
+    assert np.all(np.isfinite(fmx))
+    assert e64 < 1e-6 and eref < 1e-6
+
+
+def test_mixed_small_n_is_the_fp64_path(engine, mixed_engine):
+    """Below the large-N threshold the mixed context runs the FP64 small-N kernels."""
+    surf = O.load_case(os.path.join(REF, "test23"), 0.68, 5)
+    angs, G, F, W = _inputs(surf)
+    f64, _ = _run(engine, surf, 5, G, F, W)
+    fmx, _ = _run(mixed_engine, surf, 5, G, F, W)
+    assert np.array_equal(f64, fmx)
+
+
+def test_mixed_c3_full_curve_vs_fp64(engine, mixed_engine):
+    """The bench workload: N = 521, 40 angles, sp4 at dz 0.02, periodic renormalisation."""
+    st = synth.config(3)
+    blocks = synth.bulk_reflection(engine, st, layers=2)
+    out = {}
+    for name, eng in (("fp64", engine), ("mixed", mixed_engine)):
+        job = driver.SurfaceJob(eng, st.bp, st.surf, 0.02, lib.SP4)
+        eng.set_renorm_period(-1)
+        try:
+            G, F, W, A = job.inputs(0, blocks)
+            _, f2 = eng.propagate_batch(job.table_id, job.iv, G, F, job.nb0, W)
+        finally:
+            eng.set_renorm_period(0)
+            job.release()
+        out[name] = f2
+        b00 = job.nb0 - 1
+    f64, fmx = out["fp64"], out["mixed"]
+    errs = np.array([O.rel_err(fmx[p], f64[p]) for p in range(len(f64))])
+    l2 = np.linalg.norm(fmx - f64, axis=1) / np.linalg.norm(f64, axis=1)
+    i00 = np.abs(fmx[:, b00] - f64[:, b00]) / np.abs(f64[:, b00])
+    print("C3 mixed vs fp64: max rel %.3e (median %.3e), max L2 %.3e, (00) curve %.3e"
+          % (errs.max(), np.median(errs), l2.max(), i00.max()))
+    assert np.all(np.isfinite(fmx))
+    assert errs.max() < 1e-6

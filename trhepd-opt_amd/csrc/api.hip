@@ -587,6 +587,14 @@ int trhepd_set_renorm_period(trhepd_ctx *ctx, int period)
     return TRHEPD_OK;
 }
 
+int trhepd_set_mixed_digits(trhepd_ctx *ctx, int digits)
+{
+    if (!ctx) return TRHEPD_E_ARG;
+    if (digits < 5 || digits > 7) return fail(ctx, TRHEPD_E_ARG, "mixed digits must be 5, 6 or 7 (got %d)", digits);
+    ctx->mixed_digits = digits;
+    return TRHEPD_OK;
+}
+
 int trhepd_profile(trhepd_ctx *ctx, int enable)
 {
     if (!ctx) return TRHEPD_E_ARG;

@@ -723,6 +723,11 @@ struct Runner {
     hipEvent_t evf = nullptr;
     double2 *Q, *Qn;            // current / next Q buffer
     int forced = -1;            // this slice's renormalisation: -1 device decides, 0 no, 1 all
+    // sliced-integer stage products (mixed precision): digits of Q and of one slice's M0s
+    bool mixed = false;
+    MixedGeo mg;
+    int8_t *qd = nullptr, *md = nullptr;
+    int *eq = nullptr, *em = nullptr;
     LuArgs lu(const int *lst) const { return LuArgs{g, Q, P, piv, lst, nullptr, dinv}; }
 };
 
@@ -751,6 +756,16 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
                  ocs = take((size_t)g.A * (g.NP / 16) * g.NP * 8), oda = take((size_t)g.A * g.NP * 8),
                  oco = take((size_t)g.A * 8), oxi = take((size_t)g.A * 4), ore = take((size_t)g.A * 4),
                  opv = take((size_t)g.A * g.NP * 4), oli = take((size_t)g.A * 4), oal = take((size_t)g.A * 4);
+    R->mixed = ctx->precision == TRHEPD_PREC_MIXED;
+    size_t oqd = 0, omd = 0, oeq = 0, oem = 0;
+    if (R->mixed) {
+        const int s = rkn_coef(t.method).s;
+        R->mg = mixed_geo(g.A, g.N, g.NP, ctx->mixed_digits);
+        oqd = take(mixed_qd_bytes(R->mg));
+        omd = take(mixed_md_bytes(R->mg) * s);
+        oeq = take((size_t)R->mg.RBP * 16 * 4);
+        oem = take((size_t)R->mg.CBP * 16 * 4 * s);
+    }
     int rc = dev_alloc(ctx, (void **)&R->d, off);
     if (rc) {
         delete R;
@@ -773,6 +788,12 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
     R->piv = (int *)(d + opv);
     R->list = (int *)(d + oli);
     R->all = (int *)(d + oal);
+    if (R->mixed) {
+        R->qd = (int8_t *)(d + oqd);
+        R->md = (int8_t *)(d + omd);
+        R->eq = (int *)(d + oeq);
+        R->em = (int *)(d + oem);
+    }
     hipError_t e = hipHostMalloc((void **)&R->h, (size_t)g.A * 8 + 64, hipHostMallocDefault);
     if (!e) e = hipEventCreateWithFlags(&R->evf, hipEventDisableTiming);
     if (e) {
@@ -848,11 +869,37 @@ int runner_begin(trhepd_ctx *ctx, Batch &b, const Table &t)
     return TRHEPD_OK;
 }
 
+// stage product number `st` of the slice (its digits, on the mixed path, are md[st])
 static void stage_launch(trhepd_ctx *ctx, Batch &b, const Table &t, int mode, const VVi *col, double alpha,
-                         double beta, const double *coef)
+                         double beta, const double *coef, int st)
 {
     Runner &R = *b.run;
     const Geo &g = R.g;
+    if (R.mixed) {
+        launch_qsplit(b.stream, R.mg, R.Q, R.qd, R.eq);
+        MixedStage M;
+        M.g = R.mg;
+        M.qd = R.qd;
+        M.eq = R.eq;
+        M.md = R.md + (size_t)st * mixed_md_bytes(R.mg);
+        M.em = R.em + (size_t)st * R.mg.CBP * 16;
+        M.Q = R.Q;
+        M.Qn = R.Qn;
+        M.P = R.P;
+        M.g2c = R.g2c;
+        M.col = col;
+        M.alpha = alpha;
+        M.beta = beta;
+        M.coef = coef;
+        M.mode = mode;
+        M.colsum = R.colsum;
+        M.diagabs = R.diagabs;
+        prof_mark(ctx, b.stream);
+        launch_stage_i8(b.stream, M);
+        prof_mark(ctx, b.stream);
+        if (mode != 2) std::swap(R.Q, R.Qn);
+        return;
+    }
     const int ggrid = (int)((g.mat + 255) / 256);
     hipLaunchKernelGGL(k_gather, dim3(ggrid), dim3(256), 0, b.stream, g, b.gmap, col, R.M);
     StageArgs S;
@@ -888,11 +935,13 @@ int runner_slice_enqueue(trhepd_ctx *ctx, Batch &b, const Table &t, int l)
     const size_t ntot = g.mat * g.A;
     const dim3 eb(256), eg((unsigned)((ntot + 255) / 256));
     const VVi *col1 = t.d + (size_t)(s * (l - 1)) * t.nv;
+    // the s stage products of slice l use table columns s(l-1)+1 .. s l
+    if (R.mixed) launch_msplit(b.stream, R.mg, b.gmap, col1 + t.nv, t.nv, s, R.md, R.em);
     hipLaunchKernelGGL(k_stage1, eg, eb, 0, b.stream, g, b.gmap, col1, (const double2 *)R.g2c, (const int *)R.xid,
                        co.bab[0] * t.dz, co.bab[1] * t.dz, R.Q, R.P);
     for (int k = 2; k <= s; ++k)
         stage_launch(ctx, b, t, k == s ? 1 : 0, t.d + (size_t)(s * (l - 1) + k - 1) * t.nv, co.bab[2 * k - 2] * t.dz,
-                     co.bab[2 * k - 1] * t.dz, nullptr);
+                     co.bab[2 * k - 1] * t.dz, nullptr, k - 2);
     // periodic schedule: the decision is known on the host, no flag round trip
     const int period = batch_period(ctx, b, t);
     R.forced = period > 0 ? ((l % period == 0) || l == t.ns ? 1 : 0) : -1;
@@ -903,7 +952,7 @@ int runner_slice_enqueue(trhepd_ctx *ctx, Batch &b, const Table &t, int l)
         TRY_HIP(ctx, hipMemcpyAsync(R.h, R.ren, g.A * 4, hipMemcpyDeviceToHost, b.stream));
         TRY_HIP(ctx, hipEventRecord(R.evf, b.stream));
     }
-    stage_launch(ctx, b, t, 2, t.d + (size_t)(s * l) * t.nv, 0.0, 0.0, R.coef);
+    stage_launch(ctx, b, t, 2, t.d + (size_t)(s * l) * t.nv, 0.0, 0.0, R.coef, s - 1);
     TRY_HIP(ctx, hipGetLastError());
     return TRHEPD_OK;
 }

@@ -98,6 +98,7 @@ struct trhepd_ctx {
     trhepd::Stats stats;
     int next_batch = 1;
     int renorm_period = 0;   // 0: reference rule (cond_est > 1e3); K > 0: every K slices
+    int mixed_digits = 7;    // digits per operand of the sliced-integer stage products
     std::vector<trhepd::Batch> batches;
     trhepd::Prof prof;
     void *tune = nullptr;   // autotuned launch choices
@@ -137,6 +138,49 @@ int launch_strfac(trhepd_ctx *ctx, int natm, int nv, const double *gh, const dou
                   const int *ndeg, const double *xb, const double *yb, double dx, double dy,
                   double *st);
 int launch_table(trhepd_ctx *ctx, const trhepd_potential *pot, Table &t);
+
+// Sliced-integer stage products (precision TRHEPD_PREC_MIXED) of the batched large-N path.
+// Operands are split into five signed base-2^7 int8 digits under a power-of-two scale per
+// row of Q and per column of M0; digit products are exact in int32 (v_mfma_i32_16x16x64_i8)
+// and the complex product takes three real products (Qr Mr, Qi Mi, (Qr+Qi)(Mr+Mi)).
+// Digit planes are stored pre-tiled: plane (m, p) holds, for each 16-row (16-column) block
+// and 64-wide k chunk, the 64 lanes x 16 bytes of one MFMA operand fragment.
+struct MixedGeo {
+    int A, N, NP;
+    size_t mat;   // NP*NP
+    int KC;       // 64-wide k chunks, ceil(N / 64)
+    int RB;       // 16-row blocks of [Q_1; ...; Q_A]: A*NP/16
+    int RBP;      // RB rounded up to even
+    int CBP;      // 16-column blocks of M0 (NP/16) rounded up to even
+    int ND;       // digits per operand (5..7)
+};
+MixedGeo mixed_geo(int A, int N, int NP, int ND);
+size_t mixed_qd_bytes(const MixedGeo &g);   // digits of Q: 3 ND planes
+size_t mixed_md_bytes(const MixedGeo &g);   // digits of one stage's M0: 3 ND planes
+struct MixedStage {
+    MixedGeo g;
+    const int8_t *qd;
+    const int *eq;       // [RBP*16] row scale exponents of Q
+    const int8_t *md;
+    const int *em;       // [CBP*16] column scale exponents of M0
+    const double2 *Q;
+    double2 *Qn;
+    double2 *P;
+    const double2 *g2c;
+    const VVi *col;
+    double alpha, beta;
+    const double *coef;
+    int mode;
+    double *colsum, *diagabs;
+};
+// digits of the off-diagonal slice matrices of `nst` consecutive table columns starting at
+// `col0`, into md[st] / em[st]
+void launch_msplit(hipStream_t st, const MixedGeo &g, const int *gmap, const VVi *col0, int nv, int nst,
+                   int8_t *md, int *em);
+// digits of Q (row scales into eq)
+void launch_qsplit(hipStream_t st, const MixedGeo &g, const double2 *Q, int8_t *qd, int *eq);
+// P += alpha (Q M0 + Q diag(d)) from the digits, FP64 epilogue as the FP64 stage kernel
+void launch_stage_i8(hipStream_t st, const MixedStage &S);
 
 // slice-loop drivers
 int run_small(trhepd_ctx *ctx, hipStream_t stream, int period, const Table &t, int nb, int nang, const int *d_gmap,

@@ -26,7 +26,8 @@ SYMBOLS = ("trhepd_ctx_create", "trhepd_ctx_destroy", "trhepd_last_error", "trhe
            "trhepd_asfcrr", "trhepd_srfghk", "trhepd_asfcef", "trhepd_scpot_stf",
            "trhepd_table_build", "trhepd_table_shape", "trhepd_table_fetch", "trhepd_table_release",
            "trhepd_propagate_batch", "trhepd_batch_create", "trhepd_batch_run", "trhepd_batch_run_many", "trhepd_batch_fetch",
-           "trhepd_batch_release", "trhepd_last_stats", "trhepd_set_renorm_period", "trhepd_profile", "trhepd_kernel_stats")
+           "trhepd_batch_release", "trhepd_last_stats", "trhepd_set_renorm_period", "trhepd_set_mixed_digits", "trhepd_profile",
+           "trhepd_kernel_stats")
 
 
 class TrhepdError(RuntimeError):
@@ -84,6 +85,7 @@ def load() -> C.CDLL:
     L.trhepd_batch_release.argtypes = [vp, i]
     L.trhepd_profile.argtypes = [vp, i]
     L.trhepd_set_renorm_period.argtypes = [vp, i]
+    L.trhepd_set_mixed_digits.argtypes = [vp, i]
     L.trhepd_kernel_stats.argtypes = [vp, C.POINTER(C.c_longlong), dp]
     for name in SYMBOLS:
         getattr(L, name).restype = getattr(L, name).restype or C.c_int
@@ -280,6 +282,10 @@ class Engine:
     def set_renorm_period(self, period: int):
         """0: the reference's Gershgorin rule; K > 0: every angle after every K-th slice."""
         self._chk(self._L.trhepd_set_renorm_period(self._h, int(period)))
+
+    def set_mixed_digits(self, digits: int):
+        """Digits per operand of the mixed-precision stage products (5, 6 or 7)."""
+        self._chk(self._L.trhepd_set_mixed_digits(self._h, int(digits)))
 
     def profile(self, enable: bool):
         self._chk(self._L.trhepd_profile(self._h, 1 if enable else 0))
