@@ -17,9 +17,13 @@
  *   - plane-group orbits of the other 16 groups: closure under the group generators and
  *     the International Tables multiplicities (tests/test_oracle_math.py);
  *   - LU solve: residual against random systems and numpy.linalg.solve;
- *   - hp_int_spa1d vs hp_int_spa at N=1 and vs the analytic step-barrier reflection;
- *   - Magnus m2/m4/m6 and rk4_intgs: parity unpinned by a reference vector; checked by
- *     convergence order in dz against sp6 at fine dz and by agreement with sp6.
+ *   - cond_est: Varah's bound (an upper bound on cond_1 for column-diagonally-dominant
+ *     matrices), the diagonal closed form and the 1e100 branch (tests/test_oracle_math.py);
+ *   - hp_int_spa1d: against hp_int_spa run at N=1 on the one-beam Ge(001) sample deck (the
+ *     gap falls as dz^4) and its own order-6 convergence;
+ *   - Magnus m2/m4/m6 and rk4_intgs: parity unpinned by a reference vector; pinned by the
+ *     observed convergence order in dz against sp6 at fine dz (2, 4, 6, 4) and, for
+ *     rk4_intgs, by agreement with rk4 (which the perf-7 goldens pin).
  */
 #include <complex.h>
 #include <math.h>

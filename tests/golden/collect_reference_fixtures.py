@@ -2,7 +2,8 @@
 tests/golden/ref/ (data only: decks, bulkP.b and the recorded FP64 rocking curves).
 
 Source: test/test{23,47,521}/ of the reference repository, produced by its authors'
-builds (see test/test23/setup.sh and perf-test.sh there).  Run once in a container
+builds (see test/test23/setup.sh and perf-test.sh there), and the one-beam sample
+sim-trhepd-rheed/sample/Ge_001_c4x2/ (deck, bulkP.b and the recorded curve, E12.4).  Run once in a container
 where the reference is mounted; the GPU hosts only read the copies.
 """
 import os
@@ -26,4 +27,10 @@ for case, dzs in PICK.items():
         for dz in dzs:
             shutil.copy(os.path.join(SRC, case, "test-results", "perf-%d-%s.txt" % (m, dz)),
                         os.path.join(DST, case, "perf-%d-%s.txt" % (m, dz)))
+GE = os.path.join(os.path.dirname(SRC.rstrip("/")), "sim-trhepd-rheed", "sample", "Ge_001_c4x2")
+os.makedirs(os.path.join(DST, "ge001"), exist_ok=True)
+for f in ("bulk.txt", "surf.txt"):
+    shutil.copy(os.path.join(GE, f), os.path.join(DST, "ge001", f))
+for f in ("bulkP.b", "surf-bulkP.s"):
+    shutil.copy(os.path.join(GE, "output", f), os.path.join(DST, "ge001", f))
 print("copied to", DST)

@@ -29,7 +29,7 @@ def _run(eng, surf, method, G, F, W, period=0):
     return f2, st
 
 
-@pytest.mark.parametrize("dz,method", [("0.68", 5), ("0.1", 5), ("0.68", 6)])
+@pytest.mark.parametrize("dz,method", [("0.68", 5), ("0.47", 5), ("0.68", 6)])
 def test_mixed_n521_vs_fp64_and_reference(engine, mixed_engine, dz, method):
     print("case test521", dz, method, flush=True)
     surf = O.load_case(os.path.join(REF, "test521"), float(dz), method)

@@ -166,129 +166,146 @@ struct StageLaunch {
     int ntiles, ncolt, tiles_per_xcd;
 };
 
-// One wave computes a 32 x 32 output tile (two 16-row blocks of Q times two 16-column
-// blocks of M0); the four waves of a workgroup stack in rows and share the M0 fragments.
-// The three real products run one after the other, each over all k with its fifteen
-// digit pairs accumulated per level, then folded into the FP64 complex accumulators.
-template <int ND>
-__global__ __launch_bounds__(256, ND <= 5 ? 2 : 1) void k_stage_i8(StageLaunch Lc)
+// LDS-tiled form: a workgroup computes 64 rows (4 row blocks, one per wave) x 16 WC
+// columns.  Per 64-wide k chunk of one real product it needs 4 ND fragments of Q digits and
+// WC ND fragments of M0 digits (1 KB each); they are staged global -> registers -> LDS one
+// chunk ahead (double-buffered LDS), so the loads of chunk c+1 overlap the MFMAs of chunk c.
+// The chunks of the three real products form one pipelined sequence of 3 KC steps.
+template <int ND, int WC>
+__global__ __launch_bounds__(256, 1) void k_stage_i8l(StageLaunch Lc)
 {
+    constexpr int NA = 4 * ND, NBF = WC * ND, NF = NA + NBF, NR = (NF + 3) / 4;
+    extern __shared__ v4i lds[];   // [2][NF][64]
     const MixedStage &S = Lc.S;
     const MixedGeo g = S.g;
-    // XCD-aware tile order: the column tiles of one row tile run on the same XCD
     const int L = blockIdx.x;
     const int t = (L & 7) * Lc.tiles_per_xcd + (L >> 3);
     if (t >= Lc.ntiles) return;
     const int rt = t / Lc.ncolt, ct = t % Lc.ncolt;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, kq = lane >> 4;
-    const int rb0 = rt * 8 + wave * 2;
-    if (rb0 >= g.RBP) return;
-    const int cb0 = ct * 2;
+    const int rb0 = rt * 4, cb0 = ct * WC;
     const size_t qplane = (size_t)g.RBP * g.KC * 64, mplane = (size_t)g.CBP * g.KC * 64;
-    const v4i *__restrict__ QD = (const v4i *)S.qd + (size_t)rb0 * g.KC * 64 + lane;
-    const v4i *__restrict__ MD = (const v4i *)S.md + (size_t)cb0 * g.KC * 64 + lane;
-    double re[2][2][4], im[2][2][4];
+    const v4i *__restrict__ QD = (const v4i *)S.qd;
+    const v4i *__restrict__ MD = (const v4i *)S.md;
+    const int nchunk = 3 * g.KC;
+    v4i stage[NR];
+    auto gload = [&](int c) {
+        const int m = c / g.KC, kc = c - m * g.KC;
 #pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-        for (int cb = 0; cb < 2; ++cb)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) re[rb][cb][r] = im[rb][cb][r] = 0.0;
-    for (int m = 0; m < 3; ++m) {
-        v4i acc[ND][2][2];
-#pragma unroll
-        for (int s = 0; s < ND; ++s)
-#pragma unroll
-            for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-                for (int cb = 0; cb < 2; ++cb) acc[s][rb][cb] = v4i{0, 0, 0, 0};
-        const v4i *qa = QD + (size_t)m * ND * qplane;
-        const v4i *mb = MD + (size_t)m * ND * mplane;
-        for (int kc = 0; kc < g.KC; ++kc) {
-            v4i a[ND][2];
-#pragma unroll
-            for (int p = 0; p < ND; ++p)
-#pragma unroll
-                for (int rb = 0; rb < 2; ++rb) a[p][rb] = qa[p * qplane + ((size_t)rb * g.KC + kc) * 64];
-#pragma unroll
-            for (int q = 0; q < ND; ++q) {
-                v4i b[2];
-#pragma unroll
-                for (int cb = 0; cb < 2; ++cb) b[cb] = mb[q * mplane + ((size_t)cb * g.KC + kc) * 64];
-#pragma unroll
-                for (int p = 0; p + q < ND; ++p)
-#pragma unroll
-                    for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-                        for (int cb = 0; cb < 2; ++cb)
-                            acc[p + q][rb][cb] = mfma_i8(b[cb], a[p][rb], acc[p + q][rb][cb]);
+        for (int r = 0; r < NR; ++r) {
+            const int f = r * 4 + wave;
+            if (f < NA) {
+                const int p = f >> 2, rb = f & 3;
+                stage[r] = QD[(size_t)(m * ND + p) * qplane + ((size_t)(rb0 + rb) * g.KC + kc) * 64 + lane];
+            } else if (f < NF) {
+                const int q = (f - NA) / WC, cb = (f - NA) % WC;
+                stage[r] = MD[(size_t)(m * ND + q) * mplane + ((size_t)(cb0 + cb) * g.KC + kc) * 64 + lane];
             }
         }
-        const double sr = m == 2 ? 0.0 : (m == 0 ? 1.0 : -1.0);   // sign into re
-        const double si = m == 2 ? 1.0 : -1.0;                     // sign into im
+    };
+    auto sstore = [&](int buf) {
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb)
+        for (int r = 0; r < NR; ++r) {
+            const int f = r * 4 + wave;
+            if (f < NF) lds[((size_t)buf * NF + f) * 64 + lane] = stage[r];
+        }
+    };
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    v4i acc[ND][WC];
+    double re[WC][4], im[WC][4];
 #pragma unroll
-            for (int cb = 0; cb < 2; ++cb)
+    for (int cb = 0; cb < WC; ++cb) {
+#pragma unroll
+        for (int s = 0; s < ND; ++s) acc[s][cb] = v4i{0, 0, 0, 0};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) re[cb][r] = im[cb][r] = 0.0;
+    }
+    for (int c = 0; c < nchunk; ++c) {
+        if (c + 1 < nchunk) gload(c + 1);
+        const v4i *Lb = lds + (size_t)(c & 1) * NF * 64 + lane;
+        v4i a[ND];
+#pragma unroll
+        for (int p = 0; p < ND; ++p) a[p] = Lb[(p * 4 + wave) * 64];
+#pragma unroll
+        for (int q = 0; q < ND; ++q) {
+            v4i b[WC];
+#pragma unroll
+            for (int cb = 0; cb < WC; ++cb) b[cb] = Lb[(NA + q * WC + cb) * 64];
+#pragma unroll
+            for (int p = 0; p + q < ND; ++p)
+#pragma unroll
+                for (int cb = 0; cb < WC; ++cb) acc[p + q][cb] = mfma_i8(b[cb], a[p], acc[p + q][cb]);
+        }
+        const int m = c / g.KC;
+        if (c - m * g.KC == g.KC - 1) {
+            const double sr = m == 2 ? 0.0 : (m == 0 ? 1.0 : -1.0);
+            const double si = m == 2 ? 1.0 : -1.0;
+#pragma unroll
+            for (int cb = 0; cb < WC; ++cb)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    double T = (double)acc[ND - 1][rb][cb][r];
+                    double T = (double)acc[ND - 1][cb][r];
 #pragma unroll
-                    for (int s = ND - 2; s >= 0; --s) T = fma(T, 0x1p-7, (double)acc[s][rb][cb][r]);
-                    re[rb][cb][r] = fma(sr, T, re[rb][cb][r]);
-                    im[rb][cb][r] = fma(si, T, im[rb][cb][r]);
+                    for (int s = ND - 2; s >= 0; --s) T = fma(T, 0x1p-7, (double)acc[s][cb][r]);
+                    re[cb][r] = fma(sr, T, re[cb][r]);
+                    im[cb][r] = fma(si, T, im[cb][r]);
                 }
+#pragma unroll
+            for (int cb = 0; cb < WC; ++cb)
+#pragma unroll
+                for (int s = 0; s < ND; ++s) acc[s][cb] = v4i{0, 0, 0, 0};
+        }
+        if (c + 1 < nchunk) sstore((c + 1) & 1);
+        __syncthreads();
     }
     // FP64 epilogue: lane (l16, kq) holds row i = block row + l16, columns 4 kq + r
+    if (rb0 + wave >= g.RB) return;
     const double2 w0 = w_upper(S.col[0]);
+    const int R = (rb0 + wave) * 16 + l16;
+    const int a = R / g.NP, i = R % g.NP;
+    const double alpha = S.coef ? S.coef[a] : S.alpha;
+    const int eqr = S.eq[R] - 14;
 #pragma unroll
-    for (int rb = 0; rb < 2; ++rb) {
-        if (rb0 + rb >= g.RB) continue;
-        const int R = (rb0 + rb) * 16 + l16;
-        const int a = R / g.NP, i = R % g.NP;
-        const double alpha = S.coef ? S.coef[a] : S.alpha;
-        const int eqr = S.eq[R] - 14;
+    for (int cb = 0; cb < WC; ++cb) {
+        double csum[4];
 #pragma unroll
-        for (int cb = 0; cb < 2; ++cb) {
-            double csum[4];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                csum[r] = 0.0;
-                const int j = (cb0 + cb) * 16 + 4 * kq + r;
-                if (i < g.N && j < g.N) {
-                    const double sc = ldexp(1.0, eqr + S.em[j]);
-                    const double2 c2 = S.g2c[(size_t)a * g.NP + j];
-                    const double2 dj = make_double2(w0.x - c2.x, w0.y - c2.y);
-                    const size_t idx = (size_t)a * g.mat + i + (size_t)j * g.NP;
-                    const double2 q = S.Q[idx];
-                    const double2 qd = cmul(q, dj);
-                    double2 p = S.P[idx];
-                    p.x += alpha * (re[rb][cb][r] * sc + qd.x);
-                    p.y += alpha * (im[rb][cb][r] * sc + qd.y);
-                    S.P[idx] = p;
-                    if (S.mode != 2) {
-                        const double2 qn = make_double2(q.x + S.beta * p.x, q.y + S.beta * p.y);
-                        S.Qn[idx] = qn;
-                        if (S.mode == 1) {
-                            if (i == j) S.diagabs[(size_t)a * g.NP + j] = hypot(qn.x, qn.y);
-                            else csum[r] = sqrt(qn.x * qn.x + qn.y * qn.y);
-                        }
+        for (int r = 0; r < 4; ++r) {
+            csum[r] = 0.0;
+            const int j = (cb0 + cb) * 16 + 4 * kq + r;
+            if (i < g.N && j < g.N) {
+                const double sc = ldexp(1.0, eqr + S.em[j]);
+                const double2 c2 = S.g2c[(size_t)a * g.NP + j];
+                const double2 dj = make_double2(w0.x - c2.x, w0.y - c2.y);
+                const size_t idx = (size_t)a * g.mat + i + (size_t)j * g.NP;
+                const double2 q = S.Q[idx];
+                const double2 qd = cmul(q, dj);
+                double2 p = S.P[idx];
+                p.x += alpha * (re[cb][r] * sc + qd.x);
+                p.y += alpha * (im[cb][r] * sc + qd.y);
+                S.P[idx] = p;
+                if (S.mode != 2) {
+                    const double2 qn = make_double2(q.x + S.beta * p.x, q.y + S.beta * p.y);
+                    S.Qn[idx] = qn;
+                    if (S.mode == 1) {
+                        if (i == j) S.diagabs[(size_t)a * g.NP + j] = hypot(qn.x, qn.y);
+                        else csum[r] = sqrt(qn.x * qn.x + qn.y * qn.y);
                     }
                 }
             }
-            if (S.mode == 1) {
-                // column sums over the 16 rows of the block: lanes l16 = 0..15 of each kq
+        }
+        if (S.mode == 1) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    double v = csum[r];
-                    v += __shfl_xor(v, 1);
-                    v += __shfl_xor(v, 2);
-                    v += __shfl_xor(v, 4);
-                    v += __shfl_xor(v, 8);
-                    const int j = (cb0 + cb) * 16 + 4 * kq + r;
-                    if (l16 == 0 && j < g.NP) S.colsum[(size_t)(R >> 4) * g.NP + j] = v;
-                }
+            for (int r = 0; r < 4; ++r) {
+                double v = csum[r];
+                v += __shfl_xor(v, 1);
+                v += __shfl_xor(v, 2);
+                v += __shfl_xor(v, 4);
+                v += __shfl_xor(v, 8);
+                const int j = (cb0 + cb) * 16 + 4 * kq + r;
+                if (l16 == 0 && j < g.NP) S.colsum[(size_t)(R >> 4) * g.NP + j] = v;
             }
         }
     }
@@ -306,8 +323,13 @@ MixedGeo mixed_geo(int A, int N, int NP, int ND)
     g.mat = (size_t)NP * NP;
     g.KC = (N + 63) / 64;
     g.RB = A * NP / 16;
-    g.RBP = (g.RB + 1) / 2 * 2;
-    g.CBP = (NP / 16 + 1) / 2 * 2;
+    g.RBP = (g.RB + 3) / 4 * 4;
+    // column blocks per workgroup: the width in {4, 3, 2} that pads the least (wider on ties)
+    const int nb = NP / 16;
+    g.WC = 4;
+    for (int w = 3; w >= 2; --w)
+        if ((nb + w - 1) / w * w < (nb + g.WC - 1) / g.WC * g.WC) g.WC = w;
+    g.CBP = (nb + g.WC - 1) / g.WC * g.WC;
     return g;
 }
 
@@ -333,19 +355,42 @@ void launch_qsplit(hipStream_t st, const MixedGeo &g, const double2 *Q, int8_t *
     }
 }
 
+template <int ND, int WC>
+static void stage_i8_on(hipStream_t st, const StageLaunch &Lc)
+{
+    constexpr int NF = (4 + WC) * ND;
+    const size_t lds = (size_t)2 * NF * 1024;
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)k_stage_i8l<ND, WC>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds);
+        attr = true;
+    }
+    hipLaunchKernelGGL((k_stage_i8l<ND, WC>), dim3(8 * Lc.tiles_per_xcd), dim3(256), lds, st, Lc);
+}
+
+template <int ND>
+static void stage_i8_nd(hipStream_t st, const StageLaunch &Lc)
+{
+    switch (Lc.S.g.WC) {
+    case 2: stage_i8_on<ND, 2>(st, Lc); break;
+    case 3: stage_i8_on<ND, 3>(st, Lc); break;
+    default: stage_i8_on<ND, 4>(st, Lc); break;
+    }
+}
+
 void launch_stage_i8(hipStream_t st, const MixedStage &S)
 {
     StageLaunch Lc;
     Lc.S = S;
-    const int rowt = (S.g.RBP + 7) / 8;
-    Lc.ncolt = S.g.CBP / 2;
+    const int rowt = S.g.RBP / 4;
+    Lc.ncolt = S.g.CBP / S.g.WC;
     Lc.ntiles = rowt * Lc.ncolt;
     Lc.tiles_per_xcd = (Lc.ntiles + 7) / 8;
-    const dim3 grid(8 * Lc.tiles_per_xcd), blk(256);
     switch (S.g.ND) {
-    case 5: hipLaunchKernelGGL(k_stage_i8<5>, grid, blk, 0, st, Lc); break;
-    case 6: hipLaunchKernelGGL(k_stage_i8<6>, grid, blk, 0, st, Lc); break;
-    default: hipLaunchKernelGGL(k_stage_i8<7>, grid, blk, 0, st, Lc); break;
+    case 5: stage_i8_nd<5>(st, Lc); break;
+    case 6: stage_i8_nd<6>(st, Lc); break;
+    default: stage_i8_nd<7>(st, Lc); break;
     }
 }
 
