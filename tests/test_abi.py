@@ -56,3 +56,27 @@ def test_asfcrr_matches_oracle():
 def test_bad_arguments_are_rejected():
     with pytest.raises(lib.TrhepdError):
         lib.srfghk(2, [0], [0], [0, 1, 2], [0, 0, 0])     # nv would exceed nvm
+
+
+FORT = os.path.join(REPO, "trhepd-opt_amd", "build", "fortran", "abi_check")
+
+
+@pytest.mark.skipif(not os.path.exists(FORT), reason="amdflang not available: Fortran bindings not built")
+def test_fortran_bindings_link_and_report_status():
+    """The iso_c_binding module links against the library; without a device the context
+    call returns TRHEPD_E_HIP through the Fortran interface (with one, TRHEPD_OK)."""
+    import subprocess
+    r = subprocess.run([FORT], capture_output=True, text=True, timeout=120)
+    first = r.stdout.splitlines()[0]
+    assert first.startswith("trhepd_ctx_create status ")
+    assert int(first.split()[-1]) in (0, 2)
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(FORT), reason="amdflang not available: Fortran bindings not built")
+def test_fortran_bindings_on_device():
+    import subprocess
+    r = subprocess.run([FORT], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0
+    assert all(line.endswith("status 0") for line in r.stdout.splitlines() if "status" in line)
