@@ -94,6 +94,24 @@ def run_path(eng, st, blocks, args, method, dz2):
     return out
 
 
+def run_serving(args, precision, pool, budget):
+    """The serving workload: a bounded sample of the uneven-size Poisson request stream,
+    served by the library's continuous-batching loop under an HBM budget."""
+    import numpy as np
+    from trhepd import lib, serve
+    reqs = [pool[i % len(pool)] for i in range(args.serve_requests)]
+    arr = serve.poisson_arrivals(len(reqs), args.serve_rate, seed=args.seed)
+    eng = lib.Engine(0, precision=precision, hbm_budget=budget)
+    eng.set_renorm_period(args.renorm_period)
+    try:
+        run = serve.serve(eng, reqs, arrivals=arr)
+    finally:
+        eng.close()
+    out = serve.summary(reqs, run)
+    out["f2"] = [r[1] for r in run["results"]]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +126,11 @@ def main():
     ap.add_argument("--split", type=int, default=2,
                     help="angle groups run concurrently on separate streams (overlaps the LU chains)")
     ap.add_argument("--paths", default="fp64,mixed", help="precision paths to time (fp64, mixed)")
+    ap.add_argument("--serve-requests", type=int, default=24,
+                    help="requests of the serving sample (0: skip the serving measurement)")
+    ap.add_argument("--serve-pool", type=int, default=8, help="distinct seeded structures the requests cycle over")
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="mean arrival rate (requests/s)")
+    ap.add_argument("--serve-budget-gb", type=float, default=24.0, help="HBM budget of the serving context")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the slice loop shards by structure and angle with no "
@@ -176,6 +199,34 @@ def main():
     }
     if "mixed" in res and "fp64" in res:
         out["roofline_fp64"] = roofline(res["fp64"], False)
+    if args.serve_requests > 0:
+        from trhepd import serve
+        t0 = time.perf_counter()
+        pool = serve.request_pool(eng64, args.serve_pool, seed=args.seed)
+        log("serving pool: %d structures, N %s, angles %s (%.1f s)" % (len(pool), [r.nb for r in pool],
+                                                                       [r.nang for r in pool], time.perf_counter() - t0))
+        budget = int(args.serve_budget_gb * (1 << 30))
+        sv = {"workload": "first %d requests of a Poisson stream (mean %.0f req/s) over %d seeded structures with N "
+                          "from {23,47,97,199,307,521}, 3-12 atoms, 8-20 A slabs, 20-56 angles, sp4 dz=0.02; "
+                          "continuous batching under a %.0f GiB HBM budget"
+                          % (args.serve_requests, args.serve_rate, len(pool), args.serve_budget_gb),
+              "beams": [r.nb for r in pool], "angles": [r.nang for r in pool]}
+        f2s = {}
+        for p in paths:
+            r = run_serving(args, lib.PREC_FP64 if p == "fp64" else lib.PREC_MIXED, pool, budget)
+            f2s[p] = r.pop("f2")
+            sv[p] = r
+            log("serving %s: %.2f points/s p50 %.0f ms p99 %.0f ms (peak %d admitted)"
+                % (p, r["points_per_s"], r["p50_ms"], r["p99_ms"], r["peak_admitted"]))
+        if "fp64" in f2s and "mixed" in f2s:
+            errs = []
+            for a, b in zip(f2s["mixed"], f2s["fp64"]):
+                mx = np.max(np.abs(b), axis=1, keepdims=True)
+                errs.append(float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-12 * mx))))
+            sv["mixed_max_rel_err_vs_fp64"] = max(errs)
+            sv["mixed_over_fp64"] = sv["mixed"]["points_per_s"] / sv["fp64"]["points_per_s"]
+        out["serving"] = sv
+        out["p99_ms"] = sv.get("mixed", sv.get("fp64"))["p99_ms"]
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(st, blocks, method, dz2, args.cpu_threads, args.cpu_slices)

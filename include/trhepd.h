@@ -30,7 +30,8 @@ enum {
     TRHEPD_E_BUDGET = 4,     /* request exceeds the context's HBM budget */
     TRHEPD_E_SINGULAR = 5,   /* exact zero pivot in an LU factorisation */
     TRHEPD_E_NONFINITE = 6,  /* NaN/Inf reached an output */
-    TRHEPD_E_NOTFOUND = 7    /* unknown table / request id */
+    TRHEPD_E_NOTFOUND = 7,   /* unknown table / request id */
+    TRHEPD_E_PENDING = 8     /* serving: the request has not finished yet */
 };
 
 /* integrator codes: the int_method values of surflf.f90:28-41 */
@@ -169,6 +170,32 @@ int trhepd_set_renorm_period(trhepd_ctx *ctx, int period);
  * the int8 matrix work grows as digits (digits + 1) / 2.  Applies to batches created
  * afterwards on a TRHEPD_PREC_MIXED context. */
 int trhepd_set_mixed_digits(trhepd_ctx *ctx, int digits);
+
+/* ---- serving: continuous batching of rocking-curve requests --------------------
+ * No counterpart in the reference (its driver runs one deck per process); each request
+ * replaces one angle loop of srfref (surflf.f90:79-153) for one structure.
+ *
+ * trhepd_serve_submit queues a request: the potential descriptor of its structure (as for
+ * trhepd_table_build), dz, ns and int_method (5 or 6; 4 for N <= 48), and the angle batch
+ * (as for trhepd_batch_create; wnsga2 is required).  Every input is copied, so the caller's
+ * arrays may be freed on return.  Nothing touches the device until admission.
+ *
+ * trhepd_serve_step runs one step of the loop: requests whose last kernels have finished
+ * are copied back and their device memory is released; queued requests are admitted in
+ * arrival order while their table and batch fit the context's HBM budget (a request that
+ * cannot fit even with nothing else admitted fails with TRHEPD_E_BUDGET); every admitted
+ * request is advanced by one slice, each on its own stream, so their kernels overlap.
+ * The counts of admitted, queued and finished-but-uncollected requests are returned.
+ *
+ * trhepd_serve_result returns TRHEPD_E_PENDING until the request has finished; then it
+ * copies f (nb, nb, nang) and f2 (nb, nang) (either may be NULL), the latency from submit
+ * to completion in milliseconds, and forgets the request.  A failed request returns its
+ * status and message once. */
+int trhepd_serve_submit(trhepd_ctx *ctx, const trhepd_potential *pot, double dz, int ns, int method,
+                        int nb, int nang, const int *iv, const double *gamma, const double *f,
+                        int nb0, const double *wnsga2, int *request_id);
+int trhepd_serve_step(trhepd_ctx *ctx, int *n_active, int *n_queued, int *n_done);
+int trhepd_serve_result(trhepd_ctx *ctx, int request_id, double *f, double *f2, double *latency_ms);
 
 /* Kernel timing: when enabled, every stage-product kernel launch is bracketed by HIP
  * events on the context's stream; trhepd_kernel_stats returns the number of such

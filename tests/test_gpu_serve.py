@@ -1,0 +1,94 @@
+"""The serving loop (continuous batching of rocking-curve requests).
+
+Each request's result must be the same alone and in a mix with other requests that arrive
+while it runs; the bar is bit-identity, since every request runs its own kernels on its own
+device state and no schedule decision depends on the other requests.  Requests come from the
+seeded uneven-size configuration (N drawn from 23 ... 521), at a coarse dz and a few angles to
+keep the test short."""
+import numpy as np
+import pytest
+
+from trhepd import lib, serve, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _structure_with(nb, seed=11):
+    for i in range(200):
+        st = synth.config(5, seed=seed, index=i)
+        if st.bp.nb[0] == nb:
+            return st
+    raise LookupError(nb)
+
+
+@pytest.fixture(scope="module")
+def reqs(engine):
+    out = []
+    for nb in (23, 97, 199):
+        r = serve.structure_request(engine, _structure_with(nb), dz2=0.1, key="n%d" % nb)
+        k = slice(0, 6)
+        out.append(serve.Request(r.key, r.pot, r.dz, r.ns, r.method, r.iv, r.G[k], r.F[k], r.W[k], r.nb0))
+    return out
+
+
+def _alone(eng, reqs):
+    return [serve.serve(eng, [r])["results"][0][1] for r in reqs]
+
+
+@pytest.mark.parametrize("precision,period", [(lib.PREC_FP64, 0), (lib.PREC_FP64, -1), (lib.PREC_MIXED, -1)])
+def test_same_alone_and_in_the_mix(reqs, precision, period):
+    print("precision", precision, "period", period, flush=True)
+    with lib.Engine(0, precision=precision) as eng:
+        eng.set_renorm_period(period)
+        alone = _alone(eng, reqs)
+        # the larger requests arrive while the first one runs
+        run = serve.serve(eng, reqs, arrivals=[0.0, 0.05, 0.1])
+    mix = [r[1] for r in run["results"]]
+    print("peak admitted", run["peak_active"], "latencies ms", [round(r[2], 1) for r in run["results"]])
+    for a, b in zip(alone, mix):
+        assert np.array_equal(a, b)
+        assert np.all(np.isfinite(a)) and np.all(a >= 0)
+
+
+def test_serving_matches_propagate_batch(engine, reqs):
+    got = _alone(engine, reqs)
+    for r, f2 in zip(reqs, got):
+        tid = engine.table_build(r.pot, r.dz, r.ns, r.method)
+        try:
+            _, ref = engine.propagate_batch(tid, r.iv, r.G, r.F, r.nb0, r.W)
+        finally:
+            engine.table_release(tid)
+        assert np.array_equal(f2, ref)
+
+
+def test_admission_waits_for_the_hbm_budget(engine, reqs):
+    """With a budget that holds the largest request but not two of them, requests are
+    admitted one by one as memory frees up, and still give the same results."""
+    alone = _alone(engine, reqs)
+    r = reqs[-1]
+    with lib.Engine(0) as probe:
+        # the context keeps its staging scratch after the first table build; a request
+        # needs its table and batch on top of that
+        probe.table_release(probe.table_build(r.pot, r.dz, r.ns, r.method))
+        base = probe.bytes_held
+        tid = probe.table_build(r.pot, r.dz, r.ns, r.method)
+        bid = probe.batch_create(tid, r.iv, r.G, r.F, r.nb0, r.W)
+        need = probe.bytes_held - base
+        probe.batch_release(bid)
+    print("request bytes", need, "context scratch", base)
+    with lib.Engine(0, hbm_budget=base + int(need * 1.5)) as eng:
+        run = serve.serve(eng, [reqs[-1], reqs[-1], reqs[1]])
+    assert run["peak_active"] == 1
+    assert np.array_equal(run["results"][0][1], alone[-1])
+    assert np.array_equal(run["results"][1][1], alone[-1])
+    assert np.array_equal(run["results"][2][1], alone[1])
+
+
+def test_request_larger_than_the_budget_fails(reqs):
+    with lib.Engine(0, hbm_budget=1 << 20) as eng:
+        rid = eng.serve_submit(reqs[-1].pot, reqs[-1].dz, reqs[-1].ns, reqs[-1].method, reqs[-1].iv,
+                               reqs[-1].G, reqs[-1].F, reqs[-1].nb0, reqs[-1].W)
+        eng.serve_step()
+        with pytest.raises(lib.TrhepdError) as e:
+            eng.serve_result(rid)
+    assert e.value.code == lib.E_BUDGET

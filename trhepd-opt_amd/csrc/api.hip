@@ -144,6 +144,25 @@ RknCoef rkn_coef(int method)
     return c;
 }
 
+int batch_start(trhepd_ctx *ctx, Batch &b, const Table &t)
+{
+    const size_t nn = (size_t)b.nb * b.nb;
+    b.stats = Stats();
+    TRY_HIP(ctx, hipMemsetAsync(b.cnt, 0, 64, b.stream));
+    TRY_HIP(ctx, hipEventRecord(b.ev0, b.stream));
+    TRY_HIP(ctx, hipMemcpyAsync(b.fout, b.fin, nn * b.nang * 16, hipMemcpyDeviceToDevice, b.stream));
+    if (b.run) return runner_begin(ctx, b, t);
+    return run_small(ctx, b.stream, batch_period(ctx, b, t), t, b.nb, b.nang, b.gmap, b.gamma, b.fout,
+                     b.has_f2 ? b.f2 : nullptr, b.nb0, b.w, b.cnt);
+}
+
+Batch *find_batch(trhepd_ctx *ctx, int id)
+{
+    for (Batch &b : ctx->batches)
+        if (b.id == id) return &b;
+    return nullptr;
+}
+
 }  // namespace trhepd
 
 using namespace trhepd;
@@ -189,6 +208,7 @@ int trhepd_ctx_destroy(trhepd_ctx *ctx)
     if (!ctx) return TRHEPD_E_ARG;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
+    server_destroy(ctx);
     for (Table &t : ctx->tables) dev_free(ctx, t.d, t.bytes);
     for (Batch &b : ctx->batches) {
         runner_destroy(ctx, b);
@@ -365,13 +385,6 @@ int trhepd_table_release(trhepd_ctx *ctx, int table_id)
     return fail(ctx, TRHEPD_E_NOTFOUND, "no table %d", table_id);
 }
 
-static Batch *find_batch(trhepd_ctx *ctx, int id)
-{
-    for (Batch &b : ctx->batches)
-        if (b.id == id) return &b;
-    return nullptr;
-}
-
 int trhepd_batch_create(trhepd_ctx *ctx, int table_id, int nb, int nang, const int *iv, const double *gamma,
                         const double *f, int nb0, const double *wnsga2, int *batch_id)
 {
@@ -463,19 +476,8 @@ int trhepd_batch_run_many(trhepd_ctx *ctx, int n, const int *batch_ids)
     ctx->stats = Stats();
     int rc = TRHEPD_OK, L = 0;
     for (int i = 0; i < n && !rc; ++i) {
-        Batch &b = *bs[i];
-        const size_t nn = (size_t)b.nb * b.nb;
-        b.stats = Stats();
-        TRY_HIP(ctx, hipMemsetAsync(b.cnt, 0, 64, b.stream));
-        TRY_HIP(ctx, hipEventRecord(b.ev0, b.stream));
-        TRY_HIP(ctx, hipMemcpyAsync(b.fout, b.fin, nn * b.nang * 16, hipMemcpyDeviceToDevice, b.stream));
-        if (b.run) {
-            rc = runner_begin(ctx, b, *ts[i]);
-            L = std::max(L, ts[i]->ns);
-        } else {
-            rc = run_small(ctx, b.stream, batch_period(ctx, b, *ts[i]), *ts[i], b.nb, b.nang, b.gmap, b.gamma, b.fout, b.has_f2 ? b.f2 : nullptr,
-                           b.nb0, b.w, b.cnt);
-        }
+        rc = batch_start(ctx, *bs[i], *ts[i]);
+        if (bs[i]->run) L = std::max(L, ts[i]->ns);
     }
     // slices of all large-N batches interleaved: while the host waits for one batch's
     // renormalisation flags the others' kernels are already queued on their streams

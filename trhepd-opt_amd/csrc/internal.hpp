@@ -50,6 +50,7 @@ struct Stats {
 };
 
 struct Runner;   // working state of the batched large-N path (batched.hip)
+struct Server;   // request loop state of the serving entry points (serve.cpp)
 
 // a batch of angles with device-resident inputs, outputs and working memory; each batch
 // runs on its own stream so that several batches overlap on the device
@@ -102,6 +103,7 @@ struct trhepd_ctx {
     std::vector<trhepd::Batch> batches;
     trhepd::Prof prof;
     void *tune = nullptr;   // autotuned launch choices
+    trhepd::Server *serve = nullptr;
 };
 
 namespace trhepd {
@@ -121,6 +123,12 @@ void dev_free(trhepd_ctx *ctx, void *p, size_t bytes);
 int scratch(trhepd_ctx *ctx, int slot, size_t bytes, void **p);
 
 Table *find_table(trhepd_ctx *ctx, int id);
+Batch *find_batch(trhepd_ctx *ctx, int id);
+// starts a batch run on its stream: counters cleared, F reset to the uploaded input, then
+// the boundary transform (large N) or the whole integration (small N)
+int batch_start(trhepd_ctx *ctx, Batch &b, const Table &t);
+
+void server_destroy(trhepd_ctx *ctx);
 
 // profiling: record an event on `stream` before / after a timed launch (no-op when off)
 int prof_mark(trhepd_ctx *ctx, hipStream_t stream);

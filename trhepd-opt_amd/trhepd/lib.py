@@ -17,7 +17,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)                      # trhepd-opt_amd/
 LIB_PATH = os.path.join(ROOT, "lib", "libtrhepd_hip.so")
 
-OK, E_ARG, E_HIP, E_NOMEM, E_BUDGET, E_SINGULAR, E_NONFINITE, E_NOTFOUND = range(8)
+OK, E_ARG, E_HIP, E_NOMEM, E_BUDGET, E_SINGULAR, E_NONFINITE, E_NOTFOUND, E_PENDING = range(9)
 M2, M4, M6, LF, SP4, SP6, RK4, RK4GS = range(1, 9)
 PREC_FP64, PREC_MIXED = 0, 1
 
@@ -27,7 +27,7 @@ SYMBOLS = ("trhepd_ctx_create", "trhepd_ctx_destroy", "trhepd_last_error", "trhe
            "trhepd_table_build", "trhepd_table_shape", "trhepd_table_fetch", "trhepd_table_release",
            "trhepd_propagate_batch", "trhepd_batch_create", "trhepd_batch_run", "trhepd_batch_run_many", "trhepd_batch_fetch",
            "trhepd_batch_release", "trhepd_last_stats", "trhepd_set_renorm_period", "trhepd_set_mixed_digits", "trhepd_profile",
-           "trhepd_kernel_stats")
+           "trhepd_kernel_stats", "trhepd_serve_submit", "trhepd_serve_step", "trhepd_serve_result")
 
 
 class TrhepdError(RuntimeError):
@@ -87,6 +87,9 @@ def load() -> C.CDLL:
     L.trhepd_set_renorm_period.argtypes = [vp, i]
     L.trhepd_set_mixed_digits.argtypes = [vp, i]
     L.trhepd_kernel_stats.argtypes = [vp, C.POINTER(C.c_longlong), dp]
+    L.trhepd_serve_submit.argtypes = [vp, C.POINTER(Potential), d, i, i, i, i, vp, vp, vp, i, vp, ip]
+    L.trhepd_serve_step.argtypes = [vp, ip, ip, ip]
+    L.trhepd_serve_result.argtypes = [vp, i, vp, vp, dp]
     for name in SYMBOLS:
         getattr(L, name).restype = getattr(L, name).restype or C.c_int
     L.trhepd_last_error.restype = C.c_char_p
@@ -199,7 +202,9 @@ class Engine:
         return st
 
     # -- table ---------------------------------------------------------------
-    def table_build(self, pot: dict, dz: float, ns: int, method: int) -> int:
+    @staticmethod
+    def _potential(pot: dict):
+        """The trhepd_potential descriptor of a potential dict, and the arrays it points into."""
         keep = {}
 
         def arr(k, dt):
@@ -212,9 +217,48 @@ class Engine:
                       arr("ghk", np.float64), arr("ocr", np.float64), arr("b", np.float64),
                       arr("asf", np.float64), arr("st0", np.complex128), arr("st1", np.complex128),
                       arr("st2", np.complex128))
+        return P, keep
+
+    def table_build(self, pot: dict, dz: float, ns: int, method: int) -> int:
+        P, keep = self._potential(pot)
         tid = C.c_int(0)
         self._chk(self._L.trhepd_table_build(self._h, C.byref(P), dz, ns, method, C.byref(tid)))
         return tid.value
+
+    # -- serving ---------------------------------------------------------------
+    def serve_submit(self, pot: dict, dz: float, ns: int, method: int, iv, gamma, f, nb0: int, wnsga2) -> int:
+        """Queues one rocking-curve request (conventions of propagate_batch); returns its id."""
+        P, keep = self._potential(pot)
+        nang, nb = gamma.shape
+        ivf = np.asfortranarray(iv, dtype=np.int32)
+        g = np.ascontiguousarray(gamma, dtype=np.complex128)
+        ff = np.ascontiguousarray(np.asarray(f, dtype=np.complex128).transpose(0, 2, 1))
+        w = np.ascontiguousarray(wnsga2, dtype=np.float64)
+        rid = C.c_int(0)
+        self._chk(self._L.trhepd_serve_submit(self._h, C.byref(P), dz, ns, method, nb, nang, _p(ivf), _p(g), _p(ff),
+                                              nb0, _p(w), C.byref(rid)))
+        self._serve_shape = getattr(self, "_serve_shape", {})
+        self._serve_shape[rid.value] = (nang, nb)
+        return rid.value
+
+    def serve_step(self):
+        """One step of the request loop; returns (admitted, queued, finished-uncollected)."""
+        a, q, d = C.c_int(), C.c_int(), C.c_int()
+        self._chk(self._L.trhepd_serve_step(self._h, C.byref(a), C.byref(q), C.byref(d)))
+        return a.value, q.value, d.value
+
+    def serve_result(self, rid: int, want_f: bool = False):
+        """(f or None, f2, latency_ms) of a finished request, or None while it is pending."""
+        nang, nb = self._serve_shape[rid]
+        f = np.zeros((nang, nb, nb), dtype=np.complex128) if want_f else None
+        f2 = np.zeros((nang, nb))
+        lat = C.c_double()
+        rc = self._L.trhepd_serve_result(self._h, rid, _p(f), _p(f2), C.byref(lat))
+        if rc == E_PENDING:
+            return None
+        del self._serve_shape[rid]
+        self._chk(rc)
+        return (None if f is None else f.transpose(0, 2, 1)), f2, lat.value
 
     def table_shape(self, tid: int):
         nv, nc = C.c_int(), C.c_int()
