@@ -126,6 +126,10 @@ def main():
     ap.add_argument("--split", type=int, default=2,
                     help="angle groups run concurrently on separate streams (overlaps the LU chains)")
     ap.add_argument("--paths", default="fp64,mixed", help="precision paths to time (fp64, mixed)")
+    ap.add_argument("--mixed-digits", type=int, default=7,
+                    help="digits per operand of the mixed path's int8 stage products (5, 6 or 7)")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="also run a plain device copy of this many MiB (calibrates HBM byte counters)")
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests of the serving sample (0: skip the serving measurement)")
     ap.add_argument("--serve-pool", type=int, default=8, help="distinct seeded structures the requests cycle over")
@@ -149,8 +153,21 @@ def main():
     log("bulk reflection (2 bulk layers, device, FP64): %.1f s" % (time.perf_counter() - t0))
     paths = [p.strip() for p in args.paths.split(",") if p.strip()]
     res = {}
+    if args.copy_calibration_mb > 0:
+        import torch
+        n = args.copy_calibration_mb << 20
+        src = torch.empty(n, dtype=torch.uint8, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 3 copies of %d MiB" % args.copy_calibration_mb)
+        del src, dst
     for p in paths:
-        eng = eng64 if p == "fp64" else lib.Engine(0, precision=lib.PREC_MIXED)
+        eng = eng64
+        if p != "fp64":
+            eng = lib.Engine(0, precision=lib.PREC_MIXED)
+            eng.set_mixed_digits(args.mixed_digits)
         log("path %s" % p)
         res[p] = run_path(eng, st, blocks, args, method, dz2)
         if eng is not eng64:
@@ -177,9 +194,12 @@ def main():
     def roofline(r, mixed):
         avg_ms = r["kms"] / max(r["launches"], 1)
         if mixed:
-            # 15 digit pairs x 3 real products x 2 N^3 int8 ops per angle
-            ops = 90.0 * N ** 3 * r["per_launch"]
-            peak, unit, kern = PEAK_INT8, "TOP/s", "k_stage_i8 (int8 digit-sliced stage product, 90 N^3 int8 ops per angle)"
+            # nd (nd + 1) / 2 digit pairs x 3 real products x 2 N^3 int8 ops per angle
+            nd = args.mixed_digits
+            ops = 3.0 * nd * (nd + 1) * N ** 3 * r["per_launch"]
+            peak, unit = PEAK_INT8, "TOP/s"
+            kern = ("k_stage_i8l (int8 digit-sliced stage product, %d digits: %d N^3 int8 ops per angle)"
+                    % (nd, 3 * nd * (nd + 1)))
         else:
             ops = 8.0 * N ** 3 * r["per_launch"]
             peak, unit, kern = PEAK_FP64, "TFLOP/s", "k_stage_gemm (FP64 stage product, 8 N^3 flops per angle)"
