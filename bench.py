@@ -126,8 +126,9 @@ def main():
     ap.add_argument("--split", type=int, default=2,
                     help="angle groups run concurrently on separate streams (overlaps the LU chains)")
     ap.add_argument("--paths", default="fp64,mixed", help="precision paths to time (fp64, mixed)")
-    ap.add_argument("--mixed-digits", type=int, default=7,
-                    help="digits per operand of the mixed path's int8 stage products (5, 6 or 7)")
+    ap.add_argument("--mixed-digits", type=int, default=6,
+                    help="digits per operand of the mixed path's int8 stage products (5, 6 or 7); the run "
+                         "checks the mixed intensities against its own FP64 ones")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (calibrates HBM byte counters)")
     ap.add_argument("--serve-requests", type=int, default=24,
@@ -189,6 +190,8 @@ def main():
         rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-12 * mx)
         cfg["mixed_over_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
         cfg["mixed_max_rel_err_vs_fp64"] = float(rel.max())
+        cfg["mixed_within_1e-6"] = bool(rel.max() <= 1e-6)
+        cfg["mixed_digits"] = args.mixed_digits
         cfg["mixed_l2_err_vs_fp64"] = float(np.max(np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)))
 
     def roofline(r, mixed):

@@ -44,8 +44,8 @@ enum {
 enum {
     TRHEPD_PREC_FP64 = 0,    /* all-FP64 complex */
     TRHEPD_PREC_MIXED = 1    /* large-N stage products as exact int8 digit-sliced MFMA
-                                (five base-2^7 digits per operand, int32 accumulation,
-                                FP64 recombination); FP64 state, table, LU and boundary
+                                (base-254 digits per operand, int32 accumulation, FP64
+                                recombination); FP64 state, table, LU and boundary
                                 transforms.  N <= 48 runs the FP64 path. */
 };
 
@@ -165,10 +165,11 @@ int trhepd_last_stats(const trhepd_ctx *ctx, long long *n_renorm, long long *n_p
  * later batch runs of the context. */
 int trhepd_set_renorm_period(trhepd_ctx *ctx, int period);
 
-/* Digits per operand of the mixed-precision stage products (5, 6 or 7; default 7).  Each
- * digit adds 7 bits: the normwise error of one stage product is about 2^(1-7 digits) and
- * the int8 matrix work grows as digits (digits + 1) / 2.  Applies to batches created
- * afterwards on a TRHEPD_PREC_MIXED context. */
+/* Digits per operand of the mixed-precision stage products (5, 6 or 7; default 7).  The
+ * digits are base 254 (int8 range): each adds 8 bits, the normwise error of one stage
+ * product is about digits * 254^-digits, and the int8 matrix work grows as
+ * digits (digits + 1) / 2.  Applies to batches created afterwards on a TRHEPD_PREC_MIXED
+ * context. */
 int trhepd_set_mixed_digits(trhepd_ctx *ctx, int digits);
 
 /* ---- serving: continuous batching of rocking-curve requests --------------------

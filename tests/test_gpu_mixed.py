@@ -54,11 +54,15 @@ def test_mixed_small_n_is_the_fp64_path(engine, mixed_engine):
     assert np.array_equal(f64, fmx)
 
 
-def test_mixed_c3_full_curve_vs_fp64(engine, mixed_engine):
-    """The bench workload: N = 521, 40 angles, sp4 at dz 0.02, periodic renormalisation."""
-    st = synth.config(3)
+@pytest.mark.parametrize("digits", [6, 7])
+def test_mixed_c3_full_curve_vs_fp64(engine, digits):
+    """The bench workload: N = 521, 40 angles, sp4 at dz 0.02, periodic renormalisation;
+    6 digits is what bench.py runs, 7 the library default."""
+    st = synth.config(3, seed=1234)
     blocks = synth.bulk_reflection(engine, st, layers=2)
     out = {}
+    mixed_engine = lib.Engine(0, precision=lib.PREC_MIXED)
+    mixed_engine.set_mixed_digits(digits)
     for name, eng in (("fp64", engine), ("mixed", mixed_engine)):
         job = driver.SurfaceJob(eng, st.bp, st.surf, 0.02, lib.SP4)
         eng.set_renorm_period(-1)
@@ -70,11 +74,12 @@ def test_mixed_c3_full_curve_vs_fp64(engine, mixed_engine):
             job.release()
         out[name] = f2
         b00 = job.nb0 - 1
+    mixed_engine.close()
     f64, fmx = out["fp64"], out["mixed"]
     errs = np.array([O.rel_err(fmx[p], f64[p]) for p in range(len(f64))])
     l2 = np.linalg.norm(fmx - f64, axis=1) / np.linalg.norm(f64, axis=1)
     i00 = np.abs(fmx[:, b00] - f64[:, b00]) / np.abs(f64[:, b00])
-    print("C3 mixed vs fp64: max rel %.3e (median %.3e), max L2 %.3e, (00) curve %.3e"
-          % (errs.max(), np.median(errs), l2.max(), i00.max()))
+    print("C3 mixed (%d digits) vs fp64: max rel %.3e (median %.3e), max L2 %.3e, (00) curve %.3e"
+          % (digits, errs.max(), np.median(errs), l2.max(), i00.max()))
     assert np.all(np.isfinite(fmx))
     assert errs.max() < 1e-6

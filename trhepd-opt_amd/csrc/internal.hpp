@@ -158,9 +158,10 @@ struct MixedGeo {
     size_t mat;   // NP*NP
     int KC;       // 64-wide k chunks, ceil(N / 64)
     int RB;       // 16-row blocks of [Q_1; ...; Q_A]: A*NP/16
-    int RBP;      // RB rounded up to a multiple of 4 (row blocks per workgroup)
+    int RBP;      // RB rounded up to a multiple of 8 (row blocks per workgroup: WR)
     int CBP;      // 16-column blocks of M0 (NP/16) rounded up to a multiple of WC
     int WC;       // column blocks per workgroup tile (2..4)
+    int WR;       // row blocks (waves) per workgroup tile (4 or 8)
     int ND;       // digits per operand (5..7)
 };
 MixedGeo mixed_geo(int A, int N, int NP, int ND);

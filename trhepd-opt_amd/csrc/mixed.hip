@@ -3,16 +3,17 @@
 // The RKN kick P_a += c (Q_a M0 + Q_a diag(d_a)) keeps its FP64 state, FP64 diagonal term
 // and FP64 epilogue; only the dense product Q M0 changes arithmetic.  Each row of Q (each
 // column of M0) is scaled by a power of two 2^-e so that its largest component, taken over
-// the real part, the imaginary part and their sum, is below 127/128, and split into five
-// signed base-2^7 digits
-//     x = sum_{p=0..4} d_p 2^{-7(p+1)} + r,   d_0 in [-127, 127], d_p in [-64, 64], |r| <= 2^-36.
+// the real part, the imaginary part and their sum, is below 1/2, and split into ND signed
+// base-254 digits
+//     x = sum_{p<ND} d_p 254^{-(p+1)} + r,   d_p in [-127, 127],  |r| <= 254^-ND / 2
+// (each digit is the rounded remainder times 254, which uses the whole int8 range).
 // The digit products d_p(Q) d_q(M) are exact int8 x int8 -> int32 MFMAs; the pairs of one
-// level s = p + q (p + q <= 4, fifteen pairs) share one int32 accumulator, whose magnitude
-// stays below 2^25 for k <= 600, so every level sum over k is exact.  Levels are combined
-// in FP64 with weights 2^{-7s}.  The complex product uses three real products
+// level s = p + q (p + q < ND) share one int32 accumulator, whose magnitude stays below
+// 7 * 127^2 * 576 < 2^26 for k <= 576, so every level sum over k is exact.  Levels are
+// combined in FP64 with weights 254^-s.  The complex product uses three real products
 // (T1 = Qr Mr, T2 = Qi Mi, T3 = (Qr+Qi)(Mr+Mi); re = T1 - T2, im = T3 - T1 - T2), all under
 // the same row and column scales, so the combination is exact on the integer sums.
-// The normwise error of one product is about 2^-34 sum_k |Q(i,k)| |M(k,j)|.
+// The normwise error of one product is about ND 254^-ND sum_k |Q(i,k)| |M(k,j)|.
 //
 // Operand roles: the MFMA's A operand is the M0 digit fragment (its row = output column j)
 // and its B operand the Q digit fragment (its column = output row i), so that the 16 lanes
@@ -40,13 +41,13 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b)
 __device__ __forceinline__ double2 w_upper(const VVi &t) { return make_double2(-(t.v.x + t.vi.x), t.v.y + t.vi.y); }
 __device__ __forceinline__ double2 w_lower(const VVi &t) { return make_double2(-t.v.x + t.vi.x, -t.v.y + t.vi.y); }
 
-// exponent e with max * 2^-e < 127/128 (0 for an all-zero row)
+// exponent e with max * 2^-e < 1/2 (0 for an all-zero row)
 __device__ __forceinline__ int scale_exp(double mx)
 {
     if (!(mx > 0.0)) return 0;
     int e;
-    const double f = frexp(mx, &e);
-    return f >= 127.0 / 128.0 ? e + 1 : e;
+    (void)frexp(mx, &e);   // mx = f 2^e, f in [1/2, 1)
+    return e + 1;
 }
 
 __device__ __forceinline__ double cmax3(double2 v)
@@ -54,16 +55,16 @@ __device__ __forceinline__ double cmax3(double2 v)
     return fmax(fabs(v.x), fmax(fabs(v.y), fabs(v.x + v.y)));
 }
 
-// the ND leading digits of x (|x| < 127/128)
+// the ND leading base-254 digits of x (|x| < 1/2): every |y| <= 127, so every digit fits
 template <int ND>
 __device__ __forceinline__ void digits(double x, int *d)
 {
-    double y = x * 128.0;
+    double y = x * 254.0;
 #pragma unroll
     for (int p = 0; p < ND; ++p) {
         const double r = rint(y);
         d[p] = (int)r;
-        y = (y - r) * 128.0;
+        y = (y - r) * 254.0;
     }
 }
 
@@ -166,15 +167,17 @@ struct StageLaunch {
     int ntiles, ncolt, tiles_per_xcd;
 };
 
-// LDS-tiled form: a workgroup computes 64 rows (4 row blocks, one per wave) x 16 WC
-// columns.  Per 64-wide k chunk of one real product it needs 4 ND fragments of Q digits and
-// WC ND fragments of M0 digits (1 KB each); they are staged global -> registers -> LDS one
-// chunk ahead (double-buffered LDS), so the loads of chunk c+1 overlap the MFMAs of chunk c.
-// The chunks of the three real products form one pipelined sequence of 3 KC steps.
-template <int ND, int WC>
-__global__ __launch_bounds__(256, 1) void k_stage_i8l(StageLaunch Lc)
+// LDS-tiled form: a workgroup of WR waves computes 16 WR rows (one 16-row block per wave)
+// x 16 WC columns.  Per 64-wide k chunk of one real product it needs WR ND fragments of Q
+// digits and WC ND fragments of M0 digits (1 KB each); they are staged global -> registers
+// -> LDS one chunk ahead (double-buffered LDS), so the loads of chunk c+1 overlap the MFMAs
+// of chunk c.  The M0 fragments are shared by all WR waves: the taller the tile, the fewer
+// times the M0 digits stream through L2.  The chunks of the three real products form one
+// pipelined sequence of 3 KC steps.
+template <int ND, int WC, int WR>
+__global__ __launch_bounds__(64 * WR, 1) void k_stage_i8l(StageLaunch Lc)
 {
-    constexpr int NA = 4 * ND, NBF = WC * ND, NF = NA + NBF, NR = (NF + 3) / 4;
+    constexpr int NA = WR * ND, NBF = WC * ND, NF = NA + NBF, NR = (NF + WR - 1) / WR;
     extern __shared__ v4i lds[];   // [2][NF][64]
     const MixedStage &S = Lc.S;
     const MixedGeo g = S.g;
@@ -184,7 +187,7 @@ __global__ __launch_bounds__(256, 1) void k_stage_i8l(StageLaunch Lc)
     const int rt = t / Lc.ncolt, ct = t % Lc.ncolt;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, kq = lane >> 4;
-    const int rb0 = rt * 4, cb0 = ct * WC;
+    const int rb0 = rt * WR, cb0 = ct * WC;
     const size_t qplane = (size_t)g.RBP * g.KC * 64, mplane = (size_t)g.CBP * g.KC * 64;
     const v4i *__restrict__ QD = (const v4i *)S.qd;
     const v4i *__restrict__ MD = (const v4i *)S.md;
@@ -194,9 +197,9 @@ __global__ __launch_bounds__(256, 1) void k_stage_i8l(StageLaunch Lc)
         const int m = c / g.KC, kc = c - m * g.KC;
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
-            const int f = r * 4 + wave;
+            const int f = r * WR + wave;
             if (f < NA) {
-                const int p = f >> 2, rb = f & 3;
+                const int p = f / WR, rb = f % WR;
                 stage[r] = QD[(size_t)(m * ND + p) * qplane + ((size_t)(rb0 + rb) * g.KC + kc) * 64 + lane];
             } else if (f < NF) {
                 const int q = (f - NA) / WC, cb = (f - NA) % WC;
@@ -207,7 +210,7 @@ __global__ __launch_bounds__(256, 1) void k_stage_i8l(StageLaunch Lc)
     auto sstore = [&](int buf) {
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
-            const int f = r * 4 + wave;
+            const int f = r * WR + wave;
             if (f < NF) lds[((size_t)buf * NF + f) * 64 + lane] = stage[r];
         }
     };
@@ -228,7 +231,7 @@ __global__ __launch_bounds__(256, 1) void k_stage_i8l(StageLaunch Lc)
         const v4i *Lb = lds + (size_t)(c & 1) * NF * 64 + lane;
         v4i a[ND];
 #pragma unroll
-        for (int p = 0; p < ND; ++p) a[p] = Lb[(p * 4 + wave) * 64];
+        for (int p = 0; p < ND; ++p) a[p] = Lb[(p * WR + wave) * 64];
 #pragma unroll
         for (int q = 0; q < ND; ++q) {
             v4i b[WC];
@@ -249,7 +252,7 @@ __global__ __launch_bounds__(256, 1) void k_stage_i8l(StageLaunch Lc)
                 for (int r = 0; r < 4; ++r) {
                     double T = (double)acc[ND - 1][cb][r];
 #pragma unroll
-                    for (int s = ND - 2; s >= 0; --s) T = fma(T, 0x1p-7, (double)acc[s][cb][r]);
+                    for (int s = ND - 2; s >= 0; --s) T = fma(T, 1.0 / 254.0, (double)acc[s][cb][r]);
                     re[cb][r] = fma(sr, T, re[cb][r]);
                     im[cb][r] = fma(si, T, im[cb][r]);
                 }
@@ -267,7 +270,7 @@ __global__ __launch_bounds__(256, 1) void k_stage_i8l(StageLaunch Lc)
     const int R = (rb0 + wave) * 16 + l16;
     const int a = R / g.NP, i = R % g.NP;
     const double alpha = S.coef ? S.coef[a] : S.alpha;
-    const int eqr = S.eq[R] - 14;
+    const int eqr = S.eq[R];
 #pragma unroll
     for (int cb = 0; cb < WC; ++cb) {
         double csum[4];
@@ -276,7 +279,7 @@ __global__ __launch_bounds__(256, 1) void k_stage_i8l(StageLaunch Lc)
             csum[r] = 0.0;
             const int j = (cb0 + cb) * 16 + 4 * kq + r;
             if (i < g.N && j < g.N) {
-                const double sc = ldexp(1.0, eqr + S.em[j]);
+                const double sc = ldexp(1.0 / (254.0 * 254.0), eqr + S.em[j]);
                 const double2 c2 = S.g2c[(size_t)a * g.NP + j];
                 const double2 dj = make_double2(w0.x - c2.x, w0.y - c2.y);
                 const size_t idx = (size_t)a * g.mat + i + (size_t)j * g.NP;
@@ -323,13 +326,15 @@ MixedGeo mixed_geo(int A, int N, int NP, int ND)
     g.mat = (size_t)NP * NP;
     g.KC = (N + 63) / 64;
     g.RB = A * NP / 16;
-    g.RBP = (g.RB + 3) / 4 * 4;
+    g.RBP = (g.RB + 7) / 8 * 8;
     // column blocks per workgroup: the width in {4, 3, 2} that pads the least (wider on ties)
     const int nb = NP / 16;
     g.WC = 4;
     for (int w = 3; w >= 2; --w)
         if ((nb + w - 1) / w * w < (nb + g.WC - 1) / g.WC * g.WC) g.WC = w;
     g.CBP = (nb + g.WC - 1) / g.WC * g.WC;
+    // eight row blocks per workgroup when the double-buffered tile fits the 160 KB of LDS
+    g.WR = 2 * (8 + g.WC) * ND <= 160 ? 8 : 4;
     return g;
 }
 
@@ -355,27 +360,37 @@ void launch_qsplit(hipStream_t st, const MixedGeo &g, const double2 *Q, int8_t *
     }
 }
 
-template <int ND, int WC>
+template <int ND, int WC, int WR>
 static void stage_i8_on(hipStream_t st, const StageLaunch &Lc)
 {
-    constexpr int NF = (4 + WC) * ND;
+    constexpr int NF = (WR + WC) * ND;
     const size_t lds = (size_t)2 * NF * 1024;
+    static_assert(2 * NF <= 160, "tile exceeds the LDS");
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute((const void *)k_stage_i8l<ND, WC>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        (void)hipFuncSetAttribute((const void *)k_stage_i8l<ND, WC, WR>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds);
         attr = true;
     }
-    hipLaunchKernelGGL((k_stage_i8l<ND, WC>), dim3(8 * Lc.tiles_per_xcd), dim3(256), lds, st, Lc);
+    hipLaunchKernelGGL((k_stage_i8l<ND, WC, WR>), dim3(8 * Lc.tiles_per_xcd), dim3(64 * WR), lds, st, Lc);
+}
+
+template <int ND, int WC>
+static void stage_i8_wr(hipStream_t st, const StageLaunch &Lc)
+{
+    if constexpr (2 * (8 + WC) * ND <= 160) {
+        if (Lc.S.g.WR == 8) return stage_i8_on<ND, WC, 8>(st, Lc);
+    }
+    stage_i8_on<ND, WC, 4>(st, Lc);
 }
 
 template <int ND>
 static void stage_i8_nd(hipStream_t st, const StageLaunch &Lc)
 {
     switch (Lc.S.g.WC) {
-    case 2: stage_i8_on<ND, 2>(st, Lc); break;
-    case 3: stage_i8_on<ND, 3>(st, Lc); break;
-    default: stage_i8_on<ND, 4>(st, Lc); break;
+    case 2: stage_i8_wr<ND, 2>(st, Lc); break;
+    case 3: stage_i8_wr<ND, 3>(st, Lc); break;
+    default: stage_i8_wr<ND, 4>(st, Lc); break;
     }
 }
 
@@ -383,7 +398,7 @@ void launch_stage_i8(hipStream_t st, const MixedStage &S)
 {
     StageLaunch Lc;
     Lc.S = S;
-    const int rowt = S.g.RBP / 4;
+    const int rowt = S.g.RBP / S.g.WR;
     Lc.ncolt = S.g.CBP / S.g.WC;
     Lc.ntiles = rowt * Lc.ncolt;
     Lc.tiles_per_xcd = (Lc.ntiles + 7) / 8;
