@@ -455,34 +455,33 @@ __device__ void factor_panel(const LuArgs &L, int a, int k0, double2 *pan, doubl
     }
 }
 
-__global__ __launch_bounds__(256) void k_lu_panel0(LuArgs L)
+// factors the panel [k0, k0+16) of every listed angle (one workgroup per angle)
+__global__ __launch_bounds__(256) void k_lu_panel(LuArgs L, int k0)
 {
     extern __shared__ double2 dyn[];
     __shared__ double rbest[4];
     __shared__ int ribest[4];
-    factor_panel(L, L.list[blockIdx.x], 0, dyn, rbest, ribest);
+    factor_panel(L, L.list[blockIdx.x], k0, dyn, rbest, ribest);
 }
 
 // One elimination step for a 64-column tile right of panel [k0, k0+kw): the panel's row
 // interchanges (gathered as one permutation of the <= 32 rows they touch), U12 =
-// L11^{-1} A12, A22 -= L21 U12 on the matrix cores.  The tile that holds the next panel's
-// columns then factors that panel, so each step is one launch.
+// L11^{-1} A12, A22 -= L21 U12 on the matrix cores.  The next panel (the first 16 columns
+// of tile 0) is factored by k_lu_panel after this launch; keeping the panel's LDS out of
+// this kernel lets several tiles share a CU.
 constexpr int LU_TILE = 64;
 
 __global__ __launch_bounds__(256) void k_lu_step(LuArgs L, int k0)
 {
-    extern __shared__ double2 dyn[];   // [swp 32 x LU_TILE][U 16 x LU_TILE]; later the next panel
+    __shared__ double2 swp[32 * LU_TILE];
+    __shared__ double2 U[16][LU_TILE];
     __shared__ double2 L11[16][16];
     __shared__ int trow[32], tsrc[32], ntouch;
-    __shared__ double rbest[4];
-    __shared__ int ribest[4];
     const int a = L.list[blockIdx.y];
     const int N = L.g.N, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int kw = min(16, N - k0), first = k0 + kw;
     const int c0 = first + blockIdx.x * LU_TILE;
     const int cw = min(LU_TILE, 2 * N - c0);
-    double2 *swp = dyn;                                           // [32][LU_TILE]
-    double2 (*U)[LU_TILE] = reinterpret_cast<double2 (*)[LU_TILE]>(dyn + 32 * LU_TILE);   // [16][LU_TILE]
     for (int e = tid; e < 256; e += 256) L11[e >> 4][e & 15] = make_double2(0, 0);
     __syncthreads();
     for (int e = tid; e < kw * kw; e += 256) {
@@ -556,55 +555,60 @@ __global__ __launch_bounds__(256) void k_lu_step(LuArgs L, int k0)
     const int nrb = (N - first + 15) / 16;
     const int ks = (kw + 3) >> 2;
     const int ncb = (cw + 15) / 16;
-    for (int rb = wave; rb < nrb; rb += 4) {
-        const int i0 = first + rb * 16, ia = i0 + l16;
-        d4 cr[4], ci[4];
-#pragma unroll
-        for (int cb = 0; cb < 4; ++cb) { cr[cb] = d4{0, 0, 0, 0}; ci[cb] = d4{0, 0, 0, 0}; }
-        double2 avs[4];
+    // A22 -= L21 U12: wave w owns the 16-column block w of the tile and walks down all row
+    // blocks two at a time (independent accumulators); its U12 operands stay in registers
+    const int cb = wave;
+    if (cb < ncb) {
+        const int c = cb * 16 + l16;
+        double2 uv[4];
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int k = s * 4 + kq;
-            avs[s] = (s < ks && k < kw && ia < N) ? colp(L, a, k0 + k)[ia] : make_double2(0, 0);
+            uv[s] = (s < ks && k < kw) ? U[k][c] : make_double2(0, 0);
         }
+        double2 *col = colp(L, a, c0 + (c < cw ? c : 0));
+        for (int rb = 0; rb < nrb; rb += 2) {
+            const int i0 = first + rb * 16;
+            d4 cr[2], ci[2];
+            double2 av[2][4];
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            if (s >= ks) break;
-            const int k = s * 4 + kq;
-            const double2 av = avs[s];
+            for (int h = 0; h < 2; ++h) {
+                cr[h] = d4{0, 0, 0, 0};
+                ci[h] = d4{0, 0, 0, 0};
+                const int ia = i0 + h * 16 + l16;
 #pragma unroll
-            for (int cb = 0; cb < 4; ++cb) {
-                if (cb < ncb) {
-                    const double2 bv = U[k][cb * 16 + l16];
-                    cr[cb] = mfma4(av.x, bv.x, cr[cb]);
-                    ci[cb] = mfma4(av.x, bv.y, ci[cb]);
-                    cr[cb] = mfma4(-av.y, bv.y, cr[cb]);
-                    ci[cb] = mfma4(av.y, bv.x, ci[cb]);
+                for (int s = 0; s < 4; ++s) {
+                    const int k = s * 4 + kq;
+                    av[h][s] = (s < ks && k < kw && ia < N) ? colp(L, a, k0 + k)[ia] : make_double2(0, 0);
                 }
             }
-        }
 #pragma unroll
-        for (int cb = 0; cb < 4; ++cb) {
-            const int c = cb * 16 + l16;
-            if (cb < ncb && c < cw) {
-                double2 *col = colp(L, a, c0 + c);
+            for (int s = 0; s < 4; ++s) {
+                if (s >= ks) break;
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int i = i0 + kq + 4 * r;
-                    if (i < N) {
-                        double2 v = col[i];
-                        v.x -= cr[cb][r];
-                        v.y -= ci[cb][r];
-                        col[i] = v;
+                for (int h = 0; h < 2; ++h) cr[h] = mfma4(av[h][s].x, uv[s].x, cr[h]);
+#pragma unroll
+                for (int h = 0; h < 2; ++h) ci[h] = mfma4(av[h][s].x, uv[s].y, ci[h]);
+#pragma unroll
+                for (int h = 0; h < 2; ++h) cr[h] = mfma4(-av[h][s].y, uv[s].y, cr[h]);
+#pragma unroll
+                for (int h = 0; h < 2; ++h) ci[h] = mfma4(av[h][s].y, uv[s].x, ci[h]);
+            }
+            if (c < cw) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int i = i0 + h * 16 + kq + 4 * r;
+                        if (i < N) {
+                            double2 v = col[i];
+                            v.x -= cr[h][r];
+                            v.y -= ci[h][r];
+                            col[i] = v;
+                        }
                     }
-                }
             }
         }
-    }
-    if (blockIdx.x == 0 && k0 + 16 < N) {
-        __threadfence();
-        __syncthreads();
-        factor_panel(L, a, k0 + 16, dyn, rbest, ribest);
     }
 }
 
@@ -731,10 +735,8 @@ struct Runner {
     LuArgs lu(const int *lst) const { return LuArgs{g, Q, P, piv, lst, nullptr, dinv}; }
 };
 
-static int lds_bytes(int N)
-{
-    return (int)std::max((size_t)(N | 1) * 16 * sizeof(double2), (size_t)48 * LU_TILE * sizeof(double2));
-}
+// dynamic LDS of the panel factorisation: the panel, 16 columns of N rows (odd pitch)
+static int lds_bytes(int N) { return (int)((size_t)(N | 1) * 16 * sizeof(double2)); }
 
 int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
 {
@@ -809,8 +811,7 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
     // dynamic LDS of the LU kernels grows with N (the panel is N x 16 complex)
     static int attr_bytes = 0;
     if (lds_bytes(g.N) > attr_bytes) {
-        TRY_HIP(ctx, hipFuncSetAttribute((const void *)k_lu_panel0, hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes(g.N)));
-        TRY_HIP(ctx, hipFuncSetAttribute((const void *)k_lu_step, hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes(g.N)));
+        TRY_HIP(ctx, hipFuncSetAttribute((const void *)k_lu_panel, hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes(g.N)));
         attr_bytes = lds_bytes(g.N);
     }
     b.run = R;
@@ -833,11 +834,12 @@ static int lu_on(trhepd_ctx *ctx, hipStream_t st, LuArgs L, int npts)
     if (npts == 0) return TRHEPD_OK;
     const int N = L.g.N;
     const size_t lds = lds_bytes(N);
-    hipLaunchKernelGGL(k_lu_panel0, dim3(npts), dim3(256), lds, st, L);
+    hipLaunchKernelGGL(k_lu_panel, dim3(npts), dim3(256), lds, st, L, 0);
     for (int k0 = 0; k0 < N; k0 += 16) {
         const int kw = std::min(16, N - k0);
         const int ntile = (2 * N - k0 - kw + LU_TILE - 1) / LU_TILE;
-        hipLaunchKernelGGL(k_lu_step, dim3(ntile, npts), dim3(256), lds, st, L, k0);
+        hipLaunchKernelGGL(k_lu_step, dim3(ntile, npts), dim3(256), 0, st, L, k0);
+        if (k0 + 16 < N) hipLaunchKernelGGL(k_lu_panel, dim3(npts), dim3(256), lds, st, L, k0 + 16);
     }
     const int nblk = (N + 15) / 16;
     hipLaunchKernelGGL(k_lu_dinv, dim3(nblk, npts), dim3(64), 0, st, L, L.dinv);
