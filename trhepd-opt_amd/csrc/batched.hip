@@ -643,74 +643,85 @@ __global__ __launch_bounds__(64) void k_lu_dinv(LuArgs L, double2 *__restrict__ 
         for (int r = 0; r < 16; ++r) out[tid * 16 + r] = x[r];
     }
 }
-
-// Y <- U^{-1} Y for one 16-column tile of Y, block rows from the bottom: X_kb =
-// inv(U_kb,kb) Y_kb and Y_above -= U_above,kb X_kb, both on the matrix cores.
+// Y <- U^{-1} Y by block back substitution: each wave owns one 16-column block of Y and
+// walks up the 16-row blocks of U: X = inv(U_kk) Y_k, then Y_i -= U_ik X for the blocks
+// above (two at a time).  The MFMA result layout (rows kq + 4 r of the lane's column) is
+// the B-operand layout of the next product, so X stays in registers; the waves of a
+// workgroup read the same U blocks, which L1 serves once.
 __global__ __launch_bounds__(256) void k_lu_back(LuArgs L, const double2 *__restrict__ dinv)
 {
-    __shared__ double2 X[16][16];
     const int a = L.list[blockIdx.y];
-    const int N = L.g.N, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int j0 = blockIdx.x * 16;
-    const int cw = min(16, N - j0);
+    const int N = L.g.N, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nblk = (N + 15) / 16;
+    const int j0 = (blockIdx.x * 4 + wave) * 16;
+    if (j0 >= N) return;
+    const int cw = min(16, N - j0);
     const int l16 = lane & 15, kq = lane >> 4;
+    double2 *y = colp(L, a, N + j0 + min(l16, cw - 1));
     for (int kb = nblk - 1; kb >= 0; --kb) {
         const int r0 = kb * 16, rw = min(16, N - r0);
-        if (wave == 0) {
-            const double2 *inv = dinv + ((size_t)blockIdx.y * nblk + kb) * 256;
-            const double2 *ycol = colp(L, a, N + j0 + min(l16, cw - 1));
-            d4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
-            for (int s = 0; s < 4; ++s) {
-                const int k = s * 4 + kq;
-                const double2 av = inv[k * 16 + l16];                       // inv(l16, k)
-                const double2 bv = (k < rw && l16 < cw) ? ycol[r0 + k] : make_double2(0, 0);
-                cr = mfma4(av.x, bv.x, cr);
-                cr = mfma4(-av.y, bv.y, cr);
-                ci = mfma4(av.x, bv.y, ci);
-                ci = mfma4(av.y, bv.x, ci);
-            }
+        const double2 *inv = dinv + ((size_t)blockIdx.y * nblk + kb) * 256;
+        d4 xr = {0, 0, 0, 0}, xi = {0, 0, 0, 0};
 #pragma unroll
-            for (int r = 0; r < 4; ++r) X[kq + 4 * r][l16] = make_double2(cr[r], ci[r]);
+        for (int s = 0; s < 4; ++s) {
+            const int k = s * 4 + kq;
+            const double2 av = inv[k * 16 + l16];                       // inv(l16, k)
+            const double2 bv = (k < rw && l16 < cw) ? y[r0 + k] : make_double2(0, 0);
+            xr = mfma4(av.x, bv.x, xr);
+            xi = mfma4(av.x, bv.y, xi);
+            xr = mfma4(-av.y, bv.y, xr);
+            xi = mfma4(av.y, bv.x, xi);
         }
-        __syncthreads();
-        if (tid < cw)
-            for (int r = 0; r < rw; ++r) colp(L, a, N + j0 + tid)[r0 + r] = X[r][tid];
+        if (l16 < cw) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                if (kq + 4 * r < rw) y[r0 + kq + 4 * r] = make_double2(xr[r], xi[r]);
+        }
         const int ks = (rw + 3) >> 2;
-        for (int rb = wave; rb < kb; rb += 4) {
-            const int i0 = rb * 16, ia = i0 + l16;
-            d4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
-            double2 avs[4];
+        for (int rb = 0; rb < kb; rb += 2) {
+            d4 cr[2], ci[2];
+            double2 av[2][4], yv[2][4];
 #pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const int k = s * 4 + kq;
-                avs[s] = (s < ks && k < rw) ? colp(L, a, r0 + k)[ia] : make_double2(0, 0);
+            for (int h = 0; h < 2; ++h) {
+                cr[h] = d4{0, 0, 0, 0};
+                ci[h] = d4{0, 0, 0, 0};
+                const int i0 = (rb + h) * 16;
+                const bool hv = rb + h < kb;
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const int k = s * 4 + kq;
+                    av[h][s] = (hv && s < ks && k < rw) ? colp(L, a, r0 + k)[i0 + l16] : make_double2(0, 0);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) yv[h][r] = hv ? y[i0 + kq + 4 * r] : make_double2(0, 0);
             }
-            double2 yv[4];
-            double2 *y = colp(L, a, N + j0 + min(l16, cw - 1));
-#pragma unroll
-            for (int r = 0; r < 4; ++r) yv[r] = y[i0 + kq + 4 * r];
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 if (s >= ks) break;
-                const int k = s * 4 + kq;
-                const double2 av = avs[s], bv = X[k][l16];
-                cr = mfma4(av.x, bv.x, cr);
-                ci = mfma4(av.x, bv.y, ci);
-                cr = mfma4(-av.y, bv.y, cr);
-                ci = mfma4(av.y, bv.x, ci);
+#pragma unroll
+                for (int h = 0; h < 2; ++h) cr[h] = mfma4(av[h][s].x, xr[s], cr[h]);
+#pragma unroll
+                for (int h = 0; h < 2; ++h) ci[h] = mfma4(av[h][s].x, xi[s], ci[h]);
+#pragma unroll
+                for (int h = 0; h < 2; ++h) cr[h] = mfma4(-av[h][s].y, xi[s], cr[h]);
+#pragma unroll
+                for (int h = 0; h < 2; ++h) ci[h] = mfma4(av[h][s].y, xr[s], ci[h]);
             }
             if (l16 < cw) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    double2 v = yv[r];
-                    v.x -= cr[r];
-                    v.y -= ci[r];
-                    y[i0 + kq + 4 * r] = v;
+                for (int h = 0; h < 2; ++h) {
+                    if (rb + h >= kb) break;
+                    const int i0 = (rb + h) * 16;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        double2 v = yv[h][r];
+                        v.x -= cr[h][r];
+                        v.y -= ci[h][r];
+                        y[i0 + kq + 4 * r] = v;
+                    }
                 }
             }
         }
-        __syncthreads();
     }
 }
 
@@ -843,7 +854,7 @@ static int lu_on(trhepd_ctx *ctx, hipStream_t st, LuArgs L, int npts)
     }
     const int nblk = (N + 15) / 16;
     hipLaunchKernelGGL(k_lu_dinv, dim3(nblk, npts), dim3(64), 0, st, L, L.dinv);
-    hipLaunchKernelGGL(k_lu_back, dim3(nblk, npts), dim3(256), 0, st, L, (const double2 *)L.dinv);
+    hipLaunchKernelGGL(k_lu_back, dim3((nblk + 3) / 4, npts), dim3(256), 0, st, L, (const double2 *)L.dinv);
     TRY_HIP(ctx, hipGetLastError());
     return TRHEPD_OK;
 }
