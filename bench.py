@@ -131,6 +131,8 @@ def main():
                          "checks the mixed intensities against its own FP64 ones")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (calibrates HBM byte counters)")
+    ap.add_argument("--uneven-structures", type=int, default=16,
+                    help="structures of the uneven-size graph-captured measurement (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests of the serving sample (0: skip the serving measurement)")
     ap.add_argument("--serve-pool", type=int, default=8, help="distinct seeded structures the requests cycle over")
@@ -222,6 +224,36 @@ def main():
     }
     if "mixed" in res and "fp64" in res:
         out["roofline_fp64"] = roofline(res["fp64"], False)
+    if args.uneven_structures > 0:
+        from trhepd import uneven
+        t0 = time.perf_counter()
+        ureqs = uneven.requests(eng64, args.uneven_structures, seed=args.seed)
+        log("uneven set: %d structures, N %s (%.1f s)" % (len(ureqs), [r.nb for r in ureqs], time.perf_counter() - t0))
+        uv = {"workload": "first %d of the seeded uneven-size structures: N from {23,47,97,199,307}, 3-12 atoms, "
+                          "8-20 A slabs, 28-56 angles, sp4 dz=0.02, periodic renormalisation; all batches in one "
+                          "captured HIP graph, stage tiles autotuned per shape" % len(ureqs),
+              "beams": [r.nb for r in ureqs], "angles": [r.nang for r in ureqs]}
+        uf2 = {}
+        for p in paths:
+            eng = lib.Engine(0, precision=lib.PREC_FP64 if p == "fp64" else lib.PREC_MIXED)
+            eng.set_renorm_period(-1)
+            if p != "fp64":
+                eng.set_mixed_digits(args.mixed_digits)
+            try:
+                r = uneven.measure(eng, ureqs, steps=2, warmup=1)
+            finally:
+                eng.close()
+            uf2[p] = r.pop("f2")
+            uv[p] = r
+            log("uneven %s: %.2f points/s (%.1f ms per launch)" % (p, r["points_per_s"], r["ms_per_launch"]))
+        if "fp64" in uf2 and "mixed" in uf2:
+            errs = []
+            for a, b in zip(uf2["mixed"], uf2["fp64"]):
+                mx = np.max(np.abs(b), axis=1, keepdims=True)
+                errs.append(float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-12 * mx))))
+            uv["mixed_max_rel_err_vs_fp64"] = max(errs)
+            uv["mixed_over_fp64"] = uv["mixed"]["points_per_s"] / uv["fp64"]["points_per_s"]
+        out["uneven"] = uv
     if args.serve_requests > 0:
         from trhepd import serve
         t0 = time.perf_counter()

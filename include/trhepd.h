@@ -149,6 +149,24 @@ int trhepd_batch_run_many(trhepd_ctx *ctx, int n, const int *batch_ids);
 int trhepd_batch_fetch(trhepd_ctx *ctx, int batch_id, double *f, double *f2);
 int trhepd_batch_release(trhepd_ctx *ctx, int batch_id);
 
+/* ---- captured runs: batches of uneven beam counts in one launch ------------------
+ * No counterpart in the reference (it runs one deck per process).  trhepd_graph_create
+ * captures the complete integration of n resident batches (any mix of tables and beam
+ * counts: exactly what trhepd_batch_run_many enqueues, every batch on its own stream) into
+ * one HIP graph.  trhepd_graph_launch replays it as a single launch on the context's stream
+ * and waits for it; every batch's F and f2 are then on the device (trhepd_batch_fetch), and
+ * trhepd_last_stats reports the launch.  Batches with N > 48 need a periodic
+ * renormalisation schedule (trhepd_set_renorm_period with K > 0 or -1) when the graph is
+ * created, so that no decision needs the host.  A batch held by a graph cannot be released
+ * before the graph.
+ *
+ * Stage-product tiles: every large-N batch picks its stage-product tile variant when it is
+ * created, by timing the candidates on its own shape (FP64: 32, 48 or 64 columns per tile;
+ * mixed: 2-4 column blocks x 4 or 8 row blocks); the choice is cached per shape. */
+int trhepd_graph_create(trhepd_ctx *ctx, int n, const int *batch_ids, int *graph_id);
+int trhepd_graph_launch(trhepd_ctx *ctx, int graph_id);
+int trhepd_graph_release(trhepd_ctx *ctx, int graph_id);
+
 /* statistics of the last propagate / batch_run call: renormalisations (LU solves) summed
  * over the batch, stage products executed, and the device time in milliseconds. */
 int trhepd_last_stats(const trhepd_ctx *ctx, long long *n_renorm, long long *n_products,

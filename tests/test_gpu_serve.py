@@ -77,11 +77,16 @@ def test_admission_waits_for_the_hbm_budget(engine, reqs):
         probe.batch_release(bid)
     print("request bytes", need, "context scratch", base)
     with lib.Engine(0, hbm_budget=base + int(need * 1.5)) as eng:
-        run = serve.serve(eng, [reqs[-1], reqs[-1], reqs[1]])
+        run = serve.serve(eng, [r, r, r])
+        # a small request fits beside a large one: both are admitted together
+        mixed = serve.serve(eng, [r, reqs[1], r])
     assert run["peak_active"] == 1
-    assert np.array_equal(run["results"][0][1], alone[-1])
-    assert np.array_equal(run["results"][1][1], alone[-1])
-    assert np.array_equal(run["results"][2][1], alone[1])
+    for k in range(3):
+        assert np.array_equal(run["results"][k][1], alone[-1])
+    assert mixed["peak_active"] == 2
+    assert np.array_equal(mixed["results"][0][1], alone[-1])
+    assert np.array_equal(mixed["results"][1][1], alone[1])
+    assert np.array_equal(mixed["results"][2][1], alone[-1])
 
 
 def test_request_larger_than_the_budget_fails(reqs):

@@ -74,6 +74,7 @@ int scratch(trhepd_ctx *ctx, int slot, size_t bytes, void **p)
 
 int prof_mark(trhepd_ctx *ctx, hipStream_t stream)
 {
+    if (ctx->capturing) return TRHEPD_OK;
     Prof &P = ctx->prof;
     if (!P.on) return TRHEPD_OK;
     if (P.used == (int)P.ev.size()) {
@@ -148,12 +149,33 @@ int batch_start(trhepd_ctx *ctx, Batch &b, const Table &t)
 {
     const size_t nn = (size_t)b.nb * b.nb;
     b.stats = Stats();
-    TRY_HIP(ctx, hipMemsetAsync(b.cnt, 0, 64, b.stream));
+    launch_fill_i32(b.stream, b.cnt, 16, 0);
     TRY_HIP(ctx, hipEventRecord(b.ev0, b.stream));
     TRY_HIP(ctx, hipMemcpyAsync(b.fout, b.fin, nn * b.nang * 16, hipMemcpyDeviceToDevice, b.stream));
     if (b.run) return runner_begin(ctx, b, t);
     return run_small(ctx, b.stream, batch_period(ctx, b, t), t, b.nb, b.nang, b.gmap, b.gamma, b.fout,
                      b.has_f2 ? b.f2 : nullptr, b.nb0, b.w, b.cnt);
+}
+
+int enqueue_batches(trhepd_ctx *ctx, const std::vector<Batch *> &bs, const std::vector<const Table *> &ts)
+{
+    const int n = (int)bs.size();
+    int rc = TRHEPD_OK, L = 0;
+    for (int i = 0; i < n && !rc; ++i) {
+        rc = batch_start(ctx, *bs[i], *ts[i]);
+        if (bs[i]->run) L = std::max(L, ts[i]->ns);
+    }
+    // slices of all large-N batches interleaved: while the host waits for one batch's
+    // renormalisation flags the others' kernels are already queued on their streams
+    for (int l = 1; l <= L && !rc; ++l) {
+        for (int i = 0; i < n && !rc; ++i)
+            if (bs[i]->run && l <= ts[i]->ns) rc = runner_slice_enqueue(ctx, *bs[i], *ts[i], l);
+        for (int i = 0; i < n && !rc; ++i)
+            if (bs[i]->run && l <= ts[i]->ns) rc = runner_slice_finish(ctx, *bs[i], *ts[i]);
+    }
+    for (int i = 0; i < n && !rc; ++i)
+        if (bs[i]->run) rc = runner_end(ctx, *bs[i], *ts[i]);
+    return rc;
 }
 
 Batch *find_batch(trhepd_ctx *ctx, int id)
@@ -209,6 +231,11 @@ int trhepd_ctx_destroy(trhepd_ctx *ctx)
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
     server_destroy(ctx);
+    for (GraphRec &gr : ctx->graphs) {
+        (void)hipGraphExecDestroy(gr.exec);
+        (void)hipGraphDestroy(gr.graph);
+        for (hipEvent_t e : gr.events) (void)hipEventDestroy(e);
+    }
     for (Table &t : ctx->tables) dev_free(ctx, t.d, t.bytes);
     for (Batch &b : ctx->batches) {
         runner_destroy(ctx, b);
@@ -444,6 +471,7 @@ int trhepd_batch_create(trhepd_ctx *ctx, int table_id, int nb, int nang, const i
     int rc2 = e ? hip_check(ctx, e, "batch upload") : TRHEPD_OK;
     if (!rc2 && nb > 48) rc2 = runner_create(ctx, b, *t);
     if (rc2) {
+        runner_destroy(ctx, b);
         dev_free(ctx, b.d, b.bytes);
         if (b.ev0) (void)hipEventDestroy(b.ev0);
         if (b.ev1) (void)hipEventDestroy(b.ev1);
@@ -474,21 +502,7 @@ int trhepd_batch_run_many(trhepd_ctx *ctx, int n, const int *batch_ids)
     }
     TRY_HIP(ctx, hipSetDevice(ctx->device));
     ctx->stats = Stats();
-    int rc = TRHEPD_OK, L = 0;
-    for (int i = 0; i < n && !rc; ++i) {
-        rc = batch_start(ctx, *bs[i], *ts[i]);
-        if (bs[i]->run) L = std::max(L, ts[i]->ns);
-    }
-    // slices of all large-N batches interleaved: while the host waits for one batch's
-    // renormalisation flags the others' kernels are already queued on their streams
-    for (int l = 1; l <= L && !rc; ++l) {
-        for (int i = 0; i < n && !rc; ++i)
-            if (bs[i]->run && l <= ts[i]->ns) rc = runner_slice_enqueue(ctx, *bs[i], *ts[i], l);
-        for (int i = 0; i < n && !rc; ++i)
-            if (bs[i]->run && l <= ts[i]->ns) rc = runner_slice_finish(ctx, *bs[i], *ts[i]);
-    }
-    for (int i = 0; i < n && !rc; ++i)
-        if (bs[i]->run) rc = runner_end(ctx, *bs[i], *ts[i]);
+    int rc = enqueue_batches(ctx, bs, ts);
     for (int i = 0; i < n; ++i) {
         if (!rc) rc = hip_check(ctx, hipEventRecord(bs[i]->ev1, bs[i]->stream), "event");
     }
@@ -548,6 +562,10 @@ int trhepd_batch_fetch(trhepd_ctx *ctx, int batch_id, double *f, double *f2)
 int trhepd_batch_release(trhepd_ctx *ctx, int batch_id)
 {
     if (!ctx) return TRHEPD_E_ARG;
+    for (const GraphRec &gr : ctx->graphs)
+        for (int id : gr.batch_ids)
+            if (id == batch_id)
+                return fail(ctx, TRHEPD_E_ARG, "batch %d is part of graph %d: release the graph first", batch_id, gr.id);
     for (size_t i = 0; i < ctx->batches.size(); ++i)
         if (ctx->batches[i].id == batch_id) {
             Batch &b = ctx->batches[i];
@@ -622,6 +640,104 @@ int trhepd_last_stats(const trhepd_ctx *ctx, long long *n_renorm, long long *n_p
     if (n_products) *n_products = ctx->stats.n_products;
     if (device_ms) *device_ms = ctx->stats.device_ms;
     return TRHEPD_OK;
+}
+
+int trhepd_graph_create(trhepd_ctx *ctx, int n, const int *batch_ids, int *graph_id)
+{
+    if (!ctx) return TRHEPD_E_ARG;
+    if (n < 1 || !batch_ids || !graph_id) return fail(ctx, TRHEPD_E_ARG, "graph_create: no batches");
+    std::vector<Batch *> bs;
+    std::vector<const Table *> ts;
+    for (int i = 0; i < n; ++i) {
+        Batch *b = find_batch(ctx, batch_ids[i]);
+        if (!b) return fail(ctx, TRHEPD_E_NOTFOUND, "no batch %d", batch_ids[i]);
+        for (Batch *o : bs)
+            if (o == b) return fail(ctx, TRHEPD_E_ARG, "batch %d listed twice", batch_ids[i]);
+        const Table *t = find_table(ctx, b->table_id);
+        if (!t) return fail(ctx, TRHEPD_E_NOTFOUND, "table %d of batch %d was released", b->table_id, b->id);
+        if (b->run && batch_period(ctx, *b, *t) <= 0)
+            return fail(ctx, TRHEPD_E_ARG, "graph capture needs a periodic renormalisation schedule "
+                                           "(trhepd_set_renorm_period with K > 0 or -1)");
+        bs.push_back(b);
+        ts.push_back(t);
+    }
+    TRY_HIP(ctx, hipSetDevice(ctx->device));
+    TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    GraphRec gr;
+    gr.events.resize(n + 1, nullptr);
+    for (hipEvent_t &e : gr.events) TRY_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    int rc = hip_check(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed), "begin capture");
+    if (!rc) {
+        ctx->capturing = true;
+        rc = hip_check(ctx, hipEventRecord(gr.events[0], ctx->stream), "fork");
+        for (int i = 0; i < n && !rc; ++i) rc = hip_check(ctx, hipStreamWaitEvent(bs[i]->stream, gr.events[0], 0), "fork");
+        if (!rc) rc = enqueue_batches(ctx, bs, ts);
+        for (int i = 0; i < n && !rc; ++i) {
+            rc = hip_check(ctx, hipEventRecord(gr.events[i + 1], bs[i]->stream), "join");
+            if (!rc) rc = hip_check(ctx, hipStreamWaitEvent(ctx->stream, gr.events[i + 1], 0), "join");
+        }
+        ctx->capturing = false;
+        const hipError_t e = hipStreamEndCapture(ctx->stream, &gr.graph);
+        if (!rc && e) rc = hip_check(ctx, e, "end capture");
+    }
+    if (!rc) rc = hip_check(ctx, hipGraphInstantiate(&gr.exec, gr.graph, nullptr, nullptr, 0), "instantiate");
+    if (rc) {
+        if (gr.graph) (void)hipGraphDestroy(gr.graph);
+        for (hipEvent_t e : gr.events)
+            if (e) (void)hipEventDestroy(e);
+        return rc;
+    }
+    gr.id = ctx->next_graph++;
+    gr.batch_ids.assign(batch_ids, batch_ids + n);
+    ctx->graphs.push_back(gr);
+    *graph_id = gr.id;
+    return TRHEPD_OK;
+}
+
+int trhepd_graph_launch(trhepd_ctx *ctx, int graph_id)
+{
+    if (!ctx) return TRHEPD_E_ARG;
+    GraphRec *gr = nullptr;
+    for (GraphRec &g : ctx->graphs)
+        if (g.id == graph_id) gr = &g;
+    if (!gr) return fail(ctx, TRHEPD_E_NOTFOUND, "no graph %d", graph_id);
+    TRY_HIP(ctx, hipSetDevice(ctx->device));
+    ctx->stats = Stats();
+    TRY_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    TRY_HIP(ctx, hipGraphLaunch(gr->exec, ctx->stream));
+    TRY_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    TRY_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->stats.device_ms = ms;
+    int singular = 0;
+    for (int id : gr->batch_ids) {
+        Batch *b = find_batch(ctx, id);
+        int cnt[16];
+        TRY_HIP(ctx, hipMemcpy(cnt, b->cnt, 64, hipMemcpyDeviceToHost));
+        const Table *t = find_table(ctx, b->table_id);
+        ctx->stats.n_renorm += cnt[0];
+        ctx->stats.n_products += (long long)b->nang * t->ns * rkn_coef(t->method).s;
+        singular |= cnt[1];
+    }
+    if (singular) return fail(ctx, TRHEPD_E_SINGULAR, "exact zero pivot in a renormalisation LU");
+    return TRHEPD_OK;
+}
+
+int trhepd_graph_release(trhepd_ctx *ctx, int graph_id)
+{
+    if (!ctx) return TRHEPD_E_ARG;
+    for (size_t i = 0; i < ctx->graphs.size(); ++i)
+        if (ctx->graphs[i].id == graph_id) {
+            GraphRec &gr = ctx->graphs[i];
+            TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            (void)hipGraphExecDestroy(gr.exec);
+            (void)hipGraphDestroy(gr.graph);
+            for (hipEvent_t e : gr.events) (void)hipEventDestroy(e);
+            ctx->graphs.erase(ctx->graphs.begin() + i);
+            return TRHEPD_OK;
+        }
+    return fail(ctx, TRHEPD_E_NOTFOUND, "no graph %d", graph_id);
 }
 
 }  // extern "C"

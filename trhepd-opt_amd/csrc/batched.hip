@@ -53,6 +53,18 @@ struct Geo {
 };
 
 // ---------------------------------------------------------------------------
+// fills (kernels rather than memsets, so that a run can be captured into a HIP graph)
+__global__ void k_fill_i32(int *__restrict__ p, int n, int v)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) p[e] = v;
+}
+__global__ void k_zero2(double2 *__restrict__ p, size_t n)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) p[e] = make_double2(0, 0);
+}
+
 // per-angle constants: conj(gamma^2) and conj(gamma), zero padded to NP
 __global__ void k_prep(Geo g, const double2 *__restrict__ gamma, double2 *__restrict__ g2c,
                        double2 *__restrict__ gc)
@@ -743,11 +755,17 @@ struct Runner {
     MixedGeo mg;
     int8_t *qd = nullptr, *md = nullptr;
     int *eq = nullptr, *em = nullptr;
+    int bn = 48;                // FP64 stage GEMM tile width (autotuned)
     LuArgs lu(const int *lst) const { return LuArgs{g, Q, P, piv, lst, nullptr, dinv}; }
 };
 
 // dynamic LDS of the panel factorisation: the panel, 16 columns of N rows (odd pitch)
 static int lds_bytes(int N) { return (int)((size_t)(N | 1) * 16 * sizeof(double2)); }
+
+void launch_fill_i32(hipStream_t st, int *p, int n, int v)
+{
+    hipLaunchKernelGGL(k_fill_i32, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
+}
 
 int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
 {
@@ -772,12 +790,20 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
     R->mixed = ctx->precision == TRHEPD_PREC_MIXED;
     size_t oqd = 0, omd = 0, oeq = 0, oem = 0;
     if (R->mixed) {
+        // digit buffers sized for the widest padding any candidate tile needs
         const int s = rkn_coef(t.method).s;
         R->mg = mixed_geo(g.A, g.N, g.NP, ctx->mixed_digits);
+        size_t md = 0;
+        int cbp = 0;
+        for (int wc = 2; wc <= 4; ++wc) {
+            const MixedGeo c = mixed_geo(g.A, g.N, g.NP, ctx->mixed_digits, wc);
+            md = std::max(md, mixed_md_bytes(c));
+            cbp = std::max(cbp, c.CBP);
+        }
         oqd = take(mixed_qd_bytes(R->mg));
-        omd = take(mixed_md_bytes(R->mg) * s);
+        omd = take(md * s);
         oeq = take((size_t)R->mg.RBP * 16 * 4);
-        oem = take((size_t)R->mg.CBP * 16 * 4 * s);
+        oem = take((size_t)cbp * 16 * 4 * s);
     }
     int rc = dev_alloc(ctx, (void **)&R->d, off);
     if (rc) {
@@ -815,8 +841,11 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
         delete R;
         return hip_check(ctx, e, "runner host buffers");
     }
+    R->Q = R->Q0;
+    R->Qn = R->Q1;
     std::vector<int> all(g.A);
     for (int i = 0; i < g.A; ++i) all[i] = i;
+    TRY_HIP(ctx, hipMemsetAsync(R->d, 0, R->bytes, b.stream));
     TRY_HIP(ctx, hipMemcpyAsync(R->all, all.data(), g.A * 4, hipMemcpyHostToDevice, b.stream));
     TRY_HIP(ctx, hipStreamSynchronize(b.stream));
     // dynamic LDS of the LU kernels grows with N (the panel is N x 16 complex)
@@ -826,7 +855,7 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
         attr_bytes = lds_bytes(g.N);
     }
     b.run = R;
-    return TRHEPD_OK;
+    return autotune(ctx, b, t);
 }
 
 void runner_destroy(trhepd_ctx *ctx, Batch &b)
@@ -870,15 +899,13 @@ int runner_begin(trhepd_ctx *ctx, Batch &b, const Table &t)
     R.Q = R.Q0;
     R.Qn = R.Q1;
     hipLaunchKernelGGL(k_inv_setup, eg, eb, 0, b.stream, g, (const double2 *)b.fout, R.Q, R.P);
-    TRY_HIP(ctx, hipMemsetAsync(R.M, 0, g.mat * sizeof(double2), b.stream));
+    hipLaunchKernelGGL(k_zero2, dim3((unsigned)((g.mat + 255) / 256)), dim3(256), 0, b.stream, R.M, g.mat);
     LuArgs L = R.lu(R.all);
     L.flag = b.cnt + 1;
     int rc = lu_on(ctx, b.stream, L, g.A);
     if (rc) return rc;
     hipLaunchKernelGGL(k_inv_scale, eg, eb, 0, b.stream, g, (const double2 *)R.gc, R.P);
-    for (int i = 0; i < g.A; ++i) R.h[g.A + i] = 1;
-    TRY_HIP(ctx, hipMemcpyAsync(R.xid, R.h + g.A, g.A * 4, hipMemcpyHostToDevice, b.stream));
-    // the staging area is rewritten only after the first slice's flag event
+    hipLaunchKernelGGL(k_fill_i32, dim3((g.A + 255) / 256), dim3(256), 0, b.stream, R.xid, g.A, 1);
     return TRHEPD_OK;
 }
 
@@ -931,12 +958,68 @@ static void stage_launch(trhepd_ctx *ctx, Batch &b, const Table &t, int mode, co
     S.diagabs = R.diagabs;
     prof_mark(ctx, b.stream);
     const int rowt = (g.A * g.NP + 127) / 128;
-    S.ncolt = (g.NP + 47) / 48;
+    S.ncolt = (g.NP + R.bn - 1) / R.bn;
     S.ntiles = rowt * S.ncolt;
     S.tiles_per_xcd = (S.ntiles + 7) / 8;
-    hipLaunchKernelGGL(k_stage_gemm<48>, dim3(8 * S.tiles_per_xcd), dim3(256), 0, b.stream, S);
+    if (R.bn == 32) hipLaunchKernelGGL(k_stage_gemm<32>, dim3(8 * S.tiles_per_xcd), dim3(256), 0, b.stream, S);
+    else if (R.bn == 64) hipLaunchKernelGGL(k_stage_gemm<64>, dim3(8 * S.tiles_per_xcd), dim3(256), 0, b.stream, S);
+    else hipLaunchKernelGGL(k_stage_gemm<48>, dim3(8 * S.tiles_per_xcd), dim3(256), 0, b.stream, S);
     prof_mark(ctx, b.stream);
     if (mode != 2) std::swap(R.Q, R.Qn);
+}
+
+// Stage-product tile variants: FP64 tile widths 32 / 48 / 64 columns; mixed (column
+// blocks, row blocks) per workgroup (2..4, 8 or 4).
+static int n_variants(const Runner &R) { return R.mixed ? 6 : 3; }
+
+static bool apply_variant(Runner &R, int v)
+{
+    if (!R.mixed) {
+        R.bn = v == 0 ? 32 : v == 1 ? 48 : 64;
+        return true;
+    }
+    const int wc = 2 + v / 2, wr = (v & 1) ? 4 : 8;
+    const MixedGeo c = mixed_geo(R.g.A, R.g.N, R.g.NP, R.mg.ND, wc, wr);
+    if (c.WR != wr) return false;   // that tile does not fit the LDS
+    R.mg = c;
+    return true;
+}
+
+// Autotuner: times every variant of the stage product on this batch's own buffers (before
+// the first run, so the values are irrelevant and are overwritten by the boundary
+// transform) and keeps the fastest; the choice is cached per (precision, N, angles, digits).
+int autotune(trhepd_ctx *ctx, Batch &b, const Table &t)
+{
+    Runner &R = *b.run;
+    const std::array<int, 4> key{R.mixed ? 1 : 0, R.g.N, R.g.A, R.mixed ? R.mg.ND : 0};
+    auto it = ctx->tuned.find(key);
+    if (it != ctx->tuned.end()) {
+        apply_variant(R, it->second);
+        return TRHEPD_OK;
+    }
+    const bool prof = ctx->prof.on;
+    ctx->prof.on = false;
+    int best = -1;
+    float best_ms = 0;
+    for (int v = 0; v < n_variants(R); ++v) {
+        if (!apply_variant(R, v)) continue;
+        stage_launch(ctx, b, t, 2, t.d, 0.0, 0.0, R.coef, 0);   // warm-up (code load)
+        TRY_HIP(ctx, hipEventRecord(b.ev0, b.stream));
+        for (int k = 0; k < 3; ++k) stage_launch(ctx, b, t, 2, t.d, 0.0, 0.0, R.coef, 0);
+        TRY_HIP(ctx, hipEventRecord(b.ev1, b.stream));
+        TRY_HIP(ctx, hipEventSynchronize(b.ev1));
+        float ms = 0;
+        TRY_HIP(ctx, hipEventElapsedTime(&ms, b.ev0, b.ev1));
+        if (best < 0 || ms < best_ms) {
+            best = v;
+            best_ms = ms;
+        }
+    }
+    ctx->prof.on = prof;
+    TRY_HIP(ctx, hipGetLastError());
+    apply_variant(R, best);
+    ctx->tuned[key] = best;
+    return TRHEPD_OK;
 }
 
 int runner_slice_enqueue(trhepd_ctx *ctx, Batch &b, const Table &t, int l)

@@ -2,8 +2,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -73,6 +75,15 @@ struct Batch {
 // renormalisation period of a batch under the context's schedule (0: Gershgorin rule)
 int batch_period(const trhepd_ctx *ctx, const Batch &b, const Table &t);
 
+// a captured multi-batch run (trhepd_graph_create)
+struct GraphRec {
+    int id = 0;
+    std::vector<int> batch_ids;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipEvent_t> events;   // fork / join events of the capture
+};
+
 // event pairs bracketing stage-product launches when profiling is on
 struct Prof {
     bool on = false;
@@ -102,8 +113,12 @@ struct trhepd_ctx {
     int mixed_digits = 7;    // digits per operand of the sliced-integer stage products
     std::vector<trhepd::Batch> batches;
     trhepd::Prof prof;
-    void *tune = nullptr;   // autotuned launch choices
+    // autotuned stage-product tiles: (precision, N, angles, digits) -> variant index
+    std::map<std::array<int, 4>, int> tuned;
     trhepd::Server *serve = nullptr;
+    std::vector<trhepd::GraphRec> graphs;
+    int next_graph = 1;
+    bool capturing = false;   // a stream capture is open: no host synchronisation, no timing events
 };
 
 namespace trhepd {
@@ -127,6 +142,8 @@ Batch *find_batch(trhepd_ctx *ctx, int id);
 // starts a batch run on its stream: counters cleared, F reset to the uploaded input, then
 // the boundary transform (large N) or the whole integration (small N)
 int batch_start(trhepd_ctx *ctx, Batch &b, const Table &t);
+// enqueues complete runs of n batches (start, interleaved slices, end) on their streams
+int enqueue_batches(trhepd_ctx *ctx, const std::vector<Batch *> &bs, const std::vector<const Table *> &ts);
 
 void server_destroy(trhepd_ctx *ctx);
 
@@ -164,7 +181,8 @@ struct MixedGeo {
     int WR;       // row blocks (waves) per workgroup tile (4 or 8)
     int ND;       // digits per operand (5..7)
 };
-MixedGeo mixed_geo(int A, int N, int NP, int ND);
+// wc / wr: column / row blocks per workgroup tile (0: the default choice)
+MixedGeo mixed_geo(int A, int N, int NP, int ND, int wc = 0, int wr = 0);
 size_t mixed_qd_bytes(const MixedGeo &g);   // digits of Q: 3 ND planes
 size_t mixed_md_bytes(const MixedGeo &g);   // digits of one stage's M0: 3 ND planes
 struct MixedStage {
@@ -202,6 +220,10 @@ int run_small(trhepd_ctx *ctx, hipStream_t stream, int period, const Table &t, i
 // flags are copied to the host behind an event); slice_finish waits for the flags and
 // enqueues the LU solves; end enqueues the back transform and intensities.
 int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t);
+// p[0..n) = v on a stream (a kernel, so that it can be captured into a graph)
+void launch_fill_i32(hipStream_t st, int *p, int n, int v);
+// picks the stage-product tile variant of a new batch (timed once per shape, then cached)
+int autotune(trhepd_ctx *ctx, Batch &b, const Table &t);
 void runner_destroy(trhepd_ctx *ctx, Batch &b);
 int runner_begin(trhepd_ctx *ctx, Batch &b, const Table &t);
 int runner_slice_enqueue(trhepd_ctx *ctx, Batch &b, const Table &t, int l);

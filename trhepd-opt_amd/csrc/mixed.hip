@@ -316,7 +316,7 @@ __global__ __launch_bounds__(64 * WR, 1) void k_stage_i8l(StageLaunch Lc)
 
 }  // namespace
 
-MixedGeo mixed_geo(int A, int N, int NP, int ND)
+MixedGeo mixed_geo(int A, int N, int NP, int ND, int wc, int wr)
 {
     MixedGeo g;
     g.ND = ND;
@@ -327,14 +327,17 @@ MixedGeo mixed_geo(int A, int N, int NP, int ND)
     g.KC = (N + 63) / 64;
     g.RB = A * NP / 16;
     g.RBP = (g.RB + 7) / 8 * 8;
-    // column blocks per workgroup: the width in {4, 3, 2} that pads the least (wider on ties)
+    // column blocks per workgroup: given, or the width in {4, 3, 2} that pads the least
+    // (wider on ties)
     const int nb = NP / 16;
     g.WC = 4;
     for (int w = 3; w >= 2; --w)
         if ((nb + w - 1) / w * w < (nb + g.WC - 1) / g.WC * g.WC) g.WC = w;
+    if (wc >= 2 && wc <= 4) g.WC = wc;
     g.CBP = (nb + g.WC - 1) / g.WC * g.WC;
     // eight row blocks per workgroup when the double-buffered tile fits the 160 KB of LDS
     g.WR = 2 * (8 + g.WC) * ND <= 160 ? 8 : 4;
+    if (wr == 4) g.WR = 4;
     return g;
 }
 

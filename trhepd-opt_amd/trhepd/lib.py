@@ -27,7 +27,8 @@ SYMBOLS = ("trhepd_ctx_create", "trhepd_ctx_destroy", "trhepd_last_error", "trhe
            "trhepd_table_build", "trhepd_table_shape", "trhepd_table_fetch", "trhepd_table_release",
            "trhepd_propagate_batch", "trhepd_batch_create", "trhepd_batch_run", "trhepd_batch_run_many", "trhepd_batch_fetch",
            "trhepd_batch_release", "trhepd_last_stats", "trhepd_set_renorm_period", "trhepd_set_mixed_digits", "trhepd_profile",
-           "trhepd_kernel_stats", "trhepd_serve_submit", "trhepd_serve_step", "trhepd_serve_result")
+           "trhepd_kernel_stats", "trhepd_serve_submit", "trhepd_serve_step", "trhepd_serve_result",
+           "trhepd_graph_create", "trhepd_graph_launch", "trhepd_graph_release")
 
 
 class TrhepdError(RuntimeError):
@@ -89,6 +90,9 @@ def load() -> C.CDLL:
     L.trhepd_kernel_stats.argtypes = [vp, C.POINTER(C.c_longlong), dp]
     L.trhepd_serve_submit.argtypes = [vp, C.POINTER(Potential), d, i, i, i, i, vp, vp, vp, i, vp, ip]
     L.trhepd_serve_step.argtypes = [vp, ip, ip, ip]
+    L.trhepd_graph_create.argtypes = [vp, i, vp, ip]
+    L.trhepd_graph_launch.argtypes = [vp, i]
+    L.trhepd_graph_release.argtypes = [vp, i]
     L.trhepd_serve_result.argtypes = [vp, i, vp, vp, dp]
     for name in SYMBOLS:
         getattr(L, name).restype = getattr(L, name).restype or C.c_int
@@ -224,6 +228,20 @@ class Engine:
         tid = C.c_int(0)
         self._chk(self._L.trhepd_table_build(self._h, C.byref(P), dz, ns, method, C.byref(tid)))
         return tid.value
+
+    # -- captured multi-batch runs ---------------------------------------------
+    def graph_create(self, bids) -> int:
+        """Captures the complete run of these resident batches into one graph."""
+        ids = np.ascontiguousarray(bids, dtype=np.int32)
+        gid = C.c_int(0)
+        self._chk(self._L.trhepd_graph_create(self._h, len(ids), _p(ids), C.byref(gid)))
+        return gid.value
+
+    def graph_launch(self, gid: int):
+        self._chk(self._L.trhepd_graph_launch(self._h, gid))
+
+    def graph_release(self, gid: int):
+        self._chk(self._L.trhepd_graph_release(self._h, gid))
 
     # -- serving ---------------------------------------------------------------
     def serve_submit(self, pot: dict, dz: float, ns: int, method: int, iv, gamma, f, nb0: int, wnsga2) -> int:
