@@ -120,7 +120,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--cpu-slices", type=int, default=8)
+    ap.add_argument("--cpu-slices", type=int, default=256)
     ap.add_argument("--renorm-period", type=int, default=-1,
                     help="0: the reference's Gershgorin rule; K: every K slices; -1: K from the evanescent growth rate")
     ap.add_argument("--split", type=int, default=2,

@@ -119,6 +119,24 @@ int trhepd_table_shape(trhepd_ctx *ctx, int table_id, int *nv, int *ncols);
 int trhepd_table_fetch(trhepd_ctx *ctx, int table_id, double *v, double *vi);
 int trhepd_table_release(trhepd_ctx *ctx, int table_id);
 
+/* ---- bulk stage (replaces blkref, bulk.f90:7-124, with trmat trmatg.f90:6-59 and
+ *      gcmi3 gcmi.f90:5-15) ---------------------------------------------------------
+ * Reflection matrices of the semi-infinite bulk for nang angles and ngr groups of coupled
+ * beams.  table_id: the bulk potential at the ns slice midpoints of one unit layer (built
+ * with int_method 1 or 4, whose schedule is the midpoints, from the three-unit descriptor of
+ * bulkio.f90:189-196).  nbg: group sizes (each <= 64); iv: blkghk's in-group map
+ * iv(nbgm, nb) (bulksub.f90:297-336); gamma: complex (nb, nang) for the beams in group
+ * order; phase: complex (nb) column phases of the layer shift, or NULL when there is none.
+ * Every group's unit-layer transfer matrix is the product of the slice matrices (computed
+ * as matrix cosines and sines of the slice Hamiltonian, the same matrix trmat builds from
+ * its eigen-decomposition), then R_l = (T11 R + T12)(T21 R + T22)^-1 runs for up to ml
+ * layers until max |d|R_ii|^2| < epsb after layer 20.  blocks receives the n_g x n_g
+ * complex blocks, column-major, in (angle, group) order; layers (nang x ngr, may be NULL)
+ * the layer count each used. */
+int trhepd_bulk_reflect(trhepd_ctx *ctx, int table_id, int ngr, const int *nbg, int nbgm, const int *iv,
+                        int nang, const double *gamma, const double *phase, int ml, double epsb,
+                        double *blocks, int *layers);
+
 /* ---- slice loop (replaces the angle-loop body of srfref, surflf.f90:79-153) ---- */
 
 /* One batch of nang independent angles that share table `table_id`:

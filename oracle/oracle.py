@@ -72,6 +72,7 @@ def lib():
         L.orc_cond_est.argtypes = [i, zp, i]
         L.orc_cond_est.restype = d
         L.orc_vvi2mate.argtypes = [i, zp, zp, ip, zp, zp]
+        L.orc_comp_scpot.argtypes = [C.c_void_p, d, zp, zp]
         _lib = L
     return _lib
 

@@ -13,6 +13,7 @@ loop of srfref, with every numerical stage on the device.
 from __future__ import annotations
 
 import json
+import copy
 import math
 import os
 from typing import List, Optional, Sequence, Tuple
@@ -153,6 +154,128 @@ def bulk_header(bd: deckio.BulkDeck, inegpos: int = 1, idiag: int = 3) -> deckio
                         [r * rad for r in bd.rdom], list(bd.ih), list(bd.ik), bd.be, azi, daz, naz, gi, dg, ng,
                         0.0, dz, ml, 1e-10, bd.nsg, 0, bd.aa, bd.bb, bd.gam * rad, bd.cc, bd.dx, bd.dy, zoffset,
                         els, atoms, doms)
+
+
+# --------------------------------------------------------------------------
+# bulk stage (bulkio.f90:118-196, bulk.f90:7-124) on the device
+
+
+class BulkJob:
+    """Bulk reflection of one domain of a bulk deck, from its bulkP.b header."""
+
+    def __init__(self, eng: "lib.Engine", bp: deckio.BulkP, idom: int = 0):
+        self.eng, self.bp = eng, bp
+        pi = math.atan(1.0) * 4.0
+        nabr, nabi = bp.nabr, bp.nabi
+        nelm = len(bp.elements)
+        a = np.zeros((nelm, MAB))
+        b = np.zeros((nelm, MAB))
+        c = np.zeros((nelm, MAB))
+        dth, dtk, dtz, elm, dchg = (np.zeros(nelm) for _ in range(5))
+        for i, e in enumerate(bp.elements):
+            na = nabr + nabi
+            a[i, :na], b[i, :na] = e.a[:na], e.b[:na]
+            dth[i], dtk[i], dtz[i], elm[i], dchg[i] = e.dth, e.dtk, e.dtz, e.elm, e.dchg
+        ielm = np.array([at.ielm for at in bp.atoms], dtype=np.int32)
+        ocr = np.array([at.ocr for at in bp.atoms])
+        x = np.array([at.x for at in bp.atoms])
+        y = np.array([at.y for at in bp.atoms])
+        z = np.array([at.z for at in bp.atoms])
+        self.wn, a, b, c, dchg, dth, dtk = lib.asfcrr(bp.be, nelm, nabr, nabi, a, b, c, dchg, dth, dtk, dtz, elm,
+                                                      bp.tem, bp.aa * bp.bb * math.sin(bp.gam))
+        self.ghx = (pi + pi) / (bp.aa * bp.nh)
+        self.ghy = -(pi + pi) / (bp.aa * math.tan(bp.gam) * bp.nh)
+        self.gky = (pi + pi) / (bp.bb * math.sin(bp.gam) * bp.nk)
+        dom = bp.domains[idom]
+        ib0 = sum(bp.nb[:idom])
+        nb = bp.nb[idom]
+        iord = np.asarray(dom.iord)
+        self.ih = np.array(bp.ih[ib0:ib0 + nb])[iord - 1]     # group order of blkibg
+        self.ik = np.array(bp.ik[ib0:ib0 + nb])[iord - 1]
+        self.nbg = [int(n) for n in dom.nbg]
+        igh = np.array(dom.igh, dtype=np.int32)
+        igk = np.array(dom.igk, dtype=np.int32)
+        nv = len(igh)
+        # blkghk's iv(nbgm, nb): row = l - first beam of the group, column = beam k (k < l)
+        first = {}
+        for m in range(1, nv):
+            first.setdefault((int(igh[m]), int(igk[m])), m + 1)
+        nbgm = max(self.nbg)
+        iv = np.zeros((nbgm, nb), dtype=np.int32, order="F")
+        ib = 0
+        for n in self.nbg:
+            for k in range(n):
+                for l in range(k + 1, n):
+                    iv[l, ib + k] = first[(int(self.ih[ib + l] - self.ih[ib + k]), int(self.ik[ib + l] - self.ik[ib + k]))]
+            ib += n
+        self.iv = iv
+        asf, ghk = eng.asfcef(igh, igk, self.ghx, self.ghy, self.gky, nabr, nabi, a.reshape(-1), c.reshape(-1),
+                              dth, dtk, nelm)
+        gh = (-(pi + pi) / bp.nh) * igh.astype(np.float64)
+        gk = (-(pi + pi) / bp.nk) * igk.astype(np.float64)
+        natm = len(bp.atoms)
+        st_in = eng.scpot_stf(bp.nsg, 1, 0, 0, 1, gh, gk, x, y, 0.0, 0.0)
+        st_below = eng.scpot_stf(bp.nsg, 1, 0, 0, 1, gh, gk, x, y, -bp.dx, -bp.dy)
+        st_above = eng.scpot_stf(bp.nsg, 1, 0, 0, 1, gh, gk, x, y, bp.dx, bp.dy)
+        # the unit above (z - cc), this unit (z) and the unit below (z + cc) are the three
+        # parts of the table descriptor
+        self.pot = dict(nv=nv, nvb=nv, natms=natm, natmb=natm, nelm=nelm, nabr=nabr, nabi=nabi, cc=bp.cc,
+                        smesh=1.0, ielm=np.concatenate([ielm, ielm]), z=np.concatenate([z, z]), dchg=dchg, ghk=ghk,
+                        ocr=np.concatenate([ocr, ocr]), b=b.reshape(-1), asf=asf, st0=st_above.reshape(-1),
+                        st1=st_in.reshape(-1), st2=st_below.reshape(-1))
+        self.ns = int(round(bp.cc / bp.dz))
+        self.dz = bp.dz
+        self.azi = bp.azi + bp.rdom[idom]
+        self.phase = None
+        if abs(bp.dx) > 1e-4 or abs(bp.dy) > 1e-4:
+            pi2 = math.atan(1.0) * 8.0
+            s = (bp.dx * pi2 / bp.nh) * self.ih + (bp.dy * pi2 / bp.nk) * self.ik
+            self.phase = np.exp(1j * s)
+        self.table_id = eng.table_build(self.pot, self.dz, self.ns, lib.M2)
+
+    def release(self):
+        if self.table_id is not None:
+            self.eng.table_release(self.table_id)
+            self.table_id = None
+
+    def angles(self) -> List[Tuple[float, float]]:
+        """(glancing, azimuth) in the loop order of bulk.f90:25-50."""
+        bp = self.bp
+        n1, n2 = (bp.naz, bp.ng) if bp.ng > bp.naz else (bp.ng, bp.naz)
+        out = []
+        for r1 in range(n1):
+            for r2 in range(n2):
+                if bp.ng > bp.naz:
+                    out.append((bp.gi + r2 * bp.dg, self.azi + r1 * bp.daz))
+                else:
+                    out.append((bp.gi + r1 * bp.dg, self.azi + r2 * bp.daz))
+        return out
+
+    def gamma(self, ga: float, az: float) -> np.ndarray:
+        wnx = self.wn * math.cos(ga) * math.cos(az)
+        wny = self.wn * math.cos(ga) * math.sin(az)
+        wgx = self.ghx * self.ih + wnx
+        wgy = self.ghy * self.ih + self.gky * self.ik + wny
+        s = self.wn * self.wn - wgx * wgx - wgy * wgy
+        return np.where(s >= 0, np.sqrt(np.abs(s)) + 0j, 1j * np.sqrt(np.abs(s)))
+
+    def run(self, points: Optional[Sequence[int]] = None):
+        """Reflection blocks [angle][group] for the flat angle indices `points` (all by
+        default) and the layer counts used."""
+        angs = self.angles()
+        pts = list(range(len(angs))) if points is None else list(points)
+        G = np.array([self.gamma(*angs[p]) for p in pts])
+        return self.eng.bulk_reflect(self.table_id, self.nbg, self.iv, G, self.phase, self.bp.ml, self.bp.epsb)
+
+    def bulkp(self) -> deckio.BulkP:
+        """The bulkP.b contents with this job's reflection blocks (domain 0)."""
+        bp = self.bp
+        blocks, _ = self.run()
+        n1, n2 = (bp.naz, bp.ng) if bp.ng > bp.naz else (bp.ng, bp.naz)
+        rows = [[blocks[r1 * n2 + r2] for r2 in range(n2)] for r1 in range(n1)]
+        out = copy.deepcopy(bp)
+        out.domains[0].blocks = rows
+        return out
 
 
 # --------------------------------------------------------------------------

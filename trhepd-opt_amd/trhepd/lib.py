@@ -28,7 +28,7 @@ SYMBOLS = ("trhepd_ctx_create", "trhepd_ctx_destroy", "trhepd_last_error", "trhe
            "trhepd_propagate_batch", "trhepd_batch_create", "trhepd_batch_run", "trhepd_batch_run_many", "trhepd_batch_fetch",
            "trhepd_batch_release", "trhepd_last_stats", "trhepd_set_renorm_period", "trhepd_set_mixed_digits", "trhepd_profile",
            "trhepd_kernel_stats", "trhepd_serve_submit", "trhepd_serve_step", "trhepd_serve_result",
-           "trhepd_graph_create", "trhepd_graph_launch", "trhepd_graph_release")
+           "trhepd_graph_create", "trhepd_graph_launch", "trhepd_graph_release", "trhepd_bulk_reflect")
 
 
 class TrhepdError(RuntimeError):
@@ -91,6 +91,7 @@ def load() -> C.CDLL:
     L.trhepd_serve_submit.argtypes = [vp, C.POINTER(Potential), d, i, i, i, i, vp, vp, vp, i, vp, ip]
     L.trhepd_serve_step.argtypes = [vp, ip, ip, ip]
     L.trhepd_graph_create.argtypes = [vp, i, vp, ip]
+    L.trhepd_bulk_reflect.argtypes = [vp, i, i, vp, i, vp, i, vp, vp, i, d, vp, vp]
     L.trhepd_graph_launch.argtypes = [vp, i]
     L.trhepd_graph_release.argtypes = [vp, i]
     L.trhepd_serve_result.argtypes = [vp, i, vp, vp, dp]
@@ -228,6 +229,30 @@ class Engine:
         tid = C.c_int(0)
         self._chk(self._L.trhepd_table_build(self._h, C.byref(P), dz, ns, method, C.byref(tid)))
         return tid.value
+
+    # -- bulk stage -------------------------------------------------------------
+    def bulk_reflect(self, tid: int, nbg, iv: np.ndarray, gamma: np.ndarray, phase, ml: int, epsb: float):
+        """Bulk reflection blocks: gamma (nang, nb) complex in group order, iv blkghk's
+        (nbgm, nb) map (Fortran order), phase (nb) or None.  Returns ([angle][group] blocks,
+        layers (nang, ngr))."""
+        nbg = np.ascontiguousarray(nbg, dtype=np.int32)
+        ivf = np.asfortranarray(iv, dtype=np.int32)
+        nbgm = ivf.shape[0]
+        g = np.ascontiguousarray(gamma, dtype=np.complex128)
+        nang = g.shape[0]
+        ph = None if phase is None else np.ascontiguousarray(phase, dtype=np.complex128)
+        out = np.zeros(nang * int(np.sum(nbg.astype(np.int64) ** 2)), dtype=np.complex128)
+        layers = np.zeros((nang, len(nbg)), dtype=np.int32)
+        self._chk(self._L.trhepd_bulk_reflect(self._h, tid, len(nbg), _p(nbg), nbgm, _p(ivf), nang, _p(g), _p(ph),
+                                              ml, epsb, _p(out), _p(layers)))
+        blocks, q = [], 0
+        for a in range(nang):
+            row = []
+            for n in nbg:
+                row.append(out[q:q + n * n].reshape((n, n), order="F").copy())
+                q += n * n
+            blocks.append(row)
+        return blocks, layers
 
     # -- captured multi-batch runs ---------------------------------------------
     def graph_create(self, bids) -> int:
