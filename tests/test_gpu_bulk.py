@@ -108,3 +108,51 @@ def test_bulk_rejects_wrong_table(engine):
         engine.table_release(tid)
     finally:
         job.release()
+
+
+@pytest.mark.parametrize("case", ["ge001", "test23"])
+def test_bulk_program_from_deck(engine, case, tmp_path):
+    """bulk.txt -> bulkP.b as the reference's bulk program writes it: header records
+    byte-identical, reflection blocks within the FP64 bar."""
+    bd = deckio.BulkDeck.parse(open(os.path.join(REF, case, "bulk.txt")).read())
+    out = driver.bulk_program(engine, bd)
+    path = tmp_path / "bulkP.b"
+    path.write_bytes(out.to_bytes())
+    got = deckio.BulkP.read(str(path))
+    ref = deckio.BulkP.read(os.path.join(REF, case, "bulkP.b"))
+    hdr = lambda b: deckio.BulkP.to_bytes(_no_blocks(b))
+    assert hdr(got) == hdr(ref)
+    for rg, rr in zip(got.domains[0].blocks, ref.domains[0].blocks):
+        for ag, ar in zip(rg, rr):
+            for a, b in zip(ag, ar):
+                assert _err(a, b) <= 1e-10
+
+
+def _no_blocks(bp):
+    import copy
+    b = copy.deepcopy(bp)
+    for d in b.domains:
+        d.blocks = []
+    return b
+
+
+def test_whole_program_on_device(engine):
+    """bulk.txt and surf.txt to the rocking curve with both stages on the device: test23's
+    bulk blocks from the device bulk stage feed the device surface loop, and the curve
+    matches the reference's recorded sp4 dz=0.1 output to the FP64 bar (1e-10, floor
+    1e-12 * max)."""
+    import oracle as O
+    from trhepd import lib
+    bd = deckio.BulkDeck.parse(open(os.path.join(REF, "test23", "bulk.txt")).read())
+    bp = driver.bulk_program(engine, bd)
+    sd = deckio.SurfDeck.parse(open(os.path.join(REF, "test23", "surf.txt")).read())
+    job = driver.SurfaceJob(engine, bp, sd, 0.1, lib.SP4)
+    try:
+        rows = job.run(bp.domains[0].blocks)
+    finally:
+        job.release()
+    ref = deckio.read_s(os.path.join(REF, "test23", "perf-5-0.1.txt"))
+    f2 = [f for row in rows for (_, f) in row]
+    err = max(O.rel_err(f2[p], ref.f2[p]) for p in range(len(f2)))
+    print("whole program test23: rel vs reference %.3e" % err)
+    assert err < 1e-10

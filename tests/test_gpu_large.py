@@ -99,3 +99,112 @@ def test_auto_periodic_renormalisation_n521(engine):
     ref = deckio.read_s(os.path.join(REF, "test521", "perf-5-0.47.txt"))
     l2 = np.linalg.norm(f2[0] - ref.f2[0]) / np.linalg.norm(ref.f2[0])
     assert O.rel_err(f2[0], ref.f2[0]) < 1e-8 and l2 < 1e-10
+
+
+def _device_curve(eng, surf, method, period=0):
+    angs, G, F, W = _inputs(surf)
+    tid = eng.table_build(surf.pot, surf.dz, surf.ns, method)
+    eng.set_renorm_period(period)
+    try:
+        _, f2 = eng.propagate_batch(tid, surf.iv, G, F, surf.nb0, W)
+        st = eng.last_stats()
+    finally:
+        eng.set_renorm_period(0)
+        eng.table_release(tid)
+    return f2, st
+
+
+@pytest.mark.parametrize("dz", ["0.68", "0.47"])
+@pytest.mark.parametrize("period", [0, -1])
+def test_leapfrog_n521(engine, dz, period):
+    """Stormer-Verlet (int_method 4) on the batched path against the reference's perf-4
+    curves and the oracle, at the N = 521 FP64 bar (1e-8 per beam, 1e-10 normalised L2)."""
+    print("case test521 leapfrog", dz, period, flush=True)
+    surf = O.load_case(os.path.join(REF, "test521"), float(dz), 4)
+    f2, st = _device_curve(engine, surf, 4, period)
+    ref = deckio.read_s(os.path.join(REF, "test521", "perf-4-%s.txt" % dz))
+    angs, G, F, W = _inputs(surf)
+    _, of2 = _oracle_rows(surf, angs, G, F)
+    for other in (ref.f2[0], of2[0]):
+        e = O.rel_err(f2[0], other)
+        l2 = np.linalg.norm(f2[0] - other) / np.linalg.norm(other)
+        print("renorms", st["n_renorm"], "rel %.3e L2 %.3e" % (e, l2))
+        assert e < 1e-8 and l2 < 1e-10
+
+
+def _rk4_sensitivity(surf):
+    """How far the oracle's own RK4 curve moves when the potential table is perturbed by
+    one rounding unit: RK4 at dz >= 0.47 is outside its stability region for the strongly
+    evanescent beams of this deck, so every rounding difference is amplified."""
+    v, vi = O.table(surf)
+    r1, r2, ga, az = surf.angles()[0]
+    g = O.gamma(surf, ga, az)
+    f = surf.scatter_f(r1, r2)
+    a = O.intensity(surf, O.integrate(surf, f.copy(), g, v, vi)[0], g, ga)
+    b = O.intensity(surf, O.integrate(surf, f.copy(), g, v * (1 + 2.0 ** -52), vi)[0], g, ga)
+    return a, O.rel_err(b, a), np.linalg.norm(a - b) / np.linalg.norm(a)
+
+
+RK4_OFF = "batched RK4 (N > 48) faulted the device on its first run; switched off until a cause is shown"
+
+
+@pytest.mark.skip(reason=RK4_OFF)
+@pytest.mark.parametrize("dz", ["0.68", "0.47"])
+def test_rk4_n521(engine, dz):
+    """Classical RK4 (int_method 7) on the batched path.  Bar: within 10x of the oracle's
+    own one-ulp sensitivity (measured here: ~1e-3 per beam, ~2e-4 L2 at dz 0.68) against
+    both the oracle and the reference's perf-7 curve."""
+    print("case test521 rk4", dz, flush=True)
+    surf = O.load_case(os.path.join(REF, "test521"), float(dz), 7)
+    f2, st = _device_curve(engine, surf, 7)
+    orc, s_rel, s_l2 = _rk4_sensitivity(surf)
+    ref = deckio.read_s(os.path.join(REF, "test521", "perf-7-%s.txt" % dz))
+    for other in (ref.f2[0], orc):
+        e = O.rel_err(f2[0], other)
+        l2 = np.linalg.norm(f2[0] - other) / np.linalg.norm(other)
+        print("renorms", st["n_renorm"], "rel %.3e L2 %.3e (one-ulp sensitivity %.3e / %.3e)" % (e, l2, s_rel, s_l2))
+        assert e < 10 * s_rel and l2 < 10 * s_l2
+    assert st["n_renorm"] == surf.ns
+
+
+@pytest.mark.parametrize("method", [4, pytest.param(7, marks=pytest.mark.skip(reason=RK4_OFF))])
+def test_leapfrog_rk4_many_angles_same_as_single(engine, method):
+    surf = O.load_case(os.path.join(REF, "test521"), 0.68, method)
+    a, G, F, W = _inputs(surf)
+    scale = np.array([1.0, 0.9995, 1.0005, 0.999])
+    Gb = np.array([G[0] * s for s in scale])
+    Fb = np.array([F[0]] * 4)
+    Wb = np.array([W[0]] * 4)
+    tid = engine.table_build(surf.pot, surf.dz, surf.ns, method)
+    try:
+        _, f2b = engine.propagate_batch(tid, surf.iv, Gb, Fb, surf.nb0, Wb)
+        singles = [engine.propagate_batch(tid, surf.iv, Gb[i:i + 1], Fb[i:i + 1], surf.nb0, Wb[i:i + 1])[1][0]
+                   for i in range(4)]
+    finally:
+        engine.table_release(tid)
+    for i in range(4):
+        assert O.rel_err(f2b[i], singles[i]) < 1e-12
+
+
+def test_leapfrog_mixed_n521(engine, mixed_engine):
+    """Leapfrog on the int8 digit-sliced products: within 1e-6 of the FP64 device curve."""
+    surf = O.load_case(os.path.join(REF, "test521"), 0.68, 4)
+    f64, _ = _device_curve(engine, surf, 4)
+    fmx, _ = _device_curve(mixed_engine, surf, 4)
+    e = O.rel_err(fmx[0], f64[0])
+    print("leapfrog mixed vs FP64 rel %.3e" % e)
+    assert e < 1e-6
+
+
+def test_rk4_large_n_is_refused(engine):
+    """While the batched RK4 path is off, N > 48 with int_method 7 fails cleanly."""
+    from trhepd import lib
+    surf = O.load_case(os.path.join(REF, "test521"), 0.68, 7)
+    angs, G, F, W = _inputs(surf)
+    tid = engine.table_build(surf.pot, surf.dz, surf.ns, 7)
+    try:
+        with pytest.raises(lib.TrhepdError) as e:
+            engine.propagate_batch(tid, surf.iv, G, F, surf.nb0, W)
+        assert e.value.code == lib.E_ARG
+    finally:
+        engine.table_release(tid)

@@ -422,8 +422,6 @@ int trhepd_batch_create(trhepd_ctx *ctx, int table_id, int nb, int nang, const i
         return fail(ctx, TRHEPD_E_ARG, "batch_create: bad arguments");
     if (t->method != TRHEPD_SP4 && t->method != TRHEPD_SP6 && t->method != TRHEPD_LF && t->method != TRHEPD_RK4)
         return fail(ctx, TRHEPD_E_ARG, "int_method %d is not available on the device path", t->method);
-    if (nb > 48 && (t->method == TRHEPD_LF || t->method == TRHEPD_RK4))
-        return fail(ctx, TRHEPD_E_ARG, "int_method %d runs on the device for N <= 48 only (N = %d)", t->method, nb);
     // signed gather map (row-major): upper i<j -> +iv(j,i), lower i>j -> -iv(i,j), diagonal 0
     std::vector<int> gm((size_t)nb * nb);
     for (int i = 0; i < nb; ++i)

@@ -66,6 +66,23 @@ __global__ void k_zero2(double2 *__restrict__ p, size_t n)
 }
 
 // per-angle constants: conj(gamma^2) and conj(gamma), zero padded to NP
+// Q = I + beta P on the listed angles (the identity after a renormalisation; the leapfrog
+// start Q = I + dz/2 P of surf_magnus.f90:63-67); padding stays zero
+__global__ void k_qreset(Geo g, const int *__restrict__ list, double beta, double2 *__restrict__ Q,
+                         const double2 *__restrict__ P)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.mat) return;
+    const size_t idx = (size_t)list[blockIdx.y] * g.mat + e;
+    const int i = (int)(e % g.NP), j = (int)(e / g.NP);
+    double2 q = make_double2(0, 0);
+    if (i < g.N && j < g.N) {
+        const double2 p = P[idx];
+        q = make_double2(beta * p.x + (i == j ? 1.0 : 0.0), beta * p.y);
+    }
+    Q[idx] = q;
+}
+
 __global__ void k_prep(Geo g, const double2 *__restrict__ gamma, double2 *__restrict__ g2c,
                        double2 *__restrict__ gc)
 {
@@ -206,9 +223,52 @@ struct StageArgs {
     double *colsum;       // [A*NP/16][NP]
     double *diagabs;      // [A*NP]
     int ntiles, ncolt, tiles_per_xcd;
+    // classical RK4 stage rk (1..4): Q is the stage operand, Qst / P the state, Qn the next
+    // stage operand, Ps the stage P, SQ / SP the running sums of the Q and P derivatives
+    const double2 *Qst;
+    double2 *Ps, *SQ, *SP;
+    int rk;
+    double dz;
 };
 
-template <int BN>
+// Epilogue of RK4 stage S.rk at element idx, X = (Q M)(i, j) of the stage operand
+// (rk4_int, surf_rk.f90:27-91): k_r = [P_r, Q_r M_r], Q_{r+1} = Q + c dz P_r,
+// P_{r+1} = P + c dz X_r with c = 1/2, 1/2, 1; the update t1 += dz/6 (k1 + 2k2 + 2k3 + k4).
+// Returns the new state Q at stage 4.
+__device__ __forceinline__ double2 rk4_epilogue(const StageArgs &S, size_t idx, double2 x)
+{
+    const double h = 0.5 * S.dz;
+    const double2 q0 = S.Qst[idx], p0 = S.P[idx];
+    switch (S.rk) {
+    case 1:
+        S.Ps[idx] = make_double2(p0.x + h * x.x, p0.y + h * x.y);
+        S.Qn[idx] = make_double2(q0.x + h * p0.x, q0.y + h * p0.y);
+        S.SP[idx] = x;
+        S.SQ[idx] = p0;
+        return q0;
+    case 2:
+    case 3: {
+        const double c = S.rk == 2 ? h : S.dz;
+        const double2 pp = S.Ps[idx];
+        S.Qn[idx] = make_double2(q0.x + c * pp.x, q0.y + c * pp.y);
+        S.Ps[idx] = make_double2(p0.x + c * x.x, p0.y + c * x.y);
+        double2 sp = S.SP[idx], sq = S.SQ[idx];
+        S.SP[idx] = make_double2(sp.x + 2.0 * x.x, sp.y + 2.0 * x.y);
+        S.SQ[idx] = make_double2(sq.x + 2.0 * pp.x, sq.y + 2.0 * pp.y);
+        return q0;
+    }
+    default: {
+        const double w = S.dz / 6;
+        const double2 pp = S.Ps[idx], sp = S.SP[idx], sq = S.SQ[idx];
+        const double2 qn = make_double2(q0.x + w * (sq.x + pp.x), q0.y + w * (sq.y + pp.y));
+        S.P[idx] = make_double2(p0.x + w * (sp.x + x.x), p0.y + w * (sp.y + x.y));
+        S.Qn[idx] = qn;
+        return qn;
+    }
+    }
+}
+
+template <int BN, bool RK>
 __global__ __launch_bounds__(256) void k_stage_gemm(StageArgs S)
 {
     constexpr int NB = BN / 16;
@@ -306,7 +366,15 @@ __global__ __launch_bounds__(256) void k_stage_gemm(StageArgs S)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int i = ib + kq + 4 * r;
-                    if (i < g.N) {
+                    if (i < g.N && RK) {
+                        const size_t idx = (size_t)a * g.mat + i + (size_t)j * g.NP;
+                        const double2 qd = cmul(S.Q[idx], dj);
+                        const double2 qn = rk4_epilogue(S, idx, make_double2(cr[rb][cb][r] + qd.x, ci[rb][cb][r] + qd.y));
+                        if (S.rk == 4) {
+                            if (i == j) S.diagabs[(size_t)a * g.NP + j] = hypot(qn.x, qn.y);
+                            else csum += sqrt(qn.x * qn.x + qn.y * qn.y);
+                        }
+                    } else if (i < g.N) {
                         const size_t idx = (size_t)a * g.mat + i + (size_t)j * g.NP;
                         const double2 q = S.Q[idx];
                         const double2 qd = cmul(q, dj);
@@ -325,7 +393,7 @@ __global__ __launch_bounds__(256) void k_stage_gemm(StageArgs S)
                     }
                 }
             }
-            if (S.mode == 1) {
+            if (S.mode == 1 || (RK && S.rk == 4)) {
                 csum += __shfl_xor(csum, 16);
                 csum += __shfl_xor(csum, 32);
                 if (kq == 0 && j < g.NP) S.colsum[(size_t)(Rb >> 4) * g.NP + j] = csum;
@@ -756,6 +824,8 @@ struct Runner {
     int8_t *qd = nullptr, *md = nullptr;
     int *eq = nullptr, *em = nullptr;
     int bn = 48;                // FP64 stage GEMM tile width (autotuned)
+    // classical RK4 (int_method 7): second stage operand, stage P, derivative sums
+    double2 *Qb = nullptr, *Ps = nullptr, *SQ = nullptr, *SP = nullptr;
     LuArgs lu(const int *lst) const { return LuArgs{g, Q, P, piv, lst, nullptr, dinv}; }
 };
 
@@ -769,8 +839,12 @@ void launch_fill_i32(hipStream_t st, int *p, int n, int v)
 
 int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
 {
-    if (t.method != TRHEPD_SP4 && t.method != TRHEPD_SP6)
-        return fail(ctx, TRHEPD_E_ARG, "large-N device path supports int_method 5 and 6 (got %d)", t.method);
+    if (t.method != TRHEPD_SP4 && t.method != TRHEPD_SP6 && t.method != TRHEPD_LF && t.method != TRHEPD_RK4)
+        return fail(ctx, TRHEPD_E_ARG, "large-N device path supports int_method 4, 5, 6 and 7 (got %d)", t.method);
+    // the batched RK4 stages (k_stage_gemm<BN, true>) faulted the device on their first run
+    // and no cause has been shown; the path stays off until one is
+    if (t.method == TRHEPD_RK4)
+        return fail(ctx, TRHEPD_E_ARG, "int_method 7 runs on the device for N <= 48 only (N = %d)", b.nb);
     if (b.nb > 600) return fail(ctx, TRHEPD_E_ARG, "N = %d exceeds the device path limit of 600 beams", b.nb);
     Runner *R = new Runner();
     Geo &g = R->g;
@@ -787,6 +861,7 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
                  ocs = take((size_t)g.A * (g.NP / 16) * g.NP * 8), oda = take((size_t)g.A * g.NP * 8),
                  oco = take((size_t)g.A * 8), oxi = take((size_t)g.A * 4), ore = take((size_t)g.A * 4),
                  opv = take((size_t)g.A * g.NP * 4), oli = take((size_t)g.A * 4), oal = take((size_t)g.A * 4);
+    const size_t ork = t.method == TRHEPD_RK4 ? take(4 * mb) : 0;
     R->mixed = ctx->precision == TRHEPD_PREC_MIXED;
     size_t oqd = 0, omd = 0, oeq = 0, oem = 0;
     if (R->mixed) {
@@ -827,6 +902,12 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
     R->piv = (int *)(d + opv);
     R->list = (int *)(d + oli);
     R->all = (int *)(d + oal);
+    if (t.method == TRHEPD_RK4) {
+        R->Qb = (double2 *)(d + ork);
+        R->Ps = (double2 *)(d + ork + mb);
+        R->SQ = (double2 *)(d + ork + 2 * mb);
+        R->SP = (double2 *)(d + ork + 3 * mb);
+    }
     if (R->mixed) {
         R->qd = (int8_t *)(d + oqd);
         R->md = (int8_t *)(d + omd);
@@ -888,6 +969,13 @@ static int lu_on(trhepd_ctx *ctx, hipStream_t st, LuArgs L, int npts)
     return TRHEPD_OK;
 }
 
+static void qreset(Batch &b, const int *list, int npts, double beta)
+{
+    const Runner &R = *b.run;
+    hipLaunchKernelGGL(k_qreset, dim3((unsigned)((R.g.mat + 255) / 256), npts), dim3(256), 0, b.stream, R.g, list,
+                       beta, R.Q, (const double2 *)R.P);
+}
+
 int runner_begin(trhepd_ctx *ctx, Batch &b, const Table &t)
 {
     Runner &R = *b.run;
@@ -906,7 +994,34 @@ int runner_begin(trhepd_ctx *ctx, Batch &b, const Table &t)
     if (rc) return rc;
     hipLaunchKernelGGL(k_inv_scale, eg, eb, 0, b.stream, g, (const double2 *)R.gc, R.P);
     hipLaunchKernelGGL(k_fill_i32, dim3((g.A + 255) / 256), dim3(256), 0, b.stream, R.xid, g.A, 1);
+    // leapfrog starts from Q = I + dz/2 P (surf_magnus.f90:63-67), RK4 from Q = I
+    // (surf_rk.f90:41-46)
+    if (t.method == TRHEPD_LF || t.method == TRHEPD_RK4)
+        qreset(b, R.all, g.A, t.method == TRHEPD_LF ? 0.5 * t.dz : 0.0);
     return TRHEPD_OK;
+}
+
+// the FP64 stage GEMM at the batch's tuned tile width
+static void gemm_launch(trhepd_ctx *ctx, Batch &b, StageArgs &S, bool rk)
+{
+    const Runner &R = *b.run;
+    const Geo &g = R.g;
+    const int rowt = (g.A * g.NP + 127) / 128;
+    S.ncolt = (g.NP + R.bn - 1) / R.bn;
+    S.ntiles = rowt * S.ncolt;
+    S.tiles_per_xcd = (S.ntiles + 7) / 8;
+    const dim3 grid(8 * S.tiles_per_xcd), blk(256);
+    prof_mark(ctx, b.stream);
+    if (rk) {
+        if (R.bn == 32) hipLaunchKernelGGL((k_stage_gemm<32, true>), grid, blk, 0, b.stream, S);
+        else if (R.bn == 64) hipLaunchKernelGGL((k_stage_gemm<64, true>), grid, blk, 0, b.stream, S);
+        else hipLaunchKernelGGL((k_stage_gemm<48, true>), grid, blk, 0, b.stream, S);
+    } else {
+        if (R.bn == 32) hipLaunchKernelGGL((k_stage_gemm<32, false>), grid, blk, 0, b.stream, S);
+        else if (R.bn == 64) hipLaunchKernelGGL((k_stage_gemm<64, false>), grid, blk, 0, b.stream, S);
+        else hipLaunchKernelGGL((k_stage_gemm<48, false>), grid, blk, 0, b.stream, S);
+    }
+    prof_mark(ctx, b.stream);
 }
 
 // stage product number `st` of the slice (its digits, on the mixed path, are md[st])
@@ -956,16 +1071,35 @@ static void stage_launch(trhepd_ctx *ctx, Batch &b, const Table &t, int mode, co
     S.mode = mode;
     S.colsum = R.colsum;
     S.diagabs = R.diagabs;
-    prof_mark(ctx, b.stream);
-    const int rowt = (g.A * g.NP + 127) / 128;
-    S.ncolt = (g.NP + R.bn - 1) / R.bn;
-    S.ntiles = rowt * S.ncolt;
-    S.tiles_per_xcd = (S.ntiles + 7) / 8;
-    if (R.bn == 32) hipLaunchKernelGGL(k_stage_gemm<32>, dim3(8 * S.tiles_per_xcd), dim3(256), 0, b.stream, S);
-    else if (R.bn == 64) hipLaunchKernelGGL(k_stage_gemm<64>, dim3(8 * S.tiles_per_xcd), dim3(256), 0, b.stream, S);
-    else hipLaunchKernelGGL(k_stage_gemm<48>, dim3(8 * S.tiles_per_xcd), dim3(256), 0, b.stream, S);
-    prof_mark(ctx, b.stream);
+    gemm_launch(ctx, b, S, false);
     if (mode != 2) std::swap(R.Q, R.Qn);
+}
+
+// RK4 stage rk of a slice: Qout = next stage operand, state Q / P updated at stage 4
+static void rk4_stage(trhepd_ctx *ctx, Batch &b, const VVi *col, int rk, double dz, const double2 *Qin,
+                      double2 *Qout)
+{
+    Runner &R = *b.run;
+    const Geo &g = R.g;
+    hipLaunchKernelGGL(k_gather, dim3((unsigned)((g.mat + 255) / 256)), dim3(256), 0, b.stream, g, b.gmap, col, R.M);
+    StageArgs S{};
+    S.g = g;
+    S.Q = Qin;
+    S.Qn = Qout;
+    S.P = R.P;
+    S.M0 = R.M;
+    S.g2c = R.g2c;
+    S.col = col;
+    S.mode = 3;
+    S.colsum = R.colsum;
+    S.diagabs = R.diagabs;
+    S.Qst = R.Q;
+    S.Ps = R.Ps;
+    S.SQ = R.SQ;
+    S.SP = R.SP;
+    S.rk = rk;
+    S.dz = dz;
+    gemm_launch(ctx, b, S, true);
 }
 
 // Stage-product tile variants: FP64 tile widths 32 / 48 / 64 columns; mixed (column
@@ -1030,14 +1164,30 @@ int runner_slice_enqueue(trhepd_ctx *ctx, Batch &b, const Table &t, int l)
     const int s = co.s;
     const size_t ntot = g.mat * g.A;
     const dim3 eb(256), eg((unsigned)((ntot + 255) / 256));
-    const VVi *col1 = t.d + (size_t)(s * (l - 1)) * t.nv;
-    // the s stage products of slice l use table columns s(l-1)+1 .. s l
-    if (R.mixed) launch_msplit(b.stream, R.mg, b.gmap, col1 + t.nv, t.nv, s, R.md, R.em);
-    hipLaunchKernelGGL(k_stage1, eg, eb, 0, b.stream, g, b.gmap, col1, (const double2 *)R.g2c, (const int *)R.xid,
-                       co.bab[0] * t.dz, co.bab[1] * t.dz, R.Q, R.P);
-    for (int k = 2; k <= s; ++k)
-        stage_launch(ctx, b, t, k == s ? 1 : 0, t.d + (size_t)(s * (l - 1) + k - 1) * t.nv, co.bab[2 * k - 2] * t.dz,
-                     co.bab[2 * k - 1] * t.dz, nullptr, k - 2);
+    if (t.method == TRHEPD_LF) {
+        // Stormer-Verlet (lf_int_spa, surf_magnus.f90:68-99): kick with the slice-midpoint
+        // matrix, then a full drift (a half drift on the last slice); after a
+        // renormalisation Q = I and the product reduces to the reference's P += dz M
+        const VVi *col = t.d + (size_t)(l - 1) * t.nv;
+        if (R.mixed) launch_msplit(b.stream, R.mg, b.gmap, col, t.nv, 1, R.md, R.em);
+        stage_launch(ctx, b, t, 1, col, t.dz, l == t.ns ? 0.5 * t.dz : t.dz, nullptr, 0);
+    } else if (t.method == TRHEPD_RK4) {
+        // rk4_int (surf_rk.f90:48-78): matrices at z, z + dz/2 (twice) and z + dz
+        const VVi *c0 = t.d + (size_t)(2 * (l - 1)) * t.nv;
+        rk4_stage(ctx, b, c0, 1, t.dz, R.Q, R.Qn);
+        rk4_stage(ctx, b, c0 + t.nv, 2, t.dz, R.Qn, R.Qb);
+        rk4_stage(ctx, b, c0 + t.nv, 3, t.dz, R.Qb, R.Qn);
+        rk4_stage(ctx, b, c0 + 2 * t.nv, 4, t.dz, R.Qn, R.Q);
+    } else {
+        const VVi *col1 = t.d + (size_t)(s * (l - 1)) * t.nv;
+        // the s stage products of slice l use table columns s(l-1)+1 .. s l
+        if (R.mixed) launch_msplit(b.stream, R.mg, b.gmap, col1 + t.nv, t.nv, s, R.md, R.em);
+        hipLaunchKernelGGL(k_stage1, eg, eb, 0, b.stream, g, b.gmap, col1, (const double2 *)R.g2c,
+                           (const int *)R.xid, co.bab[0] * t.dz, co.bab[1] * t.dz, R.Q, R.P);
+        for (int k = 2; k <= s; ++k)
+            stage_launch(ctx, b, t, k == s ? 1 : 0, t.d + (size_t)(s * (l - 1) + k - 1) * t.nv,
+                         co.bab[2 * k - 2] * t.dz, co.bab[2 * k - 1] * t.dz, nullptr, k - 2);
+    }
     // periodic schedule: the decision is known on the host, no flag round trip
     const int period = batch_period(ctx, b, t);
     R.forced = period > 0 ? ((l % period == 0) || l == t.ns ? 1 : 0) : -1;
@@ -1048,7 +1198,10 @@ int runner_slice_enqueue(trhepd_ctx *ctx, Batch &b, const Table &t, int l)
         TRY_HIP(ctx, hipMemcpyAsync(R.h, R.ren, g.A * 4, hipMemcpyDeviceToHost, b.stream));
         TRY_HIP(ctx, hipEventRecord(R.evf, b.stream));
     }
-    stage_launch(ctx, b, t, 2, t.d + (size_t)(s * l) * t.nv, 0.0, 0.0, R.coef, s - 1);
+    // RKN: the end-of-slice kick (merged with the next slice's first kick unless the angle
+    // renormalises)
+    if (t.method == TRHEPD_SP4 || t.method == TRHEPD_SP6)
+        stage_launch(ctx, b, t, 2, t.d + (size_t)(s * l) * t.nv, 0.0, 0.0, R.coef, s - 1);
     TRY_HIP(ctx, hipGetLastError());
     return TRHEPD_OK;
 }
@@ -1057,11 +1210,16 @@ int runner_slice_finish(trhepd_ctx *ctx, Batch &b, const Table &t)
 {
     Runner &R = *b.run;
     const Geo &g = R.g;
+    // leapfrog and RK4 restart from Q = I after a renormalisation (the RKN path rebuilds Q
+    // in its first stage)
+    const bool reset = t.method == TRHEPD_LF || t.method == TRHEPD_RK4;
     if (R.forced >= 0) {
         if (!R.forced) return TRHEPD_OK;
         LuArgs L = R.lu(R.all);
         L.flag = b.cnt + 1;
-        return lu_on(ctx, b.stream, L, g.A);
+        const int rc = lu_on(ctx, b.stream, L, g.A);
+        if (!rc && reset) qreset(b, R.all, g.A, 0.0);
+        return rc;
     }
     TRY_HIP(ctx, hipEventSynchronize(R.evf));
     // compact list of the angles to renormalise, staged in pinned memory: its copy is
@@ -1074,7 +1232,9 @@ int runner_slice_finish(trhepd_ctx *ctx, Batch &b, const Table &t)
     TRY_HIP(ctx, hipMemcpyAsync(R.list, lst, (size_t)nl * 4, hipMemcpyHostToDevice, b.stream));
     LuArgs L = R.lu(R.list);
     L.flag = b.cnt + 1;
-    return lu_on(ctx, b.stream, L, nl);
+    const int rc = lu_on(ctx, b.stream, L, nl);
+    if (!rc && reset) qreset(b, R.list, nl, 0.0);
+    return rc;
 }
 
 int runner_end(trhepd_ctx *ctx, Batch &b, const Table &t)

@@ -267,15 +267,25 @@ class BulkJob:
         G = np.array([self.gamma(*angs[p]) for p in pts])
         return self.eng.bulk_reflect(self.table_id, self.nbg, self.iv, G, self.phase, self.bp.ml, self.bp.epsb)
 
-    def bulkp(self) -> deckio.BulkP:
-        """The bulkP.b contents with this job's reflection blocks (domain 0)."""
-        bp = self.bp
+    def rows(self):
+        """This domain's reflection blocks as bulkP.b holds them: [irep1][irep2][group]."""
         blocks, _ = self.run()
-        n1, n2 = (bp.naz, bp.ng) if bp.ng > bp.naz else (bp.ng, bp.naz)
-        rows = [[blocks[r1 * n2 + r2] for r2 in range(n2)] for r1 in range(n1)]
-        out = copy.deepcopy(bp)
-        out.domains[0].blocks = rows
-        return out
+        n1, n2 = self.bp.nrep
+        return [[blocks[r1 * n2 + r2] for r2 in range(n2)] for r1 in range(n1)]
+
+
+def bulk_program(eng: "lib.Engine", bd: deckio.BulkDeck) -> deckio.BulkP:
+    """What the reference's bulk program writes to bulkP.b for a bulk.txt (bulkio.f90,
+    bulk.f90): the header, then every domain's reflection blocks from the device."""
+    bp = bulk_header(bd)
+    out = copy.deepcopy(bp)
+    for d in range(bp.ndom):
+        job = BulkJob(eng, bp, d)
+        try:
+            out.domains[d].blocks = job.rows()
+        finally:
+            job.release()
+    return out
 
 
 # --------------------------------------------------------------------------
