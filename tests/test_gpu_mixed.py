@@ -83,3 +83,30 @@ def test_mixed_c3_full_curve_vs_fp64(engine, digits):
           % (digits, errs.max(), np.median(errs), l2.max(), i00.max()))
     assert np.all(np.isfinite(fmx))
     assert errs.max() < 1e-6
+
+
+def test_mixed_leapfrog_c3_vs_fp64(engine):
+    """Leapfrog (int_method 4) on the bench structure, 8 of its 40 angles, 6 digits, periodic
+    renormalisation (so Q is not the identity between renormalisations): within 1e-6 of FP64."""
+    st = synth.config(3, seed=1234)
+    blocks = synth.bulk_reflection(engine, st, layers=2)
+    out = {}
+    mixed_engine = lib.Engine(0, precision=lib.PREC_MIXED)
+    mixed_engine.set_mixed_digits(6)
+    for name, eng in (("fp64", engine), ("mixed", mixed_engine)):
+        job = driver.SurfaceJob(eng, st.bp, st.surf, 0.02, lib.LF)
+        eng.set_renorm_period(-1)
+        try:
+            G, F, W, A = job.inputs(0, blocks)
+            G, F, W = G[::5], F[::5], W[::5]
+            _, f2 = eng.propagate_batch(job.table_id, job.iv, G, F, job.nb0, W)
+            print(name, eng.last_stats())
+        finally:
+            eng.set_renorm_period(0)
+            job.release()
+        out[name] = f2
+    mixed_engine.close()
+    errs = np.array([O.rel_err(out["mixed"][p], out["fp64"][p]) for p in range(len(out["fp64"]))])
+    print("C3 leapfrog mixed vs fp64: max rel %.3e" % errs.max())
+    assert np.all(np.isfinite(out["mixed"]))
+    assert errs.max() < 1e-6

@@ -1114,7 +1114,7 @@ static bool apply_variant(Runner &R, int v)
     }
     const int wc = 2 + v / 2, wr = (v & 1) ? 4 : 8;
     const MixedGeo c = mixed_geo(R.g.A, R.g.N, R.g.NP, R.mg.ND, wc, wr);
-    if (c.WR != wr) return false;   // that tile does not fit the LDS
+    if (c.WR != wr) return false;   // that tile does not fit the registers
     R.mg = c;
     return true;
 }

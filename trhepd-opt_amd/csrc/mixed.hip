@@ -168,17 +168,17 @@ struct StageLaunch {
 };
 
 // LDS-tiled form: a workgroup of WR waves computes 16 WR rows (one 16-row block per wave)
-// x 16 WC columns.  Per 64-wide k chunk of one real product it needs WR ND fragments of Q
-// digits and WC ND fragments of M0 digits (1 KB each); they are staged global -> registers
-// -> LDS one chunk ahead (double-buffered LDS), so the loads of chunk c+1 overlap the MFMAs
-// of chunk c.  The M0 fragments are shared by all WR waves: the taller the tile, the fewer
-// times the M0 digits stream through L2.  The chunks of the three real products form one
-// pipelined sequence of 3 KC steps.
+// x 16 WC columns.  Per 64-wide k chunk of one real product a wave needs ND fragments of its
+// own Q digits and the WC ND fragments of M0 digits that all WR waves share (1 KB each).
+// Only the shared M0 fragments go through LDS (global -> registers -> LDS, double-buffered);
+// each wave loads its Q fragments straight into registers.  Both are fetched one chunk
+// ahead, so the loads of chunk c+1 overlap the MFMAs of chunk c.  The chunks of the three
+// real products form one pipelined sequence of 3 KC steps.
 template <int ND, int WC, int WR>
 __global__ __launch_bounds__(64 * WR, 1) void k_stage_i8l(StageLaunch Lc)
 {
-    constexpr int NA = WR * ND, NBF = WC * ND, NF = NA + NBF, NR = (NF + WR - 1) / WR;
-    extern __shared__ v4i lds[];   // [2][NF][64]
+    constexpr int NBF = WC * ND, NR = (NBF + WR - 1) / WR;
+    extern __shared__ v4i lds[];   // [2][NBF][64]
     const MixedStage &S = Lc.S;
     const MixedGeo g = S.g;
     const int L = blockIdx.x;
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(64 * WR, 1) void k_stage_i8l(StageLaunch Lc)
     const int l16 = lane & 15, kq = lane >> 4;
     const int rb0 = rt * WR, cb0 = ct * WC;
     const size_t qplane = (size_t)g.RBP * g.KC * 64, mplane = (size_t)g.CBP * g.KC * 64;
-    const v4i *__restrict__ QD = (const v4i *)S.qd;
+    const v4i *__restrict__ QD = (const v4i *)S.qd + (size_t)(rb0 + wave) * g.KC * 64 + lane;
     const v4i *__restrict__ MD = (const v4i *)S.md;
     const int nchunk = 3 * g.KC;
     v4i stage[NR];
@@ -198,22 +198,26 @@ __global__ __launch_bounds__(64 * WR, 1) void k_stage_i8l(StageLaunch Lc)
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
             const int f = r * WR + wave;
-            if (f < NA) {
-                const int p = f / WR, rb = f % WR;
-                stage[r] = QD[(size_t)(m * ND + p) * qplane + ((size_t)(rb0 + rb) * g.KC + kc) * 64 + lane];
-            } else if (f < NF) {
-                const int q = (f - NA) / WC, cb = (f - NA) % WC;
+            if (f < NBF) {
+                const int q = f / WC, cb = f % WC;
                 stage[r] = MD[(size_t)(m * ND + q) * mplane + ((size_t)(cb0 + cb) * g.KC + kc) * 64 + lane];
             }
         }
+    };
+    auto qload = [&](int c, v4i *dst) {
+        const int m = c / g.KC, kc = c - m * g.KC;
+#pragma unroll
+        for (int p = 0; p < ND; ++p) dst[p] = QD[(size_t)(m * ND + p) * qplane + (size_t)kc * 64];
     };
     auto sstore = [&](int buf) {
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
             const int f = r * WR + wave;
-            if (f < NF) lds[((size_t)buf * NF + f) * 64 + lane] = stage[r];
+            if (f < NBF) lds[((size_t)buf * NBF + f) * 64 + lane] = stage[r];
         }
     };
+    v4i qa[ND], qn[ND];
+    qload(0, qa);
     gload(0);
     sstore(0);
     __syncthreads();
@@ -227,20 +231,20 @@ __global__ __launch_bounds__(64 * WR, 1) void k_stage_i8l(StageLaunch Lc)
         for (int r = 0; r < 4; ++r) re[cb][r] = im[cb][r] = 0.0;
     }
     for (int c = 0; c < nchunk; ++c) {
-        if (c + 1 < nchunk) gload(c + 1);
-        const v4i *Lb = lds + (size_t)(c & 1) * NF * 64 + lane;
-        v4i a[ND];
-#pragma unroll
-        for (int p = 0; p < ND; ++p) a[p] = Lb[(p * WR + wave) * 64];
+        if (c + 1 < nchunk) {
+            gload(c + 1);
+            qload(c + 1, qn);
+        }
+        const v4i *Lb = lds + (size_t)(c & 1) * NBF * 64 + lane;
 #pragma unroll
         for (int q = 0; q < ND; ++q) {
             v4i b[WC];
 #pragma unroll
-            for (int cb = 0; cb < WC; ++cb) b[cb] = Lb[(NA + q * WC + cb) * 64];
+            for (int cb = 0; cb < WC; ++cb) b[cb] = Lb[(q * WC + cb) * 64];
 #pragma unroll
             for (int p = 0; p + q < ND; ++p)
 #pragma unroll
-                for (int cb = 0; cb < WC; ++cb) acc[p + q][cb] = mfma_i8(b[cb], a[p], acc[p + q][cb]);
+                for (int cb = 0; cb < WC; ++cb) acc[p + q][cb] = mfma_i8(b[cb], qa[p], acc[p + q][cb]);
         }
         const int m = c / g.KC;
         if (c - m * g.KC == g.KC - 1) {
@@ -261,7 +265,11 @@ __global__ __launch_bounds__(64 * WR, 1) void k_stage_i8l(StageLaunch Lc)
 #pragma unroll
                 for (int s = 0; s < ND; ++s) acc[s][cb] = v4i{0, 0, 0, 0};
         }
-        if (c + 1 < nchunk) sstore((c + 1) & 1);
+        if (c + 1 < nchunk) {
+            sstore((c + 1) & 1);
+#pragma unroll
+            for (int p = 0; p < ND; ++p) qa[p] = qn[p];
+        }
         __syncthreads();
     }
     // FP64 epilogue: lane (l16, kq) holds row i = block row + l16, columns 4 kq + r
@@ -335,9 +343,10 @@ MixedGeo mixed_geo(int A, int N, int NP, int ND, int wc, int wr)
         if ((nb + w - 1) / w * w < (nb + g.WC - 1) / g.WC * g.WC) g.WC = w;
     if (wc >= 2 && wc <= 4) g.WC = wc;
     g.CBP = (nb + g.WC - 1) / g.WC * g.WC;
-    // eight row blocks per workgroup when the double-buffered tile fits the 160 KB of LDS
-    g.WR = 2 * (8 + g.WC) * ND <= 160 ? 8 : 4;
-    if (wr == 4) g.WR = 4;
+    // row blocks per workgroup: eight (two waves per SIMD, at most 256 VGPRs each) unless four
+    // are asked for, or the 7-digit 4-block accumulators need the registers of one wave per
+    // SIMD.  The LDS holds only the M0 fragments, 2 WC ND KB.
+    g.WR = (wr == 4 || (ND == 7 && g.WC == 4)) ? 4 : 8;
     return g;
 }
 
@@ -366,7 +375,7 @@ void launch_qsplit(hipStream_t st, const MixedGeo &g, const double2 *Q, int8_t *
 template <int ND, int WC, int WR>
 static void stage_i8_on(hipStream_t st, const StageLaunch &Lc)
 {
-    constexpr int NF = (WR + WC) * ND;
+    constexpr int NF = WC * ND;
     const size_t lds = (size_t)2 * NF * 1024;
     static_assert(2 * NF <= 160, "tile exceeds the LDS");
     static bool attr = false;
@@ -381,7 +390,7 @@ static void stage_i8_on(hipStream_t st, const StageLaunch &Lc)
 template <int ND, int WC>
 static void stage_i8_wr(hipStream_t st, const StageLaunch &Lc)
 {
-    if constexpr (2 * (8 + WC) * ND <= 160) {
+    if constexpr (!(ND == 7 && WC == 4)) {
         if (Lc.S.g.WR == 8) return stage_i8_on<ND, WC, 8>(st, Lc);
     }
     stage_i8_on<ND, WC, 4>(st, Lc);

@@ -77,6 +77,20 @@ module trhepd_gpu
                         type(c_ptr), value :: ctx
                         integer(c_int), value :: digits
                 end function
+                ! blkref's angle and group loops (bulk.f90:51-124); phase is complex(nb) or
+                ! c_null_ptr, layers integer(nang, ngr) or c_null_ptr
+                integer(c_int) function trhepd_bulk_reflect(ctx, table_id, ngr, nbg, nbgm, iv, nang, gamma, &
+                                phase, ml, epsb, blocks, layers) bind(C, name="trhepd_bulk_reflect")
+                        import :: c_int, c_ptr, c_double, c_double_complex
+                        type(c_ptr), value :: ctx
+                        integer(c_int), value :: table_id, ngr, nbgm, nang, ml
+                        integer(c_int), intent(in) :: nbg(ngr), iv(nbgm, *)
+                        complex(c_double_complex), intent(in) :: gamma(*)
+                        type(c_ptr), value :: phase
+                        real(c_double), value :: epsb
+                        complex(c_double_complex), intent(out) :: blocks(*)
+                        type(c_ptr), value :: layers
+                end function
         end interface
 
 contains
