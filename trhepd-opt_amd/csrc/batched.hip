@@ -17,6 +17,9 @@
 // once per slice, while the end-of-slice kick runs, and launches the batched LU only for
 // the angles that need it.
 #include <algorithm>
+#include <cmath>
+#include <initializer_list>
+#include <utility>
 
 #include "internal.hpp"
 
@@ -805,6 +808,129 @@ __global__ __launch_bounds__(256) void k_lu_back(LuArgs L, const double2 *__rest
     }
 }
 
+// ---------------------------------------------------------------------------
+// Magnus integrators (m2/m4/m6_int_spa, surf_magnus.f90:114-339).  Their products involve
+// per-angle matrices on both sides (M^2, Q rp, P q), so they run as batched dense products
+// of NP x NP column-major matrices (zero padding) plus elementwise combinations.
+
+// dense per-angle slice matrix M_a = M0 + diag(d_a) (vvi2mate, matcomp.f90:253-279)
+__global__ void k_build_m(Geo g, const int *__restrict__ gmap, const VVi *__restrict__ col,
+                          const double2 *__restrict__ g2c, double2 *__restrict__ M)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.mat * g.A) return;
+    const int a = (int)(e / g.mat);
+    const int r = (int)(e % g.mat), i = r % g.NP, j = r / g.NP;
+    double2 m = make_double2(0, 0);
+    if (i < g.N && j < g.N) {
+        if (i == j) {
+            const double2 w = w_upper(col[0]), c2 = g2c[(size_t)a * g.NP + j];
+            m = make_double2(w.x - c2.x, w.y - c2.y);
+        } else {
+            const int s = gmap[i * g.N + j];
+            m = s > 0 ? w_upper(col[s - 1]) : w_lower(col[-s - 1]);
+        }
+    }
+    M[e] = m;
+}
+
+// C_a = alpha A_a B_a + beta C_a + ident I for every angle; one wave per 16 x 16 output
+// block, four waves per workgroup.  C must not alias A or B.
+struct BmmArgs {
+    Geo g;
+    const double2 *A, *B;
+    double2 *C;
+    double alpha, beta, ident;
+};
+
+__global__ __launch_bounds__(256) void k_bmm(BmmArgs S)
+{
+    const Geo g = S.g;
+    const int nbl = g.NP / 16;
+    const int a = blockIdx.y;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l16 = lane & 15, kq = lane >> 4;
+    const int blk = blockIdx.x * 4 + wave;
+    if (blk >= nbl * nbl) return;
+    const int bi = blk % nbl, bj = blk / nbl;
+    const double2 *__restrict__ Ab = S.A + (size_t)a * g.mat + bi * 16 + l16;
+    const double2 *__restrict__ Bb = S.B + (size_t)a * g.mat + (size_t)(bj * 16 + l16) * g.NP;
+    d4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+    for (int k0 = 0; k0 < g.NP; k0 += 4) {
+        const int k = k0 + kq;
+        const double2 av = Ab[(size_t)k * g.NP], bv = Bb[k];
+        cr = mfma4(av.x, bv.x, cr);
+        ci = mfma4(av.x, bv.y, ci);
+        cr = mfma4(-av.y, bv.y, cr);
+        ci = mfma4(av.y, bv.x, ci);
+    }
+    const int j = bj * 16 + l16;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = bi * 16 + kq + 4 * r;
+        if (i >= g.N || j >= g.N) continue;
+        const size_t idx = (size_t)a * g.mat + i + (size_t)j * g.NP;
+        double2 c = make_double2(S.alpha * cr[r], S.alpha * ci[r]);
+        if (S.beta != 0.0) {
+            const double2 o = S.C[idx];
+            c = make_double2(S.beta * o.x + c.x, S.beta * o.y + c.y);
+        }
+        if (i == j) c.x += S.ident;
+        S.C[idx] = c;
+    }
+}
+
+// out = c0 x0 + c1 x1 + c2 x2 + c3 x3 + ident I (real coefficients, elementwise, n terms,
+// summed left to right); out may be one of the inputs
+struct LinArgs {
+    Geo g;
+    int n;
+    double c[4];
+    const double2 *x[4];
+    double2 *out;
+    double ident;
+};
+
+__global__ void k_lin(LinArgs L)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= L.g.mat * L.g.A) return;
+    const int r = (int)(e % L.g.mat), i = r % L.g.NP, j = r / L.g.NP;
+    double2 v = make_double2(0, 0);
+    if (i < L.g.N && j < L.g.N) {
+        for (int t = 0; t < L.n; ++t) {
+            const double2 x = L.x[t][e];
+            v.x += L.c[t] * x.x;
+            v.y += L.c[t] * x.y;
+        }
+        if (i == j) v.x += L.ident;
+    }
+    L.out[e] = v;
+}
+
+// Gershgorin terms of Q (cond_est, matcomp.f90:498-528) in k_cond's layout: per angle, per
+// 16-row block b and column j, the sum of |Q(i,j)| over the block's rows i != j; |Q(j,j)|
+__global__ void k_gersh(Geo g, const double2 *__restrict__ Q, double *__restrict__ colsum,
+                        double *__restrict__ diagabs)
+{
+    const int nblk = g.NP / 16;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)g.A * nblk * g.NP) return;
+    const int j = (int)(e % g.NP);
+    const size_t ab = e / g.NP;   // a * nblk + b
+    const int a = (int)(ab / nblk), b = (int)(ab % nblk);
+    double s = 0.0;
+    if (j < g.N) {
+        const double2 *qc = Q + (size_t)a * g.mat + (size_t)j * g.NP;
+        for (int i = b * 16; i < b * 16 + 16 && i < g.N; ++i) {
+            const double2 q = qc[i];
+            if (i == j) diagabs[(size_t)a * g.NP + j] = hypot(q.x, q.y);
+            else s += sqrt(q.x * q.x + q.y * q.y);
+        }
+    }
+    colsum[ab * g.NP + j] = s;
+}
+
 }  // namespace
 
 struct Runner {
@@ -826,6 +952,9 @@ struct Runner {
     int bn = 48;                // FP64 stage GEMM tile width (autotuned)
     // classical RK4 (int_method 7): second stage operand, stage P, derivative sums
     double2 *Qb = nullptr, *Ps = nullptr, *SQ = nullptr, *SP = nullptr;
+    // Magnus (int_method 1-3): per-angle work matrices (slice matrices, exponential blocks,
+    // the carried rp)
+    double2 *W[8] = {};
     LuArgs lu(const int *lst) const { return LuArgs{g, Q, P, piv, lst, nullptr, dinv}; }
 };
 
@@ -837,10 +966,16 @@ void launch_fill_i32(hipStream_t st, int *p, int n, int v)
     hipLaunchKernelGGL(k_fill_i32, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
 }
 
+// per-angle work matrices of the Magnus methods (0: not a Magnus method)
+static int magnus_mats(int method)
+{
+    return method == TRHEPD_M2 ? 4 : method == TRHEPD_M4 ? 6 : method == TRHEPD_M6 ? 8 : 0;
+}
+
 int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
 {
-    if (t.method != TRHEPD_SP4 && t.method != TRHEPD_SP6 && t.method != TRHEPD_LF && t.method != TRHEPD_RK4)
-        return fail(ctx, TRHEPD_E_ARG, "large-N device path supports int_method 4, 5, 6 and 7 (got %d)", t.method);
+    if (t.method < TRHEPD_M2 || t.method > TRHEPD_RK4)
+        return fail(ctx, TRHEPD_E_ARG, "the batched device path supports int_method 1 to 7 (got %d)", t.method);
     // the batched RK4 stages (k_stage_gemm<BN, true>) faulted the device on their first run
     // and no cause has been shown; the path stays off until one is
     if (t.method == TRHEPD_RK4)
@@ -862,7 +997,10 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
                  oco = take((size_t)g.A * 8), oxi = take((size_t)g.A * 4), ore = take((size_t)g.A * 4),
                  opv = take((size_t)g.A * g.NP * 4), oli = take((size_t)g.A * 4), oal = take((size_t)g.A * 4);
     const size_t ork = t.method == TRHEPD_RK4 ? take(4 * mb) : 0;
-    R->mixed = ctx->precision == TRHEPD_PREC_MIXED;
+    const int nw = magnus_mats(t.method);
+    const size_t omg = nw ? take(nw * mb) : 0;
+    // the Magnus methods' products are all per-angle: they run in FP64 in either precision
+    R->mixed = ctx->precision == TRHEPD_PREC_MIXED && !magnus_mats(t.method);
     size_t oqd = 0, omd = 0, oeq = 0, oem = 0;
     if (R->mixed) {
         // digit buffers sized for the widest padding any candidate tile needs
@@ -902,6 +1040,7 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
     R->piv = (int *)(d + opv);
     R->list = (int *)(d + oli);
     R->all = (int *)(d + oal);
+    for (int w = 0; w < nw; ++w) R->W[w] = (double2 *)(d + omg + w * mb);
     if (t.method == TRHEPD_RK4) {
         R->Qb = (double2 *)(d + ork);
         R->Ps = (double2 *)(d + ork + mb);
@@ -936,7 +1075,7 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
         attr_bytes = lds_bytes(g.N);
     }
     b.run = R;
-    return autotune(ctx, b, t);
+    return magnus_mats(t.method) ? TRHEPD_OK : autotune(ctx, b, t);
 }
 
 void runner_destroy(trhepd_ctx *ctx, Batch &b)
@@ -976,6 +1115,125 @@ static void qreset(Batch &b, const int *list, int npts, double beta)
                        beta, R.Q, (const double2 *)R.P);
 }
 
+// index of rp among the Magnus work matrices
+static int magnus_rp(int method) { return method == TRHEPD_M2 ? 3 : method == TRHEPD_M4 ? 5 : 4; }
+
+static void bmm(Batch &b, const double2 *A, const double2 *B, double2 *C, double alpha, double beta)
+{
+    const Geo &g = b.run->g;
+    const int nbl = g.NP / 16;
+    const BmmArgs S{g, A, B, C, alpha, beta, 0.0};
+    hipLaunchKernelGGL(k_bmm, dim3((unsigned)((nbl * nbl + 3) / 4), g.A), dim3(256), 0, b.stream, S);
+}
+
+// out = sum of c x over the terms, + ident on the diagonal
+static void lin(Batch &b, double2 *out, std::initializer_list<std::pair<double, const double2 *>> terms,
+                double ident = 0.0)
+{
+    const Geo &g = b.run->g;
+    LinArgs L{};
+    L.g = g;
+    for (const auto &t : terms) {
+        L.c[L.n] = t.first;
+        L.x[L.n] = t.second;
+        ++L.n;
+    }
+    L.out = out;
+    L.ident = ident;
+    hipLaunchKernelGGL(k_lin, dim3((unsigned)((g.mat * g.A + 255) / 256)), dim3(256), 0, b.stream, L);
+}
+
+static void build_m(Batch &b, const VVi *col, double2 *M)
+{
+    const Runner &R = *b.run;
+    hipLaunchKernelGGL(k_build_m, dim3((unsigned)((R.g.mat * R.g.A + 255) / 256)), dim3(256), 0, b.stream, R.g,
+                       (const int *)b.gmap, col, (const double2 *)R.g2c, M);
+}
+
+// hyperbolic_exp4r (matcomp.f90:388-405): r = a^2, q = h I + h^3/6 a + h^5/120 a^2,
+// r = h/2 a - h^3/24 a^2
+static void hyp_exp4(Batch &b, double h, const double2 *a, double2 *q, double2 *r)
+{
+    const double h3 = h * h * h, h5 = h3 * h * h;
+    bmm(b, a, a, r, 1.0, 0.0);
+    lin(b, q, {{h3 / 6, a}, {h5 / 120, r}}, h);
+    lin(b, r, {{h / 2, a}, {-(h3 / 24), r}});
+}
+
+// hyperbolic_exp6r (matcomp.f90:407-428), t a scratch matrix
+static void hyp_exp6(Batch &b, double h, const double2 *a, double2 *q, double2 *r, double2 *t)
+{
+    const double cq0 = 1.0 / 6, cq1 = 1.0 / 120, cq2 = 1.0 / 5040, cr0 = 0.5, cr1 = -1.0 / 24, cr2 = 1.0 / 240;
+    const double h3 = h * h * h, h5 = h3 * h * h, h7 = h5 * h * h;
+    bmm(b, a, a, r, 1.0, 0.0);
+    bmm(b, r, a, t, 1.0, 0.0);
+    lin(b, q, {{h3 * cq0, a}, {h5 * cq1, r}, {h7 * cq2, t}}, h);
+    lin(b, r, {{h * cr0, a}, {h3 * cr1, r}, {h5 * cr2, t}});
+}
+
+// One slice of m2/m4/m6_int_spa.  After a renormalisation Q = I exactly, so the
+// reference's two branches (P += r + rp / Q = P q + I when Q = I, else the products) are
+// the same products here: I X is exact on the matrix cores.
+static void magnus_slice(Batch &b, const Table &t, int l)
+{
+    Runner &R = *b.run;
+    double2 **W = R.W;
+    const double dz = t.dz;
+    double2 *Q = R.Q, *P = R.P;
+    if (t.method == TRHEPD_M2) {   // surf_magnus.f90:114-178
+        double2 *m = W[0], *r = W[1], *q = W[2], *rp = W[3];
+        build_m(b, t.d + (size_t)(l - 1) * t.nv, m);
+        hyp_exp4(b, dz, m, q, r);
+        lin(b, rp, {{1.0, rp}, {1.0, r}});
+        bmm(b, Q, rp, P, 1.0, 1.0);
+        lin(b, rp, {{1.0, r}});
+        bmm(b, P, q, Q, 1.0, 1.0);
+        return;
+    }
+    const VVi *c0 = t.d + (size_t)(3 * (l - 1)) * t.nv;
+    if (t.method == TRHEPD_M4) {   // surf_magnus.f90:180-251
+        const double a0 = std::sqrt(15.0) / 36, a1 = 5.0 / 36;
+        double2 *m1 = W[0], *m2 = W[1], *m3 = W[2], *r = W[3], *q = W[4], *rp = W[5];
+        build_m(b, c0, m1);
+        build_m(b, c0 + t.nv, m2);
+        build_m(b, c0 + 2 * t.nv, m3);
+        lin(b, q, {{1.0, m3}, {-1.0, m1}});
+        lin(b, m3, {{1.0, m1}, {-2.0, m2}, {1.0, m3}});
+        lin(b, m1, {{1.0, q}});
+        hyp_exp4(b, dz, m2, q, r);
+        lin(b, rp, {{1.0, rp}, {1.0, r}, {-(a0 * dz), m1}, {a1 * dz, m3}});
+        bmm(b, Q, rp, P, 1.0, 1.0);
+        bmm(b, P, q, Q, 1.0, 1.0);
+        lin(b, rp, {{1.0, r}, {a0 * dz, m1}, {a1 * dz, m3}});
+        return;
+    }
+    // m6, surf_magnus.f90:253-339
+    const double a0 = std::sqrt(15.0) / 180, a1 = 1.0 / 18, a2 = 1.0 / 12960;
+    const double d0 = 4.0 / (3.0 * std::sqrt(15.0)), d1 = 1.0 / 6;
+    double2 *m1 = W[0], *m2 = W[1], *m3 = W[2], *tt = W[3], *rp = W[4], *r = W[5], *q = W[6], *tw = W[7];
+    build_m(b, c0, m1);
+    build_m(b, c0 + t.nv, m2);
+    build_m(b, c0 + 2 * t.nv, m3);
+    lin(b, r, {{1.0, m3}, {-1.0, m1}});
+    lin(b, q, {{1.0, m1}, {-2.0, m2}, {1.0, m3}});
+    lin(b, m1, {{1.0, r}});
+    lin(b, m2, {{1.0, m2}, {d1, q}});
+    bmm(b, m1, m1, m3, a2 * dz * dz * dz, 0.0);
+    lin(b, m3, {{1.0, m3}, {a1 * dz, q}});
+    lin(b, tt, {{1.0, m2}, {-d0, m1}});
+    hyp_exp6(b, dz / 2, tt, q, r, tw);
+    lin(b, rp, {{1.0, rp}, {1.0, r}, {-(a0 * dz), m1}, {1.0, m3}});
+    bmm(b, Q, rp, P, 1.0, 1.0);
+    bmm(b, P, q, Q, 1.0, 1.0);
+    lin(b, rp, {{1.0, r}});
+    lin(b, tt, {{1.0, m2}, {d0, m1}});
+    hyp_exp6(b, dz / 2, tt, q, r, tw);
+    lin(b, rp, {{1.0, rp}, {1.0, r}});
+    bmm(b, Q, rp, P, 1.0, 1.0);
+    bmm(b, P, q, Q, 1.0, 1.0);
+    lin(b, rp, {{1.0, r}, {dz * a0, m1}, {1.0, m3}});
+}
+
 int runner_begin(trhepd_ctx *ctx, Batch &b, const Table &t)
 {
     Runner &R = *b.run;
@@ -996,8 +1254,12 @@ int runner_begin(trhepd_ctx *ctx, Batch &b, const Table &t)
     hipLaunchKernelGGL(k_fill_i32, dim3((g.A + 255) / 256), dim3(256), 0, b.stream, R.xid, g.A, 1);
     // leapfrog starts from Q = I + dz/2 P (surf_magnus.f90:63-67), RK4 from Q = I
     // (surf_rk.f90:41-46)
-    if (t.method == TRHEPD_LF || t.method == TRHEPD_RK4)
+    if (t.method == TRHEPD_LF || t.method == TRHEPD_RK4 || magnus_mats(t.method))
         qreset(b, R.all, g.A, t.method == TRHEPD_LF ? 0.5 * t.dz : 0.0);
+    // the Magnus methods carry rp from slice to slice, starting at zero
+    if (magnus_mats(t.method))
+        hipLaunchKernelGGL(k_zero2, dim3((unsigned)((g.mat * g.A + 255) / 256)), dim3(256), 0, b.stream,
+                           R.W[magnus_rp(t.method)], g.mat * g.A);
     return TRHEPD_OK;
 }
 
@@ -1171,6 +1433,11 @@ int runner_slice_enqueue(trhepd_ctx *ctx, Batch &b, const Table &t, int l)
         const VVi *col = t.d + (size_t)(l - 1) * t.nv;
         if (R.mixed) launch_msplit(b.stream, R.mg, b.gmap, col, t.nv, 1, R.md, R.em);
         stage_launch(ctx, b, t, 1, col, t.dz, l == t.ns ? 0.5 * t.dz : t.dz, nullptr, 0);
+    } else if (magnus_mats(t.method)) {
+        magnus_slice(b, t, l);
+        const int nblk = g.NP / 16;
+        hipLaunchKernelGGL(k_gersh, dim3((unsigned)(((size_t)g.A * nblk * g.NP + 255) / 256)), dim3(256), 0,
+                           b.stream, g, (const double2 *)R.Q, R.colsum, R.diagabs);
     } else if (t.method == TRHEPD_RK4) {
         // rk4_int (surf_rk.f90:48-78): matrices at z, z + dz/2 (twice) and z + dz
         const VVi *c0 = t.d + (size_t)(2 * (l - 1)) * t.nv;
@@ -1212,7 +1479,7 @@ int runner_slice_finish(trhepd_ctx *ctx, Batch &b, const Table &t)
     const Geo &g = R.g;
     // leapfrog and RK4 restart from Q = I after a renormalisation (the RKN path rebuilds Q
     // in its first stage)
-    const bool reset = t.method == TRHEPD_LF || t.method == TRHEPD_RK4;
+    const bool reset = t.method == TRHEPD_LF || t.method == TRHEPD_RK4 || magnus_mats(t.method);
     if (R.forced >= 0) {
         if (!R.forced) return TRHEPD_OK;
         LuArgs L = R.lu(R.all);
@@ -1243,6 +1510,8 @@ int runner_end(trhepd_ctx *ctx, Batch &b, const Table &t)
     const Geo &g = R.g;
     const size_t ntot = g.mat * g.A;
     const dim3 eb(256), eg((unsigned)((ntot + 255) / 256));
+    // the Magnus methods' last half step (surf_magnus.f90:172, 245, 333)
+    if (magnus_mats(t.method)) lin(b, R.P, {{1.0, R.P}, {1.0, R.W[magnus_rp(t.method)]}});
     hipLaunchKernelGGL(k_st_setup, eg, eb, 0, b.stream, g, (const double2 *)R.gc, R.Q, R.P);
     LuArgs L = R.lu(R.all);
     L.flag = b.cnt + 1;
