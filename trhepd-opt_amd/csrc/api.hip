@@ -420,10 +420,9 @@ int trhepd_batch_create(trhepd_ctx *ctx, int table_id, int nb, int nang, const i
     if (!t) return fail(ctx, TRHEPD_E_NOTFOUND, "no table %d", table_id);
     if (nb < 1 || nang < 1 || !iv || !gamma || !f || nb0 < 1 || nb0 > nb || !batch_id)
         return fail(ctx, TRHEPD_E_ARG, "batch_create: bad arguments");
-    if (t->method == TRHEPD_RK4GS)
-        return fail(ctx, TRHEPD_E_ARG, "int_method %d is not available on the device path", t->method);
-    // the Magnus methods run on the batched path at every N
-    const bool magnus = t->method == TRHEPD_M2 || t->method == TRHEPD_M4 || t->method == TRHEPD_M6;
+    // the Magnus methods and rk4gs run on the batched path at every N
+    const bool magnus = t->method == TRHEPD_M2 || t->method == TRHEPD_M4 || t->method == TRHEPD_M6 ||
+                        t->method == TRHEPD_RK4GS;
     // signed gather map (row-major): upper i<j -> +iv(j,i), lower i>j -> -iv(i,j), diagonal 0
     std::vector<int> gm((size_t)nb * nb);
     for (int i = 0; i < nb; ++i)

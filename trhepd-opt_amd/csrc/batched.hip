@@ -149,8 +149,9 @@ __global__ void k_st_setup(Geo g, const double2 *__restrict__ gc, double2 *__res
 }
 
 // F(i,j) = conj(Y''(j,i)); intensities of column nb0
+// (rk4_intgs: f = Y^T, no conjugate, surf_rk.f90:154-155 with matAdivB2)
 __global__ void k_out(Geo g, const double2 *__restrict__ P, const double2 *__restrict__ gc, double2 *__restrict__ f,
-                      double *__restrict__ f2, int nb0, const double *__restrict__ wnsga2)
+                      double *__restrict__ f2, int nb0, const double *__restrict__ wnsga2, int conjugate)
 {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t nn = (size_t)g.N * g.N;
@@ -158,7 +159,7 @@ __global__ void k_out(Geo g, const double2 *__restrict__ P, const double2 *__res
     const int a = (int)(e / nn);
     const int r = (int)(e % nn), i = r % g.N, j = r / g.N;
     const double2 y = P[(size_t)a * g.mat + j + (size_t)i * g.NP];
-    f[e] = make_double2(y.x, -y.y);
+    f[e] = conjugate ? make_double2(y.x, -y.y) : y;
     if (f2 && j == nb0 - 1) {
         const double sr = gc[(size_t)a * g.NP + i].x;
         f2[(size_t)a * g.N + i] = (sr > 1e-10) ? (y.x * y.x + y.y * y.y) * wnsga2[a] / sr : 0.0;
@@ -841,24 +842,32 @@ struct BmmArgs {
     const double2 *A, *B;
     double2 *C;
     double alpha, beta, ident;
+    const int *list;   // angles (blockIdx.y indexes the list), or null for all
+    int conjA;         // use A^H instead of A
 };
 
 __global__ __launch_bounds__(256) void k_bmm(BmmArgs S)
 {
     const Geo g = S.g;
     const int nbl = g.NP / 16;
-    const int a = blockIdx.y;
+    const int a = S.list ? S.list[blockIdx.y] : (int)blockIdx.y;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, kq = lane >> 4;
     const int blk = blockIdx.x * 4 + wave;
     if (blk >= nbl * nbl) return;
     const int bi = blk % nbl, bj = blk / nbl;
-    const double2 *__restrict__ Ab = S.A + (size_t)a * g.mat + bi * 16 + l16;
+    // A(row, k) at Ab[k * ks]; with conjA the element is conj(A(k, row))
+    const double2 *__restrict__ Ab = S.A + (size_t)a * g.mat +
+                                     (S.conjA ? (size_t)(bi * 16 + l16) * g.NP : (size_t)(bi * 16 + l16));
+    const size_t ks = S.conjA ? 1 : g.NP;
+    const double sgn = S.conjA ? -1.0 : 1.0;
     const double2 *__restrict__ Bb = S.B + (size_t)a * g.mat + (size_t)(bj * 16 + l16) * g.NP;
     d4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
     for (int k0 = 0; k0 < g.NP; k0 += 4) {
         const int k = k0 + kq;
-        const double2 av = Ab[(size_t)k * g.NP], bv = Bb[k];
+        double2 av = Ab[(size_t)k * ks];
+        av.y *= sgn;
+        const double2 bv = Bb[k];
         cr = mfma4(av.x, bv.x, cr);
         ci = mfma4(av.x, bv.y, ci);
         cr = mfma4(-av.y, bv.y, cr);
@@ -931,6 +940,235 @@ __global__ void k_gersh(Geo g, const double2 *__restrict__ Q, double *__restrict
     colsum[ab * g.NP + j] = s;
 }
 
+// ---------------------------------------------------------------------------
+// rk4_intgs (surf_rk.f90:93-156): RK4 on [q; p] with left products M q, the Cholesky-QR
+// renormalisation of bcgs (matgs.f90:4-37) when the power estimate of ||[q; p]||^2 exceeds
+// 1e2 (power_est, matcomp.f90:530-594), and strans + matAdivB2 at the end.
+
+// vvi2matf (matcomp.f90:281-306): upper conj(vi - v), lower -(v + vi), diagonal
+// -(v + vi)[1] - gamma_j^2
+__global__ void k_build_mf(Geo g, const int *__restrict__ gmap, const VVi *__restrict__ col,
+                           const double2 *__restrict__ g2c, double2 *__restrict__ M)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.mat * g.A) return;
+    const int a = (int)(e / g.mat);
+    const int r = (int)(e % g.mat), i = r % g.NP, j = r / g.NP;
+    double2 m = make_double2(0, 0);
+    if (i < g.N && j < g.N) {
+        if (i == j) {
+            const VVi t = col[0];
+            const double2 c2 = g2c[(size_t)a * g.NP + j];   // conj(gamma^2)
+            m = make_double2(-(t.v.x + t.vi.x) - c2.x, -(t.v.y + t.vi.y) + c2.y);
+        } else {
+            const int s = gmap[i * g.N + j];
+            if (s > 0) {
+                const VVi t = col[s - 1];
+                m = make_double2(t.vi.x - t.v.x, -(t.vi.y - t.v.y));
+            } else {
+                const VVi t = col[-s - 1];
+                m = make_double2(-(t.v.x + t.vi.x), -(t.v.y + t.vi.y));
+            }
+        }
+    }
+    M[e] = m;
+}
+
+// invstrans (matcomp.f90:89-115): q = G^-1 (I + F), p = i (F - I); x = (1 + i) / sqrt(2N)
+__global__ void k_gs_setup(Geo g, const double2 *__restrict__ gamma, const double2 *__restrict__ f,
+                           double2 *__restrict__ Q, double2 *__restrict__ P, double2 *__restrict__ X)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.mat * g.A) return;
+    const int a = (int)(e / g.mat);
+    const int r = (int)(e % g.mat), i = r % g.NP, j = r / g.NP;
+    double2 q = make_double2(0, 0), p = make_double2(0, 0);
+    if (i < g.N && j < g.N) {
+        const double2 gi = gamma[(size_t)a * g.N + i];
+        const double2 ig = cdiv(make_double2(1, 0), gi);
+        const double2 fij = f[(size_t)a * g.N * g.N + i + (size_t)j * g.N];
+        const double2 t = cmul(ig, fij);
+        if (i == j) {
+            q = make_double2(ig.x + t.x, ig.y + t.y);
+            p = make_double2(-fij.y, -1.0 + fij.x);
+        } else {
+            q = t;
+            p = make_double2(-fij.y, fij.x);
+        }
+    }
+    Q[e] = q;
+    P[e] = p;
+    if (r < g.NP) {
+        const double v = 1.0 / sqrt(2.0 * g.N);
+        X[(size_t)a * g.NP + r] = r < g.N ? make_double2(v, v) : make_double2(0, 0);
+    }
+}
+
+// zpotrf 'U' of the listed angles' C = U^H U in place, in the order of the reference's
+// column loop; one workgroup per angle
+__global__ __launch_bounds__(256) void k_chol(Geo g, const int *__restrict__ list, double2 *__restrict__ C,
+                                              int *__restrict__ counters, int count)
+{
+    __shared__ double sd;
+    const int a = list[blockIdx.x];
+    double2 *c = C + (size_t)a * g.mat;
+    const int N = g.N, NP = g.NP;
+    for (int j = 0; j < N; ++j) {
+        if (threadIdx.x == 0) {
+            double d = c[j + (size_t)j * NP].x;
+            for (int r = 0; r < j; ++r) {
+                const double2 u = c[r + (size_t)j * NP];
+                d -= u.x * u.x + u.y * u.y;
+            }
+            d = sqrt(d);
+            c[j + (size_t)j * NP] = make_double2(d, 0.0);
+            sd = d;
+        }
+        __syncthreads();
+        const double d = sd;
+        for (int k = j + 1 + threadIdx.x; k < N; k += blockDim.x) {
+            double2 acc = c[j + (size_t)k * NP];
+            for (int r = 0; r < j; ++r) {
+                const double2 u = c[r + (size_t)j * NP], v = c[r + (size_t)k * NP];
+                // conj(u) v
+                acc.x -= u.x * v.x + u.y * v.y;
+                acc.y -= u.x * v.y - u.y * v.x;
+            }
+            c[j + (size_t)k * NP] = make_double2(acc.x / d, acc.y / d);
+        }
+        __syncthreads();
+    }
+    if (count && threadIdx.x == 0) atomicAdd(&counters[0], 1);
+}
+
+// ztrsm 'R','U','N','N': X <- X U^-1 for the listed angles; one thread per row, X = q or p
+__global__ __launch_bounds__(256) void k_trsm_r(Geo g, const int *__restrict__ list, const double2 *__restrict__ C,
+                                                double2 *__restrict__ X)
+{
+    const int a = list[blockIdx.y];
+    const int rrow = blockIdx.x * blockDim.x + threadIdx.x;
+    if (rrow >= g.N) return;
+    const double2 *u = C + (size_t)a * g.mat;
+    double2 *x = X + (size_t)a * g.mat + rrow;
+    const int NP = g.NP;
+    for (int j = 0; j < g.N; ++j) {
+        double2 v = x[(size_t)j * NP];
+        for (int k = 0; k < j; ++k) {
+            const double2 xk = x[(size_t)k * NP], ukj = u[k + (size_t)j * NP];
+            v.x -= xk.x * ukj.x - xk.y * ukj.y;
+            v.y -= xk.x * ukj.y + xk.y * ukj.x;
+        }
+        const double d = u[j + (size_t)j * NP].x;
+        x[(size_t)j * NP] = make_double2(v.x / d, v.y / d);
+    }
+}
+
+// power_est / power_step (matcomp.f90:530-594) for every angle, x carried between slices:
+// y = [q x; p x], z = q^H y1 + p^H y2, d = |x^H z|, x = z / |z|, until |z - x| < 1e-2 or
+// d grows by less than 1e-2 d, at most N steps.  Sets ren[a] = d > 1e2.
+__global__ __launch_bounds__(256) void k_power(Geo g, const double2 *__restrict__ Q, const double2 *__restrict__ P,
+                                               double2 *__restrict__ X, int *__restrict__ ren,
+                                               int *__restrict__ xid)
+{
+    extern __shared__ double2 sh[];   // x[NP], y[2 NP], z[NP]
+    __shared__ double red[3][256];
+    const int a = blockIdx.x, tid = threadIdx.x, nt = blockDim.x, N = g.N, NP = g.NP;
+    double2 *x = sh, *y = sh + NP, *z = sh + 3 * NP;
+    const double2 *q = Q + (size_t)a * g.mat, *p = P + (size_t)a * g.mat;
+    for (int i = tid; i < N; i += nt) x[i] = X[(size_t)a * NP + i];
+    __syncthreads();
+    double d = -1.0, d0 = -1.0;
+    for (int it = 0; it < N; ++it) {
+        for (int i = tid; i < N; i += nt) {
+            double2 s1 = make_double2(0, 0), s2 = make_double2(0, 0);
+            for (int j = 0; j < N; ++j) {
+                const double2 xj = x[j], qv = q[i + (size_t)j * NP], pv = p[i + (size_t)j * NP];
+                s1.x += qv.x * xj.x - qv.y * xj.y;
+                s1.y += qv.x * xj.y + qv.y * xj.x;
+                s2.x += pv.x * xj.x - pv.y * xj.y;
+                s2.y += pv.x * xj.y + pv.y * xj.x;
+            }
+            y[i] = s1;
+            y[NP + i] = s2;
+        }
+        __syncthreads();
+        double dn = 0.0, crx = 0.0, cry = 0.0;
+        for (int j = tid; j < N; j += nt) {
+            double2 s = make_double2(0, 0);
+            for (int i = 0; i < N; ++i) {   // conj(q(i,j)) y1(i)
+                const double2 qv = q[i + (size_t)j * NP], yv = y[i];
+                s.x += qv.x * yv.x + qv.y * yv.y;
+                s.y += qv.x * yv.y - qv.y * yv.x;
+            }
+            for (int i = 0; i < N; ++i) {   // + conj(p(i,j)) y2(i)
+                const double2 pv = p[i + (size_t)j * NP], yv = y[NP + i];
+                s.x += pv.x * yv.x + pv.y * yv.y;
+                s.y += pv.x * yv.y - pv.y * yv.x;
+            }
+            z[j] = s;
+            dn += s.x * s.x + s.y * s.y;
+            const double2 xj = x[j];   // conj(x) z
+            crx += xj.x * s.x + xj.y * s.y;
+            cry += xj.x * s.y - xj.y * s.x;
+        }
+        red[0][tid] = dn;
+        red[1][tid] = crx;
+        red[2][tid] = cry;
+        __syncthreads();
+        for (int o = nt / 2; o > 0; o >>= 1) {
+            if (tid < o)
+                for (int k = 0; k < 3; ++k) red[k][tid] += red[k][tid + o];
+            __syncthreads();
+        }
+        d = hypot(red[1][0], red[2][0]);
+        const double inv = 1.0 / sqrt(red[0][0]);
+        __syncthreads();
+        double dp = 0.0;
+        for (int j = tid; j < N; j += nt) {
+            const double2 zz = make_double2(z[j].x * inv, z[j].y * inv);
+            const double cx = zz.x - x[j].x, cy = zz.y - x[j].y;
+            dp += cx * cx + cy * cy;
+            x[j] = zz;
+        }
+        red[0][tid] = dp;
+        __syncthreads();
+        for (int o = nt / 2; o > 0; o >>= 1) {
+            if (tid < o) red[0][tid] += red[0][tid + o];
+            __syncthreads();
+        }
+        dp = sqrt(red[0][0]);
+        __syncthreads();
+        if (dp < 1e-2 || d - d0 < 1e-2 * d) break;
+        d0 = d;
+    }
+    for (int i = tid; i < N; i += nt) X[(size_t)a * NP + i] = x[i];
+    if (tid == 0) {
+        ren[a] = d > 1e2;
+        xid[a] = d > 1e2;
+    }
+}
+
+// strans (matcomp.f90:117-138): q' = G q + i p, p' = G q - i p, written transposed for
+// matAdivB2's solve p' q'^-1 = (q'^T \ p'^T)^T
+__global__ void k_gs_strans(Geo g, const double2 *__restrict__ gamma, const double2 *__restrict__ Q,
+                            const double2 *__restrict__ P, double2 *__restrict__ Qt, double2 *__restrict__ Pt)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.mat * g.A) return;
+    const int a = (int)(e / g.mat);
+    const int r = (int)(e % g.mat), i = r % g.NP, j = r / g.NP;
+    const size_t et = (size_t)a * g.mat + j + (size_t)i * g.NP;
+    if (i >= g.N || j >= g.N) {
+        Qt[et] = make_double2(0, 0);
+        Pt[et] = make_double2(0, 0);
+        return;
+    }
+    const double2 gi = gamma[(size_t)a * g.N + i], x = Q[e], y = P[e];
+    const double2 gx = cmul(gi, x);
+    Qt[et] = make_double2(gx.x - y.y, gx.y + y.x);
+    Pt[et] = make_double2(gx.x + y.y, gx.y - y.x);
+}
+
 }  // namespace
 
 struct Runner {
@@ -954,7 +1192,8 @@ struct Runner {
     double2 *Qb = nullptr, *Ps = nullptr, *SQ = nullptr, *SP = nullptr;
     // Magnus (int_method 1-3): per-angle work matrices (slice matrices, exponential blocks,
     // the carried rp)
-    double2 *W[8] = {};
+    double2 *W[10] = {};
+    double2 *X = nullptr;       // rk4_intgs: power-iteration vector per angle
     LuArgs lu(const int *lst) const { return LuArgs{g, Q, P, piv, lst, nullptr, dinv}; }
 };
 
@@ -971,11 +1210,15 @@ static int magnus_mats(int method)
 {
     return method == TRHEPD_M2 ? 4 : method == TRHEPD_M4 ? 6 : method == TRHEPD_M6 ? 8 : 0;
 }
+// rk4_intgs: slice matrices, stage q, product, stage p, the two derivative sums, Cholesky factor
+constexpr int GS_MATS = 9;
+// methods whose products are all per-angle (FP64 in either precision, no autotuned tiles)
+static bool dense_method(int method) { return magnus_mats(method) || method == TRHEPD_RK4GS; }
 
 int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
 {
-    if (t.method < TRHEPD_M2 || t.method > TRHEPD_RK4)
-        return fail(ctx, TRHEPD_E_ARG, "the batched device path supports int_method 1 to 7 (got %d)", t.method);
+    if (t.method < TRHEPD_M2 || t.method > TRHEPD_RK4GS)
+        return fail(ctx, TRHEPD_E_ARG, "unknown int_method %d", t.method);
     // the batched RK4 stages (k_stage_gemm<BN, true>) faulted the device on their first run
     // and no cause has been shown; the path stays off until one is
     if (t.method == TRHEPD_RK4)
@@ -997,10 +1240,11 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
                  oco = take((size_t)g.A * 8), oxi = take((size_t)g.A * 4), ore = take((size_t)g.A * 4),
                  opv = take((size_t)g.A * g.NP * 4), oli = take((size_t)g.A * 4), oal = take((size_t)g.A * 4);
     const size_t ork = t.method == TRHEPD_RK4 ? take(4 * mb) : 0;
-    const int nw = magnus_mats(t.method);
+    const int nw = t.method == TRHEPD_RK4GS ? GS_MATS : magnus_mats(t.method);
     const size_t omg = nw ? take(nw * mb) : 0;
-    // the Magnus methods' products are all per-angle: they run in FP64 in either precision
-    R->mixed = ctx->precision == TRHEPD_PREC_MIXED && !magnus_mats(t.method);
+    const size_t ox = t.method == TRHEPD_RK4GS ? take((size_t)g.A * g.NP * sizeof(double2)) : 0;
+    // the Magnus and rk4gs products are all per-angle: they run in FP64 in either precision
+    R->mixed = ctx->precision == TRHEPD_PREC_MIXED && !dense_method(t.method);
     size_t oqd = 0, omd = 0, oeq = 0, oem = 0;
     if (R->mixed) {
         // digit buffers sized for the widest padding any candidate tile needs
@@ -1041,6 +1285,7 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
     R->list = (int *)(d + oli);
     R->all = (int *)(d + oal);
     for (int w = 0; w < nw; ++w) R->W[w] = (double2 *)(d + omg + w * mb);
+    if (t.method == TRHEPD_RK4GS) R->X = (double2 *)(d + ox);
     if (t.method == TRHEPD_RK4) {
         R->Qb = (double2 *)(d + ork);
         R->Ps = (double2 *)(d + ork + mb);
@@ -1075,7 +1320,7 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
         attr_bytes = lds_bytes(g.N);
     }
     b.run = R;
-    return magnus_mats(t.method) ? TRHEPD_OK : autotune(ctx, b, t);
+    return dense_method(t.method) ? TRHEPD_OK : autotune(ctx, b, t);
 }
 
 void runner_destroy(trhepd_ctx *ctx, Batch &b)
@@ -1118,12 +1363,13 @@ static void qreset(Batch &b, const int *list, int npts, double beta)
 // index of rp among the Magnus work matrices
 static int magnus_rp(int method) { return method == TRHEPD_M2 ? 3 : method == TRHEPD_M4 ? 5 : 4; }
 
-static void bmm(Batch &b, const double2 *A, const double2 *B, double2 *C, double alpha, double beta)
+static void bmm(Batch &b, const double2 *A, const double2 *B, double2 *C, double alpha, double beta,
+                const int *list = nullptr, int npts = 0, int conjA = 0)
 {
     const Geo &g = b.run->g;
     const int nbl = g.NP / 16;
-    const BmmArgs S{g, A, B, C, alpha, beta, 0.0};
-    hipLaunchKernelGGL(k_bmm, dim3((unsigned)((nbl * nbl + 3) / 4), g.A), dim3(256), 0, b.stream, S);
+    const BmmArgs S{g, A, B, C, alpha, beta, 0.0, list, conjA};
+    hipLaunchKernelGGL(k_bmm, dim3((unsigned)((nbl * nbl + 3) / 4), list ? npts : g.A), dim3(256), 0, b.stream, S);
 }
 
 // out = sum of c x over the terms, + ident on the diagonal
@@ -1234,6 +1480,61 @@ static void magnus_slice(Batch &b, const Table &t, int l)
     lin(b, rp, {{1.0, r}, {dz * a0, m1}, {1.0, m3}});
 }
 
+// bcgs (matgs.f90:4-37) of the listed angles: C = q^H q + p^H p, C = U^H U, [q; p] <- [q; p] U^-1
+static void cqr(Batch &b, const int *list, int npts, bool count)
+{
+    Runner &R = *b.run;
+    const Geo &g = R.g;
+    double2 *C = R.W[GS_MATS - 1];
+    bmm(b, R.Q, R.Q, C, 1.0, 0.0, list, npts, 1);
+    bmm(b, R.P, R.P, C, 1.0, 1.0, list, npts, 1);
+    hipLaunchKernelGGL(k_chol, dim3(npts), dim3(256), 0, b.stream, g, list, C, b.cnt, count ? 1 : 0);
+    const dim3 tg((unsigned)((g.N + 255) / 256), npts);
+    hipLaunchKernelGGL(k_trsm_r, tg, dim3(256), 0, b.stream, g, list, (const double2 *)C, R.Q);
+    hipLaunchKernelGGL(k_trsm_r, tg, dim3(256), 0, b.stream, g, list, (const double2 *)C, R.P);
+}
+
+static void build_mf(Batch &b, const VVi *col, double2 *M)
+{
+    const Runner &R = *b.run;
+    hipLaunchKernelGGL(k_build_mf, dim3((unsigned)((R.g.mat * R.g.A + 255) / 256)), dim3(256), 0, b.stream, R.g,
+                       (const int *)b.gmap, col, (const double2 *)R.g2c, M);
+}
+
+// one slice of rk4_intgs (surf_rk.f90:114-143)
+static void gs_slice(Batch &b, const Table &t, int l)
+{
+    Runner &R = *b.run;
+    double2 **W = R.W;
+    double2 *m0 = W[0], *m1 = W[1], *m2 = W[2], *q2 = W[3], *sm = W[4], *ps = W[5], *SQ = W[6], *SP = W[7];
+    double2 *Q = R.Q, *P = R.P;
+    const double dz = t.dz, h = 0.5 * dz;
+    const VVi *c0 = t.d + (size_t)(2 * (l - 1)) * t.nv;
+    build_mf(b, c0, m0);
+    build_mf(b, c0 + t.nv, m1);
+    build_mf(b, c0 + 2 * t.nv, m2);
+    bmm(b, m0, Q, sm, 1.0, 0.0);   // s1 = m0 q, r1 = p
+    lin(b, SQ, {{1.0, P}});
+    lin(b, SP, {{1.0, sm}});
+    lin(b, q2, {{1.0, Q}, {h, P}});
+    lin(b, ps, {{1.0, P}, {h, sm}});   // r2
+    bmm(b, m1, q2, sm, 1.0, 0.0);      // s2
+    lin(b, SQ, {{1.0, SQ}, {2.0, ps}});
+    lin(b, SP, {{1.0, SP}, {2.0, sm}});
+    lin(b, q2, {{1.0, Q}, {h, ps}});
+    lin(b, ps, {{1.0, P}, {h, sm}});   // r3
+    bmm(b, m1, q2, sm, 1.0, 0.0);      // s3
+    lin(b, SQ, {{1.0, SQ}, {2.0, ps}});
+    lin(b, SP, {{1.0, SP}, {2.0, sm}});
+    lin(b, q2, {{1.0, Q}, {dz, ps}});
+    lin(b, ps, {{1.0, P}, {dz, sm}});  // r4
+    bmm(b, m2, q2, sm, 1.0, 0.0);      // s4
+    lin(b, SQ, {{1.0, SQ}, {1.0, ps}});
+    lin(b, SP, {{1.0, SP}, {1.0, sm}});
+    lin(b, Q, {{1.0, Q}, {dz / 6, SQ}});
+    lin(b, P, {{1.0, P}, {dz / 6, SP}});
+}
+
 int runner_begin(trhepd_ctx *ctx, Batch &b, const Table &t)
 {
     Runner &R = *b.run;
@@ -1244,6 +1545,13 @@ int runner_begin(trhepd_ctx *ctx, Batch &b, const Table &t)
                        R.g2c, R.gc);
     R.Q = R.Q0;
     R.Qn = R.Q1;
+    if (t.method == TRHEPD_RK4GS) {   // invstrans, bcgs, x of power_est(-100) (surf_rk.f90:109-111)
+        hipLaunchKernelGGL(k_gs_setup, eg, eb, 0, b.stream, g, (const double2 *)b.gamma, (const double2 *)b.fout, R.Q,
+                           R.P, R.X);
+        cqr(b, R.all, g.A, false);
+        TRY_HIP(ctx, hipGetLastError());
+        return TRHEPD_OK;
+    }
     hipLaunchKernelGGL(k_inv_setup, eg, eb, 0, b.stream, g, (const double2 *)b.fout, R.Q, R.P);
     hipLaunchKernelGGL(k_zero2, dim3((unsigned)((g.mat + 255) / 256)), dim3(256), 0, b.stream, R.M, g.mat);
     LuArgs L = R.lu(R.all);
@@ -1433,6 +1741,19 @@ int runner_slice_enqueue(trhepd_ctx *ctx, Batch &b, const Table &t, int l)
         const VVi *col = t.d + (size_t)(l - 1) * t.nv;
         if (R.mixed) launch_msplit(b.stream, R.mg, b.gmap, col, t.nv, 1, R.md, R.em);
         stage_launch(ctx, b, t, 1, col, t.dz, l == t.ns ? 0.5 * t.dz : t.dz, nullptr, 0);
+    } else if (t.method == TRHEPD_RK4GS) {
+        gs_slice(b, t, l);
+        // renormalisation: the power estimate (device) or every K-th slice
+        const int period = batch_period(ctx, b, t);
+        R.forced = period > 0 ? (l % period == 0 ? 1 : 0) : -1;
+        if (R.forced < 0) {
+            hipLaunchKernelGGL(k_power, dim3(g.A), dim3(256), (size_t)4 * g.NP * sizeof(double2), b.stream, g,
+                               (const double2 *)R.Q, (const double2 *)R.P, R.X, R.ren, R.xid);
+            TRY_HIP(ctx, hipMemcpyAsync(R.h, R.ren, g.A * 4, hipMemcpyDeviceToHost, b.stream));
+            TRY_HIP(ctx, hipEventRecord(R.evf, b.stream));
+        }
+        TRY_HIP(ctx, hipGetLastError());
+        return TRHEPD_OK;
     } else if (magnus_mats(t.method)) {
         magnus_slice(b, t, l);
         const int nblk = g.NP / 16;
@@ -1480,6 +1801,12 @@ int runner_slice_finish(trhepd_ctx *ctx, Batch &b, const Table &t)
     // leapfrog and RK4 restart from Q = I after a renormalisation (the RKN path rebuilds Q
     // in its first stage)
     const bool reset = t.method == TRHEPD_LF || t.method == TRHEPD_RK4 || magnus_mats(t.method);
+    const bool gs = t.method == TRHEPD_RK4GS;
+    if (gs && R.forced >= 0) {
+        if (R.forced) cqr(b, R.all, g.A, true);
+        TRY_HIP(ctx, hipGetLastError());
+        return TRHEPD_OK;
+    }
     if (R.forced >= 0) {
         if (!R.forced) return TRHEPD_OK;
         LuArgs L = R.lu(R.all);
@@ -1497,6 +1824,11 @@ int runner_slice_finish(trhepd_ctx *ctx, Batch &b, const Table &t)
         if (R.h[i]) lst[nl++] = i;
     if (!nl) return TRHEPD_OK;
     TRY_HIP(ctx, hipMemcpyAsync(R.list, lst, (size_t)nl * 4, hipMemcpyHostToDevice, b.stream));
+    if (gs) {
+        cqr(b, R.list, nl, true);
+        TRY_HIP(ctx, hipGetLastError());
+        return TRHEPD_OK;
+    }
     LuArgs L = R.lu(R.list);
     L.flag = b.cnt + 1;
     const int rc = lu_on(ctx, b.stream, L, nl);
@@ -1510,6 +1842,20 @@ int runner_end(trhepd_ctx *ctx, Batch &b, const Table &t)
     const Geo &g = R.g;
     const size_t ntot = g.mat * g.A;
     const dim3 eb(256), eg((unsigned)((ntot + 255) / 256));
+    if (t.method == TRHEPD_RK4GS) {   // strans, matAdivB2, f = p (surf_rk.f90:153-155)
+        double2 *Qt = R.W[0], *Pt = R.W[1];
+        hipLaunchKernelGGL(k_gs_strans, eg, eb, 0, b.stream, g, (const double2 *)b.gamma, (const double2 *)R.Q,
+                           (const double2 *)R.P, Qt, Pt);
+        LuArgs L{g, Qt, Pt, R.piv, R.all, b.cnt + 1, R.dinv};
+        int rc = lu_on(ctx, b.stream, L, g.A);
+        if (rc) return rc;
+        const size_t nout = (size_t)g.N * g.N * g.A;
+        hipLaunchKernelGGL(k_out, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, b.stream, g,
+                           (const double2 *)Pt, (const double2 *)R.gc, b.fout, b.has_f2 ? b.f2 : nullptr, b.nb0,
+                           (const double *)b.w, 0);
+        TRY_HIP(ctx, hipGetLastError());
+        return TRHEPD_OK;
+    }
     // the Magnus methods' last half step (surf_magnus.f90:172, 245, 333)
     if (magnus_mats(t.method)) lin(b, R.P, {{1.0, R.P}, {1.0, R.W[magnus_rp(t.method)]}});
     hipLaunchKernelGGL(k_st_setup, eg, eb, 0, b.stream, g, (const double2 *)R.gc, R.Q, R.P);
@@ -1519,7 +1865,7 @@ int runner_end(trhepd_ctx *ctx, Batch &b, const Table &t)
     if (rc) return rc;
     const size_t nout = (size_t)g.N * g.N * g.A;
     hipLaunchKernelGGL(k_out, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, b.stream, g, (const double2 *)R.P,
-                       (const double2 *)R.gc, b.fout, b.has_f2 ? b.f2 : nullptr, b.nb0, (const double *)b.w);
+                       (const double2 *)R.gc, b.fout, b.has_f2 ? b.f2 : nullptr, b.nb0, (const double *)b.w, 1);
     TRY_HIP(ctx, hipGetLastError());
     return TRHEPD_OK;
 }
