@@ -3,7 +3,9 @@ tests/golden/ref/ (data only: decks, bulkP.b and the recorded FP64 rocking curve
 
 Source: test/test{23,47,521}/ of the reference repository, produced by its authors'
 builds (see test/test23/setup.sh and perf-test.sh there), and the one-beam sample
-sim-trhepd-rheed/sample/Ge_001_c4x2/ (deck, bulkP.b and the recorded curve, E12.4).  Run once in a container
+sim-trhepd-rheed/sample/Ge_001_c4x2/ (deck, bulkP.b, the recorded curve, E12.4, and its angular
+convolution), and str-orig/sample/T4Al_on_Si_111/output (a recorded curve and its convolution by
+that program's tool).  Run once in a container
 where the reference is mounted; the GPU hosts only read the copies.
 """
 import os
@@ -31,6 +33,12 @@ GE = os.path.join(os.path.dirname(SRC.rstrip("/")), "sim-trhepd-rheed", "sample"
 os.makedirs(os.path.join(DST, "ge001"), exist_ok=True)
 for f in ("bulk.txt", "surf.txt"):
     shutil.copy(os.path.join(GE, f), os.path.join(DST, "ge001", f))
-for f in ("bulkP.b", "surf-bulkP.s"):
+for f in ("bulkP.b", "surf-bulkP.s", "convolution.txt"):
     shutil.copy(os.path.join(GE, "output", f), os.path.join(DST, "ge001", f))
+# the angular convolution of the original program's tool (str-orig/tool/make_convolution.py)
+# on its T4 Al / Si(111) sample: the recorded curve and its convolution
+T4 = os.path.join(os.path.dirname(SRC.rstrip("/")), "str-orig", "sample", "T4Al_on_Si_111", "output")
+os.makedirs(os.path.join(DST, "t4al"), exist_ok=True)
+for f in ("surf-bulkP.s", "convolution.txt"):
+    shutil.copy(os.path.join(T4, f), os.path.join(DST, "t4al", f))
 print("copied to", DST)

@@ -148,11 +148,19 @@ class BulkDeck:
         return BulkDeck(nh, nk, ndom, nb, rdom, ih, ik, be, azi, azf, daz, gi, gf, dg, dz, ml,
                         els, nsg, aa, bb, gam, cc, dx, dy, atoms)
 
+    def _beam_lines(self) -> List[str]:
+        # one record per domain: each domain's beams are one list-directed read (bulkio.f90)
+        out, ib = [], 0
+        for n in self.nb:
+            out.append(",".join("%d,%d" % (self.ih[j], self.ik[j]) for j in range(ib, ib + n)) + "\t,(IH(I),IK(I)")
+            ib += n
+        return out
+
     def format(self) -> str:
         o = ["%d,%d,%d\t\t,NH,NK,NDOM" % (self.nh, self.nk, self.ndom),
              ",".join(str(n) for n in self.nb) + "\t\t,NB",
              ",".join(_g(x) for x in self.rdom) + "\t\t,RDOM",
-             ",".join("%d,%d" % (h, k) for h, k in zip(self.ih, self.ik)) + "\t,(IH(I),IK(I)",
+             *self._beam_lines(),
              ",".join(_g(x) for x in (self.be, self.azi, self.azf, self.daz, self.gi, self.gf, self.dg))
              + "\t,BE,AZI,AZF,DAZ,GI,GF,DG",
              "%s,%d\t\t,DZ,ML" % (_g(self.dz), self.ml),
@@ -442,17 +450,18 @@ def fortran_e(x: float, w: int, d: int) -> str:
     return (("-" if x < 0 else "") + "0." + digits.replace(".", "") + es).rjust(w)
 
 
-def format_s_row(angle_deg: float, f2: Sequence[float]) -> str:
-    """One angle row of ``(E12.4,200(",",E23.15))`` with format reversion every 200 values.
-    Output stops at the first data edit descriptor without a list item, so a record that
-    ends inside the 200-group carries the group's leading "," at its end."""
+def format_s_row(angle_deg: float, f2: Sequence[float], w: int = 23, d: int = 15) -> str:
+    """One angle row of ``(E12.4,200(",",Ew.d))`` (E23.15 for srfref, E12.4 for domainsum)
+    with format reversion every 200 values.  Output stops at the first data edit descriptor
+    without a list item, so a record that ends inside the 200-group carries the group's
+    leading "," at its end."""
     lines = []
     cur = fortran_e(angle_deg, 12, 4)
     for i, v in enumerate(f2):
         if i > 0 and i % 200 == 0:
             lines.append(cur)
             cur = ""
-        cur += "," + fortran_e(float(v), 23, 15)
+        cur += "," + fortran_e(float(v), w, d)
     if len(f2) % 200 != 0 or len(f2) == 0:
         cur += ","
     lines.append(cur)
@@ -473,6 +482,31 @@ def write_s(path: str, naz: int, ng: int, ih: Sequence[int], ik: Sequence[int],
         out.append("")
     with open(path, "w") as fh:
         fh.write("\n".join(out) + "\n")
+
+
+def format_md(naz: int, ng: int, nh: int, nk: int, gam_deg: float, nb: Sequence[int], rdom_deg: Sequence[float],
+              wdom: Sequence[float], ih: Sequence[int], ik: Sequence[int], rows_by_domain) -> str:
+    """Text of a multi-domain ``.md`` file (surfio.f90:276-288 header, then each domain's rows
+    as srfref writes them, surflf.f90:154-165).  ``rows_by_domain[idom][irep1]`` is a list of
+    ``(angle_deg, f2)``; ih/ik list every domain's beams in turn."""
+    ndom = len(nb)
+    out = ["#azimuths,g-angles,domains,nh,nk,gam", "%4d%4d%4d%4d%4d%10.3f" % (naz, ng, ndom, nh, nk, gam_deg),
+           "#beams,domain angle,domain weight"]
+    for d in range(ndom):
+        out.append("%4d%10.3f%s" % (nb[d], rdom_deg[d], fortran_e(wdom[d], 12, 3)))
+    out.append("#ih,ik")
+    ib = 0
+    for d in range(ndom):
+        pairs = [v for j in range(ib, ib + nb[d]) for v in (ih[j], ik[j])]
+        for i in range(0, len(pairs), 400):
+            out.append("".join("%4d" % v for v in pairs[i:i + 400]))
+        ib += nb[d]
+    for rows in rows_by_domain:
+        for block in rows:
+            for ang, f2 in block:
+                out.append(format_s_row(ang, f2))
+            out.append("")
+    return "\n".join(out) + "\n"
 
 
 _NUM = re.compile(r"[-+]?(?:\d+\.\d*|\.\d+|\d+)(?:[eEdD]?[-+]\d+)?")
@@ -499,7 +533,12 @@ class RockingCurve:
 
 def read_s(path: str) -> RockingCurve:
     """Read a single-domain ``.s`` file the way test/test23/compare.py does."""
-    text = open(path).read().splitlines()
+    return read_s_text(open(path).read())
+
+
+def read_s_text(s_text: str) -> RockingCurve:
+    """`read_s` of the file's text."""
+    text = s_text.splitlines()
     it = iter(text)
 
     def nextline():

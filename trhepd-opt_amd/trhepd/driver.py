@@ -16,7 +16,7 @@ import json
 import copy
 import math
 import os
-from typing import List, Optional, Sequence, Tuple
+from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -440,3 +440,38 @@ class SurfaceJob:
 
     def write_s(self, path: str, rows) -> None:
         deckio.write_s(path, self.naz, self.ng, list(self.ih), list(self.ik), rows)
+
+
+def surface_program(eng: "lib.Engine", bp: deckio.BulkP, sd: deckio.SurfDeck, dz2: float, method: int,
+                    wdom: Optional[Sequence[float]] = None) -> Dict[str, Optional[str]]:
+    """What the reference's surface program writes for a bulkP.b and a surf.txt
+    (surfm.f90:55-90): the ``.s`` text of a single domain; for several domains the ``.md``
+    text (every domain's curve, surfio.f90:276-288) and its domain sum as the ``.s``
+    (domainsum.f90).  The weights are the deck's unless `wdom` is given."""
+    from . import post
+    per_dom = []
+    jobs = []
+    for d in range(bp.ndom):
+        job = SurfaceJob(eng, bp, sd, dz2, method, idom=d)
+        try:
+            per_dom.append(job.run(bp.domains[d].blocks))
+        finally:
+            job.release()
+        jobs.append(job)
+    j0 = jobs[0]
+    if bp.ndom == 1:
+        nb = bp.nb[0]
+        out = ["#azimuths,g-angles,beams", "%4d%4d%4d" % (j0.naz, j0.ng, nb), "#ih,ik"]
+        pairs = [v for i in range(nb) for v in (bp.ih[i], bp.ik[i])]
+        for i in range(0, len(pairs), 400):
+            out.append("".join("%4d" % v for v in pairs[i:i + 400]))
+        for block in per_dom[0]:
+            for ang, f2 in block:
+                out.append(deckio.format_s_row(ang, f2))
+            out.append("")
+        return {"s": "\n".join(out) + "\n", "md": None}
+    w = list(wdom) if wdom is not None else list(sd.wdom)
+    deg = 180.0 / math.pi
+    md = deckio.format_md(j0.naz, j0.ng, bp.nh, bp.nk, bp.gam * deg, bp.nb, [r * deg for r in bp.rdom], w,
+                          bp.ih, bp.ik, per_dom)
+    return {"s": post.domainsum(md), "md": md}
