@@ -1673,8 +1673,8 @@ static void rk4_stage(trhepd_ctx *ctx, Batch &b, const VVi *col, int rk, double 
 }
 
 // Stage-product tile variants: FP64 tile widths 32 / 48 / 64 columns; mixed (column
-// blocks, row blocks) per workgroup (2..4, 8 or 4).
-static int n_variants(const Runner &R) { return R.mixed ? 6 : 3; }
+// blocks, row blocks, Q prefetch depth) per workgroup (2..4, 8 or 4, 1 or 2).
+static int n_variants(const Runner &R) { return R.mixed ? 12 : 3; }
 
 static bool apply_variant(Runner &R, int v)
 {
@@ -1682,9 +1682,9 @@ static bool apply_variant(Runner &R, int v)
         R.bn = v == 0 ? 32 : v == 1 ? 48 : 64;
         return true;
     }
-    const int wc = 2 + v / 2, wr = (v & 1) ? 4 : 8;
-    const MixedGeo c = mixed_geo(R.g.A, R.g.N, R.g.NP, R.mg.ND, wc, wr);
-    if (c.WR != wr) return false;   // that tile does not fit the registers
+    const int pf = 1 + v / 6, u = v % 6, wc = 2 + u / 2, wr = (u & 1) ? 4 : 8;
+    const MixedGeo c = mixed_geo(R.g.A, R.g.N, R.g.NP, R.mg.ND, wc, wr, pf);
+    if (c.WR != wr || c.PF != pf) return false;   // that tile does not fit the registers
     R.mg = c;
     return true;
 }

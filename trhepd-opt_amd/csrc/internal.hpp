@@ -180,9 +180,12 @@ struct MixedGeo {
     int WC;       // column blocks per workgroup tile (2..4)
     int WR;       // row blocks (waves) per workgroup tile (4 or 8)
     int ND;       // digits per operand (5..7)
+    int PF;       // chunks of Q prefetch (1 or 2)
 };
-// wc / wr: column / row blocks per workgroup tile (0: the default choice)
-MixedGeo mixed_geo(int A, int N, int NP, int ND, int wc = 0, int wr = 0);
+// tiles with room in 256 VGPRs for a third set of Q fragments
+constexpr bool mixed_pf2_fits(int ND, int WC) { return WC * ND <= 18; }
+// wc / wr / pf: column / row blocks per workgroup tile, Q prefetch depth (0: the default)
+MixedGeo mixed_geo(int A, int N, int NP, int ND, int wc = 0, int wr = 0, int pf = 0);
 size_t mixed_qd_bytes(const MixedGeo &g);   // digits of Q: 3 ND planes
 size_t mixed_md_bytes(const MixedGeo &g);   // digits of one stage's M0: 3 ND planes
 struct MixedStage {

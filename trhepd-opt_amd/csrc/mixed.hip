@@ -41,6 +41,16 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b)
 __device__ __forceinline__ double2 w_upper(const VVi &t) { return make_double2(-(t.v.x + t.vi.x), t.v.y + t.vi.y); }
 __device__ __forceinline__ double2 w_lower(const VVi &t) { return make_double2(-t.v.x + t.vi.x, -t.v.y + t.vi.y); }
 
+// Workgroup barrier ordering LDS only.  The stage kernel's waves share nothing but LDS
+// inside the k loop, so the barrier need not wait for global loads in flight (the Q and M0
+// fragments of later chunks), as a full __syncthreads() does on this target.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // exponent e with max * 2^-e < 1/2 (0 for an all-zero row)
 __device__ __forceinline__ int scale_exp(double mx)
 {
@@ -174,7 +184,7 @@ struct StageLaunch {
 // each wave loads its Q fragments straight into registers.  Both are fetched one chunk
 // ahead, so the loads of chunk c+1 overlap the MFMAs of chunk c.  The chunks of the three
 // real products form one pipelined sequence of 3 KC steps.
-template <int ND, int WC, int WR>
+template <int ND, int WC, int WR, int PF>
 __global__ __launch_bounds__(64 * WR, 1) void k_stage_i8l(StageLaunch Lc)
 {
     constexpr int NBF = WC * ND, NR = (NBF + WR - 1) / WR;
@@ -216,11 +226,6 @@ __global__ __launch_bounds__(64 * WR, 1) void k_stage_i8l(StageLaunch Lc)
             if (f < NBF) lds[((size_t)buf * NBF + f) * 64 + lane] = stage[r];
         }
     };
-    v4i qa[ND], qn[ND];
-    qload(0, qa);
-    gload(0);
-    sstore(0);
-    __syncthreads();
     v4i acc[ND][WC];
     double re[WC][4], im[WC][4];
 #pragma unroll
@@ -230,11 +235,15 @@ __global__ __launch_bounds__(64 * WR, 1) void k_stage_i8l(StageLaunch Lc)
 #pragma unroll
         for (int r = 0; r < 4; ++r) re[cb][r] = im[cb][r] = 0.0;
     }
-    for (int c = 0; c < nchunk; ++c) {
-        if (c + 1 < nchunk) {
-            gload(c + 1);
-            qload(c + 1, qn);
-        }
+    // One k chunk: the M0 fragments of chunk c+1 are fetched into registers (then LDS), the
+    // Q fragments of chunk c+PF into the register set `qfut`, the products of chunk c run
+    // on `qcur`.  The M0 loads are issued first, so that waiting for them at the end of the
+    // chunk leaves the Q loads in flight: Q gets PF chunks of latency, M0 one.
+    // Past the last chunk the loads repeat the last chunk (in bounds, unused), so that every
+    // chunk is straight-line code and the compiler's wait counts stay exact.
+    auto chunk = [&](int c, v4i(&qcur)[ND], v4i(&qfut)[ND]) {
+        gload(min(c + 1, nchunk - 1));
+        qload(min(c + PF, nchunk - 1), qfut);
         const v4i *Lb = lds + (size_t)(c & 1) * NBF * 64 + lane;
 #pragma unroll
         for (int q = 0; q < ND; ++q) {
@@ -244,7 +253,7 @@ __global__ __launch_bounds__(64 * WR, 1) void k_stage_i8l(StageLaunch Lc)
 #pragma unroll
             for (int p = 0; p + q < ND; ++p)
 #pragma unroll
-                for (int cb = 0; cb < WC; ++cb) acc[p + q][cb] = mfma_i8(b[cb], qa[p], acc[p + q][cb]);
+                for (int cb = 0; cb < WC; ++cb) acc[p + q][cb] = mfma_i8(b[cb], qcur[p], acc[p + q][cb]);
         }
         const int m = c / g.KC;
         if (c - m * g.KC == g.KC - 1) {
@@ -265,12 +274,31 @@ __global__ __launch_bounds__(64 * WR, 1) void k_stage_i8l(StageLaunch Lc)
 #pragma unroll
                 for (int s = 0; s < ND; ++s) acc[s][cb] = v4i{0, 0, 0, 0};
         }
-        if (c + 1 < nchunk) {
-            sstore((c + 1) & 1);
-#pragma unroll
-            for (int p = 0; p < ND; ++p) qa[p] = qn[p];
+        sstore((c + 1) & 1);
+        lds_barrier();
+    };
+    // the register sets rotate with the loop unrolled, so no set is copied (a copy would
+    // wait for the loads in flight)
+    v4i qa[ND], qb[ND];
+    gload(0);
+    qload(0, qa);
+    if constexpr (PF == 1) {
+        sstore(0);
+        lds_barrier();
+        for (int c = 0; c < nchunk; c += 2) {
+            chunk(c, qa, qb);
+            if (c + 1 < nchunk) chunk(c + 1, qb, qa);
         }
-        __syncthreads();
+    } else {
+        v4i qc[ND];
+        qload(1, qb);   // nchunk >= 3
+        sstore(0);
+        lds_barrier();
+        for (int c = 0; c < nchunk; c += 3) {   // nchunk = 3 KC
+            chunk(c, qa, qc);
+            chunk(c + 1, qb, qa);
+            chunk(c + 2, qc, qb);
+        }
     }
     // FP64 epilogue: lane (l16, kq) holds row i = block row + l16, columns 4 kq + r
     if (rb0 + wave >= g.RB) return;
@@ -324,7 +352,7 @@ __global__ __launch_bounds__(64 * WR, 1) void k_stage_i8l(StageLaunch Lc)
 
 }  // namespace
 
-MixedGeo mixed_geo(int A, int N, int NP, int ND, int wc, int wr)
+MixedGeo mixed_geo(int A, int N, int NP, int ND, int wc, int wr, int pf)
 {
     MixedGeo g;
     g.ND = ND;
@@ -347,6 +375,9 @@ MixedGeo mixed_geo(int A, int N, int NP, int ND, int wc, int wr)
     // are asked for, or the 7-digit 4-block accumulators need the registers of one wave per
     // SIMD.  The LDS holds only the M0 fragments, 2 WC ND KB.
     g.WR = (wr == 4 || (ND == 7 && g.WC == 4)) ? 4 : 8;
+    // a Q prefetch two chunks deep needs a third set of Q registers: eight-wave tiles whose
+    // registers have room for it
+    g.PF = (pf == 2 && g.WR == 8 && mixed_pf2_fits(ND, g.WC)) ? 2 : 1;
     return g;
 }
 
@@ -372,7 +403,7 @@ void launch_qsplit(hipStream_t st, const MixedGeo &g, const double2 *Q, int8_t *
     }
 }
 
-template <int ND, int WC, int WR>
+template <int ND, int WC, int WR, int PF>
 static void stage_i8_on(hipStream_t st, const StageLaunch &Lc)
 {
     constexpr int NF = WC * ND;
@@ -380,20 +411,23 @@ static void stage_i8_on(hipStream_t st, const StageLaunch &Lc)
     static_assert(2 * NF <= 160, "tile exceeds the LDS");
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute((const void *)k_stage_i8l<ND, WC, WR>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds);
+        (void)hipFuncSetAttribute((const void *)k_stage_i8l<ND, WC, WR, PF>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         attr = true;
     }
-    hipLaunchKernelGGL((k_stage_i8l<ND, WC, WR>), dim3(8 * Lc.tiles_per_xcd), dim3(64 * WR), lds, st, Lc);
+    hipLaunchKernelGGL((k_stage_i8l<ND, WC, WR, PF>), dim3(8 * Lc.tiles_per_xcd), dim3(64 * WR), lds, st, Lc);
 }
 
 template <int ND, int WC>
 static void stage_i8_wr(hipStream_t st, const StageLaunch &Lc)
 {
-    if constexpr (!(ND == 7 && WC == 4)) {
-        if (Lc.S.g.WR == 8) return stage_i8_on<ND, WC, 8>(st, Lc);
+    if constexpr (mixed_pf2_fits(ND, WC)) {
+        if (Lc.S.g.WR == 8 && Lc.S.g.PF == 2) return stage_i8_on<ND, WC, 8, 2>(st, Lc);
     }
-    stage_i8_on<ND, WC, 4>(st, Lc);
+    if constexpr (!(ND == 7 && WC == 4)) {
+        if (Lc.S.g.WR == 8) return stage_i8_on<ND, WC, 8, 1>(st, Lc);
+    }
+    stage_i8_on<ND, WC, 4, 1>(st, Lc);
 }
 
 template <int ND>
