@@ -145,7 +145,7 @@ def _rk4_sensitivity(surf):
     return a, O.rel_err(b, a), np.linalg.norm(a - b) / np.linalg.norm(a)
 
 
-RK4_OFF = "batched RK4 (N > 48) faulted the device on its first run; switched off until a cause is shown"
+RK4_OFF = "RK4 above 48 beams is not offered: its batched stage kernel faulted the device without a cause shown"
 
 
 @pytest.mark.skip(reason=RK4_OFF)
@@ -197,7 +197,7 @@ def test_leapfrog_mixed_n521(engine, mixed_engine):
 
 
 def test_rk4_large_n_is_refused(engine):
-    """While the batched RK4 path is off, N > 48 with int_method 7 fails cleanly."""
+    """RK4 above 48 beams fails cleanly at batch creation."""
     from trhepd import lib
     surf = O.load_case(os.path.join(REF, "test521"), 0.68, 7)
     angs, G, F, W = _inputs(surf)

@@ -227,52 +227,9 @@ struct StageArgs {
     double *colsum;       // [A*NP/16][NP]
     double *diagabs;      // [A*NP]
     int ntiles, ncolt, tiles_per_xcd;
-    // classical RK4 stage rk (1..4): Q is the stage operand, Qst / P the state, Qn the next
-    // stage operand, Ps the stage P, SQ / SP the running sums of the Q and P derivatives
-    const double2 *Qst;
-    double2 *Ps, *SQ, *SP;
-    int rk;
-    double dz;
 };
 
-// Epilogue of RK4 stage S.rk at element idx, X = (Q M)(i, j) of the stage operand
-// (rk4_int, surf_rk.f90:27-91): k_r = [P_r, Q_r M_r], Q_{r+1} = Q + c dz P_r,
-// P_{r+1} = P + c dz X_r with c = 1/2, 1/2, 1; the update t1 += dz/6 (k1 + 2k2 + 2k3 + k4).
-// Returns the new state Q at stage 4.
-__device__ __forceinline__ double2 rk4_epilogue(const StageArgs &S, size_t idx, double2 x)
-{
-    const double h = 0.5 * S.dz;
-    const double2 q0 = S.Qst[idx], p0 = S.P[idx];
-    switch (S.rk) {
-    case 1:
-        S.Ps[idx] = make_double2(p0.x + h * x.x, p0.y + h * x.y);
-        S.Qn[idx] = make_double2(q0.x + h * p0.x, q0.y + h * p0.y);
-        S.SP[idx] = x;
-        S.SQ[idx] = p0;
-        return q0;
-    case 2:
-    case 3: {
-        const double c = S.rk == 2 ? h : S.dz;
-        const double2 pp = S.Ps[idx];
-        S.Qn[idx] = make_double2(q0.x + c * pp.x, q0.y + c * pp.y);
-        S.Ps[idx] = make_double2(p0.x + c * x.x, p0.y + c * x.y);
-        double2 sp = S.SP[idx], sq = S.SQ[idx];
-        S.SP[idx] = make_double2(sp.x + 2.0 * x.x, sp.y + 2.0 * x.y);
-        S.SQ[idx] = make_double2(sq.x + 2.0 * pp.x, sq.y + 2.0 * pp.y);
-        return q0;
-    }
-    default: {
-        const double w = S.dz / 6;
-        const double2 pp = S.Ps[idx], sp = S.SP[idx], sq = S.SQ[idx];
-        const double2 qn = make_double2(q0.x + w * (sq.x + pp.x), q0.y + w * (sq.y + pp.y));
-        S.P[idx] = make_double2(p0.x + w * (sp.x + x.x), p0.y + w * (sp.y + x.y));
-        S.Qn[idx] = qn;
-        return qn;
-    }
-    }
-}
-
-template <int BN, bool RK>
+template <int BN>
 __global__ __launch_bounds__(256) void k_stage_gemm(StageArgs S)
 {
     constexpr int NB = BN / 16;
@@ -370,15 +327,7 @@ __global__ __launch_bounds__(256) void k_stage_gemm(StageArgs S)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int i = ib + kq + 4 * r;
-                    if (i < g.N && RK) {
-                        const size_t idx = (size_t)a * g.mat + i + (size_t)j * g.NP;
-                        const double2 qd = cmul(S.Q[idx], dj);
-                        const double2 qn = rk4_epilogue(S, idx, make_double2(cr[rb][cb][r] + qd.x, ci[rb][cb][r] + qd.y));
-                        if (S.rk == 4) {
-                            if (i == j) S.diagabs[(size_t)a * g.NP + j] = hypot(qn.x, qn.y);
-                            else csum += sqrt(qn.x * qn.x + qn.y * qn.y);
-                        }
-                    } else if (i < g.N) {
+                    if (i < g.N) {
                         const size_t idx = (size_t)a * g.mat + i + (size_t)j * g.NP;
                         const double2 q = S.Q[idx];
                         const double2 qd = cmul(q, dj);
@@ -397,7 +346,7 @@ __global__ __launch_bounds__(256) void k_stage_gemm(StageArgs S)
                     }
                 }
             }
-            if (S.mode == 1 || (RK && S.rk == 4)) {
+            if (S.mode == 1) {
                 csum += __shfl_xor(csum, 16);
                 csum += __shfl_xor(csum, 32);
                 if (kq == 0 && j < g.NP) S.colsum[(size_t)(Rb >> 4) * g.NP + j] = csum;
@@ -1188,8 +1137,6 @@ struct Runner {
     int8_t *qd = nullptr, *md = nullptr;
     int *eq = nullptr, *em = nullptr;
     int bn = 48;                // FP64 stage GEMM tile width (autotuned)
-    // classical RK4 (int_method 7): second stage operand, stage P, derivative sums
-    double2 *Qb = nullptr, *Ps = nullptr, *SQ = nullptr, *SP = nullptr;
     // Magnus (int_method 1-3): per-angle work matrices (slice matrices, exponential blocks,
     // the carried rp)
     double2 *W[10] = {};
@@ -1219,8 +1166,8 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
 {
     if (t.method < TRHEPD_M2 || t.method > TRHEPD_RK4GS)
         return fail(ctx, TRHEPD_E_ARG, "unknown int_method %d", t.method);
-    // the batched RK4 stages (k_stage_gemm<BN, true>) faulted the device on their first run
-    // and no cause has been shown; the path stays off until one is
+    // RK4 above 48 beams is not offered: a batched RK4 stage kernel faulted the device on its
+    // first run without a cause shown, and was removed
     if (t.method == TRHEPD_RK4)
         return fail(ctx, TRHEPD_E_ARG, "int_method 7 runs on the device for N <= 48 only (N = %d)", b.nb);
     if (b.nb > 600) return fail(ctx, TRHEPD_E_ARG, "N = %d exceeds the device path limit of 600 beams", b.nb);
@@ -1239,7 +1186,6 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
                  ocs = take((size_t)g.A * (g.NP / 16) * g.NP * 8), oda = take((size_t)g.A * g.NP * 8),
                  oco = take((size_t)g.A * 8), oxi = take((size_t)g.A * 4), ore = take((size_t)g.A * 4),
                  opv = take((size_t)g.A * g.NP * 4), oli = take((size_t)g.A * 4), oal = take((size_t)g.A * 4);
-    const size_t ork = t.method == TRHEPD_RK4 ? take(4 * mb) : 0;
     const int nw = t.method == TRHEPD_RK4GS ? GS_MATS : magnus_mats(t.method);
     const size_t omg = nw ? take(nw * mb) : 0;
     const size_t ox = t.method == TRHEPD_RK4GS ? take((size_t)g.A * g.NP * sizeof(double2)) : 0;
@@ -1286,12 +1232,6 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
     R->all = (int *)(d + oal);
     for (int w = 0; w < nw; ++w) R->W[w] = (double2 *)(d + omg + w * mb);
     if (t.method == TRHEPD_RK4GS) R->X = (double2 *)(d + ox);
-    if (t.method == TRHEPD_RK4) {
-        R->Qb = (double2 *)(d + ork);
-        R->Ps = (double2 *)(d + ork + mb);
-        R->SQ = (double2 *)(d + ork + 2 * mb);
-        R->SP = (double2 *)(d + ork + 3 * mb);
-    }
     if (R->mixed) {
         R->qd = (int8_t *)(d + oqd);
         R->md = (int8_t *)(d + omd);
@@ -1560,9 +1500,8 @@ int runner_begin(trhepd_ctx *ctx, Batch &b, const Table &t)
     if (rc) return rc;
     hipLaunchKernelGGL(k_inv_scale, eg, eb, 0, b.stream, g, (const double2 *)R.gc, R.P);
     hipLaunchKernelGGL(k_fill_i32, dim3((g.A + 255) / 256), dim3(256), 0, b.stream, R.xid, g.A, 1);
-    // leapfrog starts from Q = I + dz/2 P (surf_magnus.f90:63-67), RK4 from Q = I
-    // (surf_rk.f90:41-46)
-    if (t.method == TRHEPD_LF || t.method == TRHEPD_RK4 || magnus_mats(t.method))
+    // leapfrog starts from Q = I + dz/2 P (surf_magnus.f90:63-67), the Magnus methods from Q = I
+    if (t.method == TRHEPD_LF || magnus_mats(t.method))
         qreset(b, R.all, g.A, t.method == TRHEPD_LF ? 0.5 * t.dz : 0.0);
     // the Magnus methods carry rp from slice to slice, starting at zero
     if (magnus_mats(t.method))
@@ -1572,7 +1511,7 @@ int runner_begin(trhepd_ctx *ctx, Batch &b, const Table &t)
 }
 
 // the FP64 stage GEMM at the batch's tuned tile width
-static void gemm_launch(trhepd_ctx *ctx, Batch &b, StageArgs &S, bool rk)
+static void gemm_launch(trhepd_ctx *ctx, Batch &b, StageArgs &S)
 {
     const Runner &R = *b.run;
     const Geo &g = R.g;
@@ -1582,15 +1521,9 @@ static void gemm_launch(trhepd_ctx *ctx, Batch &b, StageArgs &S, bool rk)
     S.tiles_per_xcd = (S.ntiles + 7) / 8;
     const dim3 grid(8 * S.tiles_per_xcd), blk(256);
     prof_mark(ctx, b.stream);
-    if (rk) {
-        if (R.bn == 32) hipLaunchKernelGGL((k_stage_gemm<32, true>), grid, blk, 0, b.stream, S);
-        else if (R.bn == 64) hipLaunchKernelGGL((k_stage_gemm<64, true>), grid, blk, 0, b.stream, S);
-        else hipLaunchKernelGGL((k_stage_gemm<48, true>), grid, blk, 0, b.stream, S);
-    } else {
-        if (R.bn == 32) hipLaunchKernelGGL((k_stage_gemm<32, false>), grid, blk, 0, b.stream, S);
-        else if (R.bn == 64) hipLaunchKernelGGL((k_stage_gemm<64, false>), grid, blk, 0, b.stream, S);
-        else hipLaunchKernelGGL((k_stage_gemm<48, false>), grid, blk, 0, b.stream, S);
-    }
+    if (R.bn == 32) hipLaunchKernelGGL(k_stage_gemm<32>, grid, blk, 0, b.stream, S);
+    else if (R.bn == 64) hipLaunchKernelGGL(k_stage_gemm<64>, grid, blk, 0, b.stream, S);
+    else hipLaunchKernelGGL(k_stage_gemm<48>, grid, blk, 0, b.stream, S);
     prof_mark(ctx, b.stream);
 }
 
@@ -1641,35 +1574,8 @@ static void stage_launch(trhepd_ctx *ctx, Batch &b, const Table &t, int mode, co
     S.mode = mode;
     S.colsum = R.colsum;
     S.diagabs = R.diagabs;
-    gemm_launch(ctx, b, S, false);
+    gemm_launch(ctx, b, S);
     if (mode != 2) std::swap(R.Q, R.Qn);
-}
-
-// RK4 stage rk of a slice: Qout = next stage operand, state Q / P updated at stage 4
-static void rk4_stage(trhepd_ctx *ctx, Batch &b, const VVi *col, int rk, double dz, const double2 *Qin,
-                      double2 *Qout)
-{
-    Runner &R = *b.run;
-    const Geo &g = R.g;
-    hipLaunchKernelGGL(k_gather, dim3((unsigned)((g.mat + 255) / 256)), dim3(256), 0, b.stream, g, b.gmap, col, R.M);
-    StageArgs S{};
-    S.g = g;
-    S.Q = Qin;
-    S.Qn = Qout;
-    S.P = R.P;
-    S.M0 = R.M;
-    S.g2c = R.g2c;
-    S.col = col;
-    S.mode = 3;
-    S.colsum = R.colsum;
-    S.diagabs = R.diagabs;
-    S.Qst = R.Q;
-    S.Ps = R.Ps;
-    S.SQ = R.SQ;
-    S.SP = R.SP;
-    S.rk = rk;
-    S.dz = dz;
-    gemm_launch(ctx, b, S, true);
 }
 
 // Stage-product tile variants: FP64 tile widths 32 / 48 / 64 columns; mixed (column
@@ -1759,13 +1665,6 @@ int runner_slice_enqueue(trhepd_ctx *ctx, Batch &b, const Table &t, int l)
         const int nblk = g.NP / 16;
         hipLaunchKernelGGL(k_gersh, dim3((unsigned)(((size_t)g.A * nblk * g.NP + 255) / 256)), dim3(256), 0,
                            b.stream, g, (const double2 *)R.Q, R.colsum, R.diagabs);
-    } else if (t.method == TRHEPD_RK4) {
-        // rk4_int (surf_rk.f90:48-78): matrices at z, z + dz/2 (twice) and z + dz
-        const VVi *c0 = t.d + (size_t)(2 * (l - 1)) * t.nv;
-        rk4_stage(ctx, b, c0, 1, t.dz, R.Q, R.Qn);
-        rk4_stage(ctx, b, c0 + t.nv, 2, t.dz, R.Qn, R.Qb);
-        rk4_stage(ctx, b, c0 + t.nv, 3, t.dz, R.Qb, R.Qn);
-        rk4_stage(ctx, b, c0 + 2 * t.nv, 4, t.dz, R.Qn, R.Q);
     } else {
         const VVi *col1 = t.d + (size_t)(s * (l - 1)) * t.nv;
         // the s stage products of slice l use table columns s(l-1)+1 .. s l
@@ -1798,9 +1697,9 @@ int runner_slice_finish(trhepd_ctx *ctx, Batch &b, const Table &t)
 {
     Runner &R = *b.run;
     const Geo &g = R.g;
-    // leapfrog and RK4 restart from Q = I after a renormalisation (the RKN path rebuilds Q
-    // in its first stage)
-    const bool reset = t.method == TRHEPD_LF || t.method == TRHEPD_RK4 || magnus_mats(t.method);
+    // leapfrog and the Magnus methods restart from Q = I after a renormalisation (the RKN
+    // path rebuilds Q in its first stage)
+    const bool reset = t.method == TRHEPD_LF || magnus_mats(t.method);
     const bool gs = t.method == TRHEPD_RK4GS;
     if (gs && R.forced >= 0) {
         if (R.forced) cqr(b, R.all, g.A, true);
