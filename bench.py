@@ -2,14 +2,18 @@
 
 Workload (the headline configuration): a seeded synthetic Si-surface structure with N = 521
 beams, 12 surface atoms of Z = 14, 8, 1, 40 glancing angles 0.5-4.4 deg, sp4 (int_method 5)
-at dz = 0.02 A.  One step = one full rocking curve of that structure = one batch_run over
-the 40 angles with inputs (bulk reflection matrices, gamma, potential table) resident in
-HBM.  value = points (angles) per second.
+at dz = 0.02 A.  One step = one full rocking curve of that structure (the 40 angles as two
+concurrent batches) with inputs (bulk reflection matrices, gamma, potential table) resident
+in HBM.  value = points (angles) per second of the mixed-precision path; the FP64 path is
+timed the same way.
 
-Also reported: the dominant kernel's roofline (stage-product GEMM, FP64 matrix peak
-78.6 TFLOP/s; algorithmic flops per launch = 8 N^3 x angles) measured with HIP events on
-the library's stream during the timed steps, and the CPU oracle timed on a bounded sample
-of the same workload on this host.
+Also reported: the dominant kernel's roofline (the int8 digit-sliced stage product,
+3 ND (ND + 1) N^3 int8 ops per angle against the 5.0 POP/s dense int8 peak; the FP64 stage
+product, 8 N^3 flops per angle against 78.6 TFLOP/s, for the FP64 path) from HIP events
+around every launch on the batch streams during the timed steps; the same kernels measured
+in one untimed curve run as a single batch (no concurrent stream sharing the device); the
+uneven-size batch in one captured graph; the serving sample; and the CPU oracle timed on a
+bounded sample of the same workload on this host.
 """
 import argparse
 import json
@@ -119,6 +123,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-one-batch", action="store_true",
+                    help="skip the untimed single-batch curve that measures the kernels without stream sharing")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-slices", type=int, default=256)
     ap.add_argument("--renorm-period", type=int, default=-1,
@@ -173,6 +179,12 @@ def main():
             eng.set_mixed_digits(args.mixed_digits)
         log("path %s" % p)
         res[p] = run_path(eng, st, blocks, args, method, dz2)
+        if args.split > 1 and not args.no_one_batch:
+            # the same kernels in one batch of all angles, one untimed curve: per-launch
+            # durations without a concurrent stream sharing the device
+            one = argparse.Namespace(**vars(args))
+            one.split, one.steps, one.warmup = 1, 1, 0
+            res[p]["one_batch"] = run_path(eng, st, blocks, one, method, dz2)
         if eng is not eng64:
             eng.close()
     head = res.get("mixed", res.get("fp64"))
@@ -196,7 +208,7 @@ def main():
         cfg["mixed_digits"] = args.mixed_digits
         cfg["mixed_l2_err_vs_fp64"] = float(np.max(np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)))
 
-    def roofline(r, mixed):
+    def roofline(r, mixed, shared=True):
         avg_ms = r["kms"] / max(r["launches"], 1)
         if mixed:
             # nd (nd + 1) / 2 digit pairs x 3 real products x 2 N^3 int8 ops per angle
@@ -209,8 +221,9 @@ def main():
             ops = 8.0 * N ** 3 * r["per_launch"]
             peak, unit, kern = PEAK_FP64, "TFLOP/s", "k_stage_gemm (FP64 stage product, 8 N^3 flops per angle)"
         ach = ops / (avg_ms * 1e-3) / 1e12
-        return {"bound": "mfma", "kernel": kern + "; concurrent batches overlap, so the per-launch duration is an "
-                                                  "upper bound on the kernel's own time",
+        note = ("; concurrent batches overlap, so the per-launch duration is an upper bound on the kernel's own "
+                "time" if shared and r["nbatch"] > 1 else "; one batch, no concurrent stream")
+        return {"bound": "mfma", "kernel": kern + note,
                 "achieved": ach, "peak": peak, "unit": unit, "frac": ach / peak, "traffic": None,
                 "launches": r["launches"], "avg_launch_ms": avg_ms}
 
@@ -224,6 +237,10 @@ def main():
     }
     if "mixed" in res and "fp64" in res:
         out["roofline_fp64"] = roofline(res["fp64"], False)
+    if "one_batch" in head:
+        out["roofline_one_batch"] = roofline(head["one_batch"], "mixed" in res, shared=False)
+        if "fp64" in res and "mixed" in res and "one_batch" in res["fp64"]:
+            out["roofline_fp64_one_batch"] = roofline(res["fp64"]["one_batch"], False, shared=False)
     if args.uneven_structures > 0:
         from trhepd import uneven
         t0 = time.perf_counter()
