@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cmath>
 #include <initializer_list>
+#include <type_traits>
 #include <utility>
 
 #include "internal.hpp"
@@ -403,7 +404,13 @@ struct LuArgs {
     const int *list;
     int *flag;         // singular pivot flag
     double2 *dinv;     // [npts][nblk][16*16] inverses of U's diagonal blocks
+    int *perm;         // [A][LU_PERM] the current panel's row interchanges as one permutation
 };
+
+// Layout of one angle's perm record: [0] the number n <= 32 of rows the panel's interchanges
+// touch, [8 + i] the row that slot i writes, [40 + i] the row whose old content it receives.
+// Slots 0..kw-1 are the panel's own rows k0..k0+kw-1, in order.
+constexpr int LU_PERM = 72;
 
 __device__ __forceinline__ double2 *colp(const LuArgs &L, int a, int c)
 {
@@ -411,90 +418,180 @@ __device__ __forceinline__ double2 *colp(const LuArgs &L, int a, int c)
                        : L.P + (size_t)a * L.g.mat + (size_t)(c - L.g.N) * L.g.NP;
 }
 
-// Factor the panel of columns [k0, k0+kw) (rows k0..N-1) in LDS `pan`, stored
-// column-major with an odd row pitch so that a wave reading consecutive rows, or 16
-// threads reading one row across the columns, hit distinct banks.  Per column: one pivot
-// search (|re|+|im|, first maximum), the row swap inside the panel, then the scaling and
-// rank-1 update of the remaining panel columns, each thread owning whole rows.  Writes the
-// factored panel and the pivots back.  All 256 threads call it.
-__device__ void factor_panel(const LuArgs &L, int a, int k0, double2 *pan, double *rbest, int *ribest)
+// Factors the panel [k0, k0+16) (rows k0..N-1) of every listed angle: one workgroup per
+// angle, the panel held in registers (thread t owns panel rows t, t + 256 and t + 512, all
+// 16 columns).  Per column: the pivot search (|re|+|im|, the first maximum), the row
+// interchange (the pivot row is published in LDS to every thread, the displaced row goes to
+// the pivot's owner), then each thread scales and updates its own rows.  LDS holds only the
+// published rows and the per-wave search results, double-buffered by column parity, so
+// that two barriers per column suffice.  Writes the factored panel and the pivots back.
+constexpr int LU_RPT = 3;   // rows per thread: panels of up to 768 rows
+
+struct PanelLds {
+    double2 prow[2][16], crow[2][16];
+    double wbest[2][4];
+    int wbi[2][4];
+    int pidx[2], cidx[2];   // original panel rows of the published rows
+    int ntouch;
+};
+
+// A thread's panel rows are three separate arrays, visited through `rows3` with the row
+// number as a constant: no row is ever addressed by a run-time index, so all three stay
+// in registers.
+struct PanelRows {
+    double2 (&r0)[16], (&r1)[16], (&r2)[16];
+};
+template <class F> __device__ __forceinline__ void rows3(const PanelRows &v, F f)
 {
-    const int N = L.g.N, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int kw = min(16, N - k0), rows = N - k0, ld = rows | 1;
-    // all loads of a 256-row chunk are issued before any LDS store
-    for (int r0 = 0; r0 < rows; r0 += 256) {
-        const int r = r0 + tid;
-        double2 buf[16];
-#pragma unroll
-        for (int c = 0; c < 16; ++c)
-            buf[c] = (c < kw && r < rows) ? colp(L, a, k0 + c)[k0 + r] : make_double2(0, 0);
-        if (r < rows)
-#pragma unroll
-            for (int c = 0; c < 16; ++c)
-                if (c < kw) pan[c * ld + r] = buf[c];
-    }
-    __syncthreads();
-    for (int c = 0; c < kw; ++c) {
-        double best = -1.0;
-        int bi = rows;
-        for (int r = c + tid; r < rows; r += 256) {
-            const double2 v = pan[c * ld + r];
-            const double m = fabs(v.x) + fabs(v.y);
-            if (m > best) { best = m; bi = r; }
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-            const double ob = __shfl_xor(best, o);
-            const int oi = __shfl_xor(bi, o);
-            if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
-        }
-        if (lane == 0) { rbest[wave] = best; ribest[wave] = bi; }
-        __syncthreads();
-        best = rbest[0];
-        bi = ribest[0];
-        for (int w = 1; w < 4; ++w)
-            if (rbest[w] > best || (rbest[w] == best && ribest[w] < bi)) { best = rbest[w]; bi = ribest[w]; }
-        if (tid == 0) {
-            L.piv[(size_t)a * L.g.NP + k0 + c] = k0 + bi;
-            if (best == 0.0) *L.flag = 1;
-        }
-        if (bi != c && tid < kw) {
-            const double2 t0 = pan[tid * ld + c];
-            pan[tid * ld + c] = pan[tid * ld + bi];
-            pan[tid * ld + bi] = t0;
-        }
-        __syncthreads();
-        const double2 pk = pan[c * ld + c];
-        const bool nz = (pk.x != 0.0 || pk.y != 0.0);
-        const double2 rp = nz ? cdiv(make_double2(1, 0), pk) : make_double2(0, 0);
-        for (int r = c + 1 + tid; r < rows; r += 256) {
-            double2 l = pan[c * ld + r];
-            if (nz) {
-                l = cmul(l, rp);
-                pan[c * ld + r] = l;
-            }
-            for (int cc = c + 1; cc < kw; ++cc) {
-                const double2 u = pan[cc * ld + c];
-                double2 v = pan[cc * ld + r];
-                v.x -= l.x * u.x - l.y * u.y;
-                v.y -= l.x * u.y + l.y * u.x;
-                pan[cc * ld + r] = v;
-            }
-        }
-        __syncthreads();
-    }
-    for (int c = 0; c < kw; ++c) {
-        double2 *dst = colp(L, a, k0 + c) + k0;
-        for (int r = tid; r < rows; r += 256) dst[r] = pan[c * ld + r];
-    }
+    f(std::integral_constant<int, 0>{}, v.r0);
+    f(std::integral_constant<int, 1>{}, v.r1);
+    f(std::integral_constant<int, 2>{}, v.r2);
 }
 
-// factors the panel [k0, k0+16) of every listed angle (one workgroup per angle)
+// column C of the panel (a template, so that every column index is a constant)
+template <int C>
+__device__ __forceinline__ void panel_column(const LuArgs &L, int a, int k0, int kw, int rows, const PanelRows &v,
+                                             int (&oi)[LU_RPT], PanelLds &sh)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr int s = C & 1;
+    double best = -1.0;
+    int bi = rows;
+    rows3(v, [&](auto J, double2 (&x)[16]) {
+        const int r = tid + 256 * J;
+        if (r >= C && r < rows) {
+            const double m = fabs(x[C].x) + fabs(x[C].y);
+            if (m > best) {
+                best = m;
+                bi = r;
+            }
+        }
+    });
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ob = __shfl_xor(best, o);
+        const int oi = __shfl_xor(bi, o);
+        if (ob > best || (ob == best && oi < bi)) {
+            best = ob;
+            bi = oi;
+        }
+    }
+    if (lane == 0) {
+        sh.wbest[s][wave] = best;
+        sh.wbi[s][wave] = bi;
+    }
+    __syncthreads();
+    best = sh.wbest[s][0];
+    bi = sh.wbi[s][0];
+    for (int w = 1; w < 4; ++w)
+        if (sh.wbest[s][w] > best || (sh.wbest[s][w] == best && sh.wbi[s][w] < bi)) {
+            best = sh.wbest[s][w];
+            bi = sh.wbi[s][w];
+        }
+    if (bi >= rows) bi = C;   // no candidate compares (NaN column): no interchange
+    if (tid == 0) {
+        L.piv[(size_t)a * L.g.NP + k0 + C] = k0 + bi;
+        if (!(best > 0.0)) *L.flag = 1;
+    }
+    // publish the pivot row; row C (thread C's first row) goes to the pivot row's owner
+    rows3(v, [&](auto J, double2 (&x)[16]) {
+        if (tid + 256 * J == bi) {
+#pragma unroll
+            for (int cc = 0; cc < 16; ++cc) sh.prow[s][cc] = x[cc];
+            sh.pidx[s] = oi[J];
+        }
+    });
+    if (tid == C && bi != C) {
+#pragma unroll
+        for (int cc = 0; cc < 16; ++cc) sh.crow[s][cc] = v.r0[cc];
+        sh.cidx[s] = oi[0];
+    }
+    __syncthreads();
+    if (bi != C) {
+        if (tid == C) {
+#pragma unroll
+            for (int cc = 0; cc < 16; ++cc) v.r0[cc] = sh.prow[s][cc];
+            oi[0] = sh.pidx[s];
+        }
+        rows3(v, [&](auto J, double2 (&x)[16]) {
+            if (tid + 256 * J == bi) {
+#pragma unroll
+                for (int cc = 0; cc < 16; ++cc) x[cc] = sh.crow[s][cc];
+                oi[J] = sh.cidx[s];
+            }
+        });
+    }
+    const double2 pk = sh.prow[s][C];
+    const bool nz = (pk.x != 0.0 || pk.y != 0.0);
+    const double2 rp = nz ? cdiv(make_double2(1, 0), pk) : make_double2(0, 0);
+    double2 u[16];
+#pragma unroll
+    for (int cc = C + 1; cc < 16; ++cc) u[cc] = sh.prow[s][cc];
+    rows3(v, [&](auto J, double2 (&x)[16]) {
+        const int r = tid + 256 * J;
+        if (r > C && r < rows) {
+            double2 l = x[C];
+            if (nz) {
+                l = cmul(l, rp);
+                x[C] = l;
+            }
+#pragma unroll
+            for (int cc = C + 1; cc < 16; ++cc) {
+                if (cc < kw) {
+                    x[cc].x -= l.x * u[cc].x - l.y * u[cc].y;
+                    x[cc].y -= l.x * u[cc].y + l.y * u[cc].x;
+                }
+            }
+        }
+    });
+}
+
+template <int C>
+__device__ __forceinline__ void panel_columns(const LuArgs &L, int a, int k0, int kw, int rows, const PanelRows &v,
+                                              int (&oi)[LU_RPT], PanelLds &sh)
+{
+    // kw is uniform, so every thread takes the same branches to the barriers
+    if (C < kw) panel_column<C>(L, a, k0, kw, rows, v, oi, sh);
+    if constexpr (C + 1 < 16) panel_columns<C + 1>(L, a, k0, kw, rows, v, oi, sh);
+}
+
 __global__ __launch_bounds__(256) void k_lu_panel(LuArgs L, int k0)
 {
-    extern __shared__ double2 dyn[];
-    __shared__ double rbest[4];
-    __shared__ int ribest[4];
-    factor_panel(L, L.list[blockIdx.x], k0, dyn, rbest, ribest);
+    __shared__ PanelLds sh;
+    const int a = L.list[blockIdx.x];
+    const int N = L.g.N, tid = threadIdx.x;
+    const int kw = min(16, N - k0), rows = N - k0;
+    double2 v0[16], v1[16], v2[16];
+    const PanelRows v{v0, v1, v2};
+    rows3(v, [&](auto J, double2 (&x)[16]) {
+        const int r = tid + 256 * J;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) x[c] = (c < kw && r < rows) ? colp(L, a, k0 + c)[k0 + r] : make_double2(0, 0);
+    });
+    int oi[LU_RPT];   // the original panel row of each row the thread holds
+#pragma unroll
+    for (int j = 0; j < LU_RPT; ++j) oi[j] = tid + 256 * j;
+    if (tid == 0) sh.ntouch = kw;
+    panel_columns<0>(L, a, k0, kw, rows, v, oi, sh);
+    // the interchanges as one permutation for the trailing updates: the panel's own rows
+    // take slots 0..kw-1, every other row whose content changed one of the following slots
+    int *pm = L.perm + (size_t)a * LU_PERM;
+    rows3(v, [&](auto J, double2 (&x)[16]) {
+        const int r = tid + 256 * J;
+        if (r < rows) {
+#pragma unroll
+            for (int c = 0; c < 16; ++c)
+                if (c < kw) colp(L, a, k0 + c)[k0 + r] = x[c];
+            int slot = -1;
+            if (r < kw) slot = r;
+            else if (oi[J] != r) slot = atomicAdd(&sh.ntouch, 1);
+            if (slot >= 0) {
+                pm[8 + slot] = k0 + r;
+                pm[40 + slot] = k0 + oi[J];
+            }
+        }
+    });
+    __syncthreads();
+    if (tid == 0) pm[0] = sh.ntouch;
 }
 
 // One elimination step for a 64-column tile right of panel [k0, k0+kw): the panel's row
@@ -509,43 +606,24 @@ __global__ __launch_bounds__(256) void k_lu_step(LuArgs L, int k0)
     __shared__ double2 swp[32 * LU_TILE];
     __shared__ double2 U[16][LU_TILE];
     __shared__ double2 L11[16][16];
-    __shared__ int trow[32], tsrc[32], ntouch;
+    __shared__ int trow[32], tsrc[32];
     const int a = L.list[blockIdx.y];
     const int N = L.g.N, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int kw = min(16, N - k0), first = k0 + kw;
     const int c0 = first + blockIdx.x * LU_TILE;
     const int cw = min(LU_TILE, 2 * N - c0);
-    for (int e = tid; e < 256; e += 256) L11[e >> 4][e & 15] = make_double2(0, 0);
-    __syncthreads();
-    for (int e = tid; e < kw * kw; e += 256) {
-        const int r = e / kw, c = e % kw;
-        L11[r][c] = colp(L, a, k0 + c)[k0 + r];
+    // the panel's interchanges, gathered by the panel kernel as one permutation
+    const int *pm = L.perm + (size_t)a * LU_PERM;
+    const int nt = pm[0];
+    if (tid < nt) {
+        trow[tid] = pm[8 + tid];
+        tsrc[tid] = pm[40 + tid];
     }
-    if (tid == 0) {
-        // the panel rows k0..k0+kw-1 take slots 0..kw-1, rows they displace follow
-        int n = 0;
-        for (int c = 0; c < kw; ++c) {
-            trow[n] = k0 + c;
-            tsrc[n] = k0 + c;
-            ++n;
-        }
-        auto idx = [&](int r) {
-            for (int i = 0; i < n; ++i)
-                if (trow[i] == r) return i;
-            trow[n] = r;
-            tsrc[n] = r;
-            return n++;
-        };
-        for (int c = 0; c < kw; ++c) {
-            const int i1 = idx(k0 + c), i2 = idx(L.piv[(size_t)a * L.g.NP + k0 + c]);
-            const int t0 = tsrc[i1];
-            tsrc[i1] = tsrc[i2];
-            tsrc[i2] = t0;
-        }
-        ntouch = n;
+    {
+        const int r = tid >> 4, c = tid & 15;
+        L11[r][c] = (r < kw && c < kw) ? colp(L, a, k0 + c)[k0 + r] : make_double2(0, 0);
     }
     __syncthreads();
-    const int nt = ntouch;
     {
         // up to 32 x 64 elements: 8 per thread, loads issued together
         double2 buf[8];
@@ -589,52 +667,62 @@ __global__ __launch_bounds__(256) void k_lu_step(LuArgs L, int k0)
     const int ks = (kw + 3) >> 2;
     const int ncb = (cw + 15) / 16;
     // A22 -= L21 U12: wave w owns the 16-column block w of the tile and walks down all row
-    // blocks two at a time (independent accumulators); its U12 operands stay in registers
+    // blocks four at a time; the L21 fragments and the C values of the four blocks are loaded
+    // together, before their products, and the wave's U12 operands stay in registers
     const int cb = wave;
     if (cb < ncb) {
         const int c = cb * 16 + l16;
+        const bool cv_ok = c < cw;
         double2 uv[4];
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int k = s * 4 + kq;
             uv[s] = (s < ks && k < kw) ? U[k][c] : make_double2(0, 0);
         }
-        double2 *col = colp(L, a, c0 + (c < cw ? c : 0));
-        for (int rb = 0; rb < nrb; rb += 2) {
+        double2 *col = colp(L, a, c0 + (cv_ok ? c : 0));
+        for (int rb = 0; rb < nrb; rb += 4) {
             const int i0 = first + rb * 16;
-            d4 cr[2], ci[2];
-            double2 av[2][4];
+            double2 av[4][4], cv[4][4];
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                cr[h] = d4{0, 0, 0, 0};
-                ci[h] = d4{0, 0, 0, 0};
+            for (int h = 0; h < 4; ++h) {
                 const int ia = i0 + h * 16 + l16;
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
                     const int k = s * 4 + kq;
                     av[h][s] = (s < ks && k < kw && ia < N) ? colp(L, a, k0 + k)[ia] : make_double2(0, 0);
                 }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int i = i0 + h * 16 + kq + 4 * r;
+                    cv[h][r] = (cv_ok && i < N) ? col[i] : make_double2(0, 0);
+                }
+            }
+            d4 cr[4], ci[4];
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                cr[h] = d4{0, 0, 0, 0};
+                ci[h] = d4{0, 0, 0, 0};
             }
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 if (s >= ks) break;
 #pragma unroll
-                for (int h = 0; h < 2; ++h) cr[h] = mfma4(av[h][s].x, uv[s].x, cr[h]);
+                for (int h = 0; h < 4; ++h) cr[h] = mfma4(av[h][s].x, uv[s].x, cr[h]);
 #pragma unroll
-                for (int h = 0; h < 2; ++h) ci[h] = mfma4(av[h][s].x, uv[s].y, ci[h]);
+                for (int h = 0; h < 4; ++h) ci[h] = mfma4(av[h][s].x, uv[s].y, ci[h]);
 #pragma unroll
-                for (int h = 0; h < 2; ++h) cr[h] = mfma4(-av[h][s].y, uv[s].y, cr[h]);
+                for (int h = 0; h < 4; ++h) cr[h] = mfma4(-av[h][s].y, uv[s].y, cr[h]);
 #pragma unroll
-                for (int h = 0; h < 2; ++h) ci[h] = mfma4(av[h][s].y, uv[s].x, ci[h]);
+                for (int h = 0; h < 4; ++h) ci[h] = mfma4(av[h][s].y, uv[s].x, ci[h]);
             }
-            if (c < cw) {
+            if (cv_ok) {
 #pragma unroll
-                for (int h = 0; h < 2; ++h)
+                for (int h = 0; h < 4; ++h)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const int i = i0 + h * 16 + kq + 4 * r;
                         if (i < N) {
-                            double2 v = col[i];
+                            double2 v = cv[h][r];
                             v.x -= cr[h][r];
                             v.y -= ci[h][r];
                             col[i] = v;
@@ -676,50 +764,40 @@ __global__ __launch_bounds__(64) void k_lu_dinv(LuArgs L, double2 *__restrict__ 
         for (int r = 0; r < 16; ++r) out[tid * 16 + r] = x[r];
     }
 }
-// Y <- U^{-1} Y by block back substitution: each wave owns one 16-column block of Y and
-// walks up the 16-row blocks of U: X = inv(U_kk) Y_k, then Y_i -= U_ik X for the blocks
-// above (two at a time).  The MFMA result layout (rows kq + 4 r of the lane's column) is
-// the B-operand layout of the next product, so X stays in registers; the waves of a
-// workgroup read the same U blocks, which L1 serves once.
+// Y <- U^{-1} Y by block back substitution.  A workgroup owns one 16-column block of Y
+// and walks up the 16-row blocks of U.  Step k: every wave forms X = inv(U_kk) Y_k, then
+// Y_i -= U_ik X for the blocks i < k, block i always by wave i % 4, two blocks per wave and
+// round with their loads issued together (the first round's with the loads of X's
+// operands).  The MFMA result layout (rows kq + 4 r of the lane's column) is the B-operand
+// layout of the update, so X stays in registers.  Wave 0 stores X after the barrier that
+// ends the step, when no wave reads Y_k any more.
 __global__ __launch_bounds__(256) void k_lu_back(LuArgs L, const double2 *__restrict__ dinv)
 {
+    constexpr int BH = 2;   // blocks per wave and round
     const int a = L.list[blockIdx.y];
     const int N = L.g.N, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nblk = (N + 15) / 16;
-    const int j0 = (blockIdx.x * 4 + wave) * 16;
-    if (j0 >= N) return;
+    const int j0 = blockIdx.x * 16;
     const int cw = min(16, N - j0);
     const int l16 = lane & 15, kq = lane >> 4;
     double2 *y = colp(L, a, N + j0 + min(l16, cw - 1));
     for (int kb = nblk - 1; kb >= 0; --kb) {
         const int r0 = kb * 16, rw = min(16, N - r0);
+        const int ks = (rw + 3) >> 2;
         const double2 *inv = dinv + ((size_t)blockIdx.y * nblk + kb) * 256;
-        d4 xr = {0, 0, 0, 0}, xi = {0, 0, 0, 0};
+        double2 iv[4], yb[4];
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int k = s * 4 + kq;
-            const double2 av = inv[k * 16 + l16];                       // inv(l16, k)
-            const double2 bv = (k < rw && l16 < cw) ? y[r0 + k] : make_double2(0, 0);
-            xr = mfma4(av.x, bv.x, xr);
-            xi = mfma4(av.x, bv.y, xi);
-            xr = mfma4(-av.y, bv.y, xr);
-            xi = mfma4(av.y, bv.x, xi);
+            iv[s] = inv[k * 16 + l16];   // inv(l16, k)
+            yb[s] = (k < rw && l16 < cw) ? y[r0 + k] : make_double2(0, 0);
         }
-        if (l16 < cw) {
+        double2 av[BH][4], yv[BH][4];
+        auto upd_load = [&](int round) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-                if (kq + 4 * r < rw) y[r0 + kq + 4 * r] = make_double2(xr[r], xi[r]);
-        }
-        const int ks = (rw + 3) >> 2;
-        for (int rb = 0; rb < kb; rb += 2) {
-            d4 cr[2], ci[2];
-            double2 av[2][4], yv[2][4];
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                cr[h] = d4{0, 0, 0, 0};
-                ci[h] = d4{0, 0, 0, 0};
-                const int i0 = (rb + h) * 16;
-                const bool hv = rb + h < kb;
+            for (int h = 0; h < BH; ++h) {
+                const int rb = wave + 4 * h + 4 * BH * round, i0 = rb * 16;
+                const bool hv = rb < kb;
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
                     const int k = s * 4 + kq;
@@ -728,23 +806,41 @@ __global__ __launch_bounds__(256) void k_lu_back(LuArgs L, const double2 *__rest
 #pragma unroll
                 for (int r = 0; r < 4; ++r) yv[h][r] = hv ? y[i0 + kq + 4 * r] : make_double2(0, 0);
             }
+        };
+        upd_load(0);
+        d4 xr = {0, 0, 0, 0}, xi = {0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            xr = mfma4(iv[s].x, yb[s].x, xr);
+            xi = mfma4(iv[s].x, yb[s].y, xi);
+            xr = mfma4(-iv[s].y, yb[s].y, xr);
+            xi = mfma4(iv[s].y, yb[s].x, xi);
+        }
+        for (int round = 0; round * 4 * BH < kb; ++round) {
+            if (round > 0) upd_load(round);
+            d4 cr[BH], ci[BH];
+#pragma unroll
+            for (int h = 0; h < BH; ++h) {
+                cr[h] = d4{0, 0, 0, 0};
+                ci[h] = d4{0, 0, 0, 0};
+            }
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 if (s >= ks) break;
 #pragma unroll
-                for (int h = 0; h < 2; ++h) cr[h] = mfma4(av[h][s].x, xr[s], cr[h]);
+                for (int h = 0; h < BH; ++h) cr[h] = mfma4(av[h][s].x, xr[s], cr[h]);
 #pragma unroll
-                for (int h = 0; h < 2; ++h) ci[h] = mfma4(av[h][s].x, xi[s], ci[h]);
+                for (int h = 0; h < BH; ++h) ci[h] = mfma4(av[h][s].x, xi[s], ci[h]);
 #pragma unroll
-                for (int h = 0; h < 2; ++h) cr[h] = mfma4(-av[h][s].y, xi[s], cr[h]);
+                for (int h = 0; h < BH; ++h) cr[h] = mfma4(-av[h][s].y, xi[s], cr[h]);
 #pragma unroll
-                for (int h = 0; h < 2; ++h) ci[h] = mfma4(av[h][s].y, xr[s], ci[h]);
+                for (int h = 0; h < BH; ++h) ci[h] = mfma4(av[h][s].y, xr[s], ci[h]);
             }
             if (l16 < cw) {
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    if (rb + h >= kb) break;
-                    const int i0 = (rb + h) * 16;
+                for (int h = 0; h < BH; ++h) {
+                    const int rb = wave + 4 * h + 4 * BH * round, i0 = rb * 16;
+                    if (rb >= kb) continue;
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         double2 v = yv[h][r];
@@ -754,6 +850,12 @@ __global__ __launch_bounds__(256) void k_lu_back(LuArgs L, const double2 *__rest
                     }
                 }
             }
+        }
+        __syncthreads();
+        if (wave == 0 && l16 < cw) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                if (kq + 4 * r < rw) y[r0 + kq + 4 * r] = make_double2(xr[r], xi[r]);
         }
     }
 }
@@ -1126,7 +1228,7 @@ struct Runner {
     size_t bytes = 0;
     double2 *Q0, *Q1, *P, *M, *g2c, *gc, *dinv;
     double *colsum, *diagabs, *coef;
-    int *xid, *ren, *piv, *list, *all;
+    int *xid, *ren, *piv, *list, *all, *perm;
     int *h = nullptr;           // pinned: [A] flags, [A] list
     hipEvent_t evf = nullptr;
     double2 *Q, *Qn;            // current / next Q buffer
@@ -1141,11 +1243,8 @@ struct Runner {
     // the carried rp)
     double2 *W[10] = {};
     double2 *X = nullptr;       // rk4_intgs: power-iteration vector per angle
-    LuArgs lu(const int *lst) const { return LuArgs{g, Q, P, piv, lst, nullptr, dinv}; }
+    LuArgs lu(const int *lst) const { return LuArgs{g, Q, P, piv, lst, nullptr, dinv, perm}; }
 };
-
-// dynamic LDS of the panel factorisation: the panel, 16 columns of N rows (odd pitch)
-static int lds_bytes(int N) { return (int)((size_t)(N | 1) * 16 * sizeof(double2)); }
 
 void launch_fill_i32(hipStream_t st, int *p, int n, int v)
 {
@@ -1185,7 +1284,8 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
                  odi = take((size_t)g.A * (g.NP / 16) * 256 * sizeof(double2)),
                  ocs = take((size_t)g.A * (g.NP / 16) * g.NP * 8), oda = take((size_t)g.A * g.NP * 8),
                  oco = take((size_t)g.A * 8), oxi = take((size_t)g.A * 4), ore = take((size_t)g.A * 4),
-                 opv = take((size_t)g.A * g.NP * 4), oli = take((size_t)g.A * 4), oal = take((size_t)g.A * 4);
+                 opv = take((size_t)g.A * g.NP * 4), oli = take((size_t)g.A * 4), oal = take((size_t)g.A * 4),
+                 opm = take((size_t)g.A * LU_PERM * 4);
     const int nw = t.method == TRHEPD_RK4GS ? GS_MATS : magnus_mats(t.method);
     const size_t omg = nw ? take(nw * mb) : 0;
     const size_t ox = t.method == TRHEPD_RK4GS ? take((size_t)g.A * g.NP * sizeof(double2)) : 0;
@@ -1228,6 +1328,7 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
     R->xid = (int *)(d + oxi);
     R->ren = (int *)(d + ore);
     R->piv = (int *)(d + opv);
+    R->perm = (int *)(d + opm);
     R->list = (int *)(d + oli);
     R->all = (int *)(d + oal);
     for (int w = 0; w < nw; ++w) R->W[w] = (double2 *)(d + omg + w * mb);
@@ -1253,12 +1354,6 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
     TRY_HIP(ctx, hipMemsetAsync(R->d, 0, R->bytes, b.stream));
     TRY_HIP(ctx, hipMemcpyAsync(R->all, all.data(), g.A * 4, hipMemcpyHostToDevice, b.stream));
     TRY_HIP(ctx, hipStreamSynchronize(b.stream));
-    // dynamic LDS of the LU kernels grows with N (the panel is N x 16 complex)
-    static int attr_bytes = 0;
-    if (lds_bytes(g.N) > attr_bytes) {
-        TRY_HIP(ctx, hipFuncSetAttribute((const void *)k_lu_panel, hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes(g.N)));
-        attr_bytes = lds_bytes(g.N);
-    }
     b.run = R;
     return dense_method(t.method) ? TRHEPD_OK : autotune(ctx, b, t);
 }
@@ -1278,17 +1373,17 @@ static int lu_on(trhepd_ctx *ctx, hipStream_t st, LuArgs L, int npts)
 {
     if (npts == 0) return TRHEPD_OK;
     const int N = L.g.N;
-    const size_t lds = lds_bytes(N);
-    hipLaunchKernelGGL(k_lu_panel, dim3(npts), dim3(256), lds, st, L, 0);
+    static_assert(LU_RPT == 3 && LU_RPT * 256 >= 600, "panel rows per workgroup below the device path's beam limit");
+    hipLaunchKernelGGL(k_lu_panel, dim3(npts), dim3(256), 0, st, L, 0);
     for (int k0 = 0; k0 < N; k0 += 16) {
         const int kw = std::min(16, N - k0);
         const int ntile = (2 * N - k0 - kw + LU_TILE - 1) / LU_TILE;
         hipLaunchKernelGGL(k_lu_step, dim3(ntile, npts), dim3(256), 0, st, L, k0);
-        if (k0 + 16 < N) hipLaunchKernelGGL(k_lu_panel, dim3(npts), dim3(256), lds, st, L, k0 + 16);
+        if (k0 + 16 < N) hipLaunchKernelGGL(k_lu_panel, dim3(npts), dim3(256), 0, st, L, k0 + 16);
     }
     const int nblk = (N + 15) / 16;
     hipLaunchKernelGGL(k_lu_dinv, dim3(nblk, npts), dim3(64), 0, st, L, L.dinv);
-    hipLaunchKernelGGL(k_lu_back, dim3((nblk + 3) / 4, npts), dim3(256), 0, st, L, (const double2 *)L.dinv);
+    hipLaunchKernelGGL(k_lu_back, dim3(nblk, npts), dim3(256), 0, st, L, (const double2 *)L.dinv);
     TRY_HIP(ctx, hipGetLastError());
     return TRHEPD_OK;
 }
@@ -1745,7 +1840,7 @@ int runner_end(trhepd_ctx *ctx, Batch &b, const Table &t)
         double2 *Qt = R.W[0], *Pt = R.W[1];
         hipLaunchKernelGGL(k_gs_strans, eg, eb, 0, b.stream, g, (const double2 *)b.gamma, (const double2 *)R.Q,
                            (const double2 *)R.P, Qt, Pt);
-        LuArgs L{g, Qt, Pt, R.piv, R.all, b.cnt + 1, R.dinv};
+        LuArgs L{g, Qt, Pt, R.piv, R.all, b.cnt + 1, R.dinv, R.perm};
         int rc = lu_on(ctx, b.stream, L, g.A);
         if (rc) return rc;
         const size_t nout = (size_t)g.N * g.N * g.A;
