@@ -404,13 +404,20 @@ struct LuArgs {
     const int *list;
     int *flag;         // singular pivot flag
     double2 *dinv;     // [npts][nblk][16*16] inverses of U's diagonal blocks
-    int *perm;         // [A][LU_PERM] the current panel's row interchanges as one permutation
+    int *perm;         // [2][A][LU_PERM] a panel's row interchanges as one permutation, by panel parity
 };
 
 // Layout of one angle's perm record: [0] the number n <= 32 of rows the panel's interchanges
 // touch, [8 + i] the row that slot i writes, [40 + i] the row whose old content it receives.
-// Slots 0..kw-1 are the panel's own rows k0..k0+kw-1, in order.
+// Slots 0..kw-1 are the panel's own rows k0..k0+kw-1, in order.  Panel k0 writes the record
+// set (k0 / 16) & 1, so that the next panel can be factored while the trailing update of
+// this one still reads it.
 constexpr int LU_PERM = 72;
+
+__device__ __forceinline__ int *perm_rec(const LuArgs &L, int k0, int a)
+{
+    return L.perm + ((size_t)((k0 >> 4) & 1) * L.g.A + a) * LU_PERM;
+}
 
 __device__ __forceinline__ double2 *colp(const LuArgs &L, int a, int c)
 {
@@ -523,9 +530,7 @@ __device__ __forceinline__ void panel_column(const LuArgs &L, int a, int k0, int
     const double2 pk = sh.prow[s][C];
     const bool nz = (pk.x != 0.0 || pk.y != 0.0);
     const double2 rp = nz ? cdiv(make_double2(1, 0), pk) : make_double2(0, 0);
-    double2 u[16];
-#pragma unroll
-    for (int cc = C + 1; cc < 16; ++cc) u[cc] = sh.prow[s][cc];
+    const double2 *u = sh.prow[s];
     rows3(v, [&](auto J, double2 (&x)[16]) {
         const int r = tid + 256 * J;
         if (r > C && r < rows) {
@@ -554,10 +559,9 @@ __device__ __forceinline__ void panel_columns(const LuArgs &L, int a, int k0, in
     if constexpr (C + 1 < 16) panel_columns<C + 1>(L, a, k0, kw, rows, v, oi, sh);
 }
 
-__global__ __launch_bounds__(256) void k_lu_panel(LuArgs L, int k0)
+// Factors panel k0 of angle a with all 256 threads of the workgroup (see k_lu_panel).
+__device__ __forceinline__ void factor_panel(const LuArgs &L, int a, int k0, PanelLds &sh)
 {
-    __shared__ PanelLds sh;
-    const int a = L.list[blockIdx.x];
     const int N = L.g.N, tid = threadIdx.x;
     const int kw = min(16, N - k0), rows = N - k0;
     double2 v0[16], v1[16], v2[16];
@@ -574,7 +578,7 @@ __global__ __launch_bounds__(256) void k_lu_panel(LuArgs L, int k0)
     panel_columns<0>(L, a, k0, kw, rows, v, oi, sh);
     // the interchanges as one permutation for the trailing updates: the panel's own rows
     // take slots 0..kw-1, every other row whose content changed one of the following slots
-    int *pm = L.perm + (size_t)a * LU_PERM;
+    int *pm = perm_rec(L, k0, a);
     rows3(v, [&](auto J, double2 (&x)[16]) {
         const int r = tid + 256 * J;
         if (r < rows) {
@@ -594,14 +598,21 @@ __global__ __launch_bounds__(256) void k_lu_panel(LuArgs L, int k0)
     if (tid == 0) pm[0] = sh.ntouch;
 }
 
+// the first panel of every listed angle; the later ones are factored inside k_lu_step
+__global__ __launch_bounds__(256) void k_lu_panel(LuArgs L, int k0)
+{
+    __shared__ PanelLds sh;
+    factor_panel(L, L.list[blockIdx.x], k0, sh);
+}
+
 // One elimination step for a 64-column tile right of panel [k0, k0+kw): the panel's row
 // interchanges (gathered as one permutation of the <= 32 rows they touch), U12 =
-// L11^{-1} A12, A22 -= L21 U12 on the matrix cores.  The next panel (the first 16 columns
-// of tile 0) is factored by k_lu_panel after this launch; keeping the panel's LDS out of
-// this kernel lets several tiles share a CU.
+// L11^{-1} A12, A22 -= L21 U12 on the matrix cores.  The next panel is the first 16 columns
+// of tile 0: once tile 0's update is done, its workgroup factors that panel, while the other
+// tiles of the step still update theirs.
 constexpr int LU_TILE = 64;
 
-__global__ __launch_bounds__(256) void k_lu_step(LuArgs L, int k0)
+__global__ __launch_bounds__(256, 2) void k_lu_step(LuArgs L, int k0)
 {
     __shared__ double2 swp[32 * LU_TILE];
     __shared__ double2 U[16][LU_TILE];
@@ -610,10 +621,12 @@ __global__ __launch_bounds__(256) void k_lu_step(LuArgs L, int k0)
     const int a = L.list[blockIdx.y];
     const int N = L.g.N, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int kw = min(16, N - k0), first = k0 + kw;
-    const int c0 = first + blockIdx.x * LU_TILE;
-    const int cw = min(LU_TILE, 2 * N - c0);
+    // tile 0 is the next panel's 16 columns, the other tiles LU_TILE columns each
+    const bool t0 = blockIdx.x == 0;
+    const int c0 = t0 ? first : first + 16 + (blockIdx.x - 1) * LU_TILE;
+    const int cw = min(t0 ? 16 : LU_TILE, 2 * N - c0);
     // the panel's interchanges, gathered by the panel kernel as one permutation
-    const int *pm = L.perm + (size_t)a * LU_PERM;
+    const int *pm = perm_rec(L, k0, a);
     const int nt = pm[0];
     if (tid < nt) {
         trow[tid] = pm[8 + tid];
@@ -668,34 +681,39 @@ __global__ __launch_bounds__(256) void k_lu_step(LuArgs L, int k0)
     const int ncb = (cw + 15) / 16;
     // A22 -= L21 U12: wave w owns the 16-column block w of the tile and walks down all row
     // blocks four at a time; the L21 fragments and the C values of the four blocks are loaded
-    // together, before their products, and the wave's U12 operands stay in registers
-    const int cb = wave;
+    // together, before their products, and the wave's U12 operands stay in registers.  The
+    // product is formed transposed (U12^T as the A operand), so that a lane's results are
+    // rows l16 of columns kq + 4 r: 16 lanes cover 16 consecutive rows of a column.
+    // (tile 0: all four waves on its one column block, row blocks interleaved, so that it
+    // reaches the panel factorisation early)
+    const int cb = t0 ? 0 : wave, hs = t0 ? 4 : 1;
     if (cb < ncb) {
-        const int c = cb * 16 + l16;
-        const bool cv_ok = c < cw;
         double2 uv[4];
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            const int k = s * 4 + kq;
-            uv[s] = (s < ks && k < kw) ? U[k][c] : make_double2(0, 0);
+            const int k = s * 4 + kq, c = cb * 16 + l16;
+            uv[s] = (s < ks && k < kw && c < cw) ? U[k][c] : make_double2(0, 0);
         }
-        double2 *col = colp(L, a, c0 + (cv_ok ? c : 0));
-        for (int rb = 0; rb < nrb; rb += 4) {
-            const int i0 = first + rb * 16;
+        double2 *col[4];
+        bool cok[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int c = cb * 16 + kq + 4 * r;
+            cok[r] = c < cw;
+            col[r] = colp(L, a, c0 + (cok[r] ? c : 0));
+        }
+        for (int rb = t0 ? wave : 0; rb < nrb; rb += 4 * hs) {
             double2 av[4][4], cv[4][4];
 #pragma unroll
             for (int h = 0; h < 4; ++h) {
-                const int ia = i0 + h * 16 + l16;
+                const int ia = first + (rb + h * hs) * 16 + l16;
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
                     const int k = s * 4 + kq;
                     av[h][s] = (s < ks && k < kw && ia < N) ? colp(L, a, k0 + k)[ia] : make_double2(0, 0);
                 }
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int i = i0 + h * 16 + kq + 4 * r;
-                    cv[h][r] = (cv_ok && i < N) ? col[i] : make_double2(0, 0);
-                }
+                for (int r = 0; r < 4; ++r) cv[h][r] = (cok[r] && ia < N) ? col[r][ia] : make_double2(0, 0);
             }
             d4 cr[4], ci[4];
 #pragma unroll
@@ -707,29 +725,32 @@ __global__ __launch_bounds__(256) void k_lu_step(LuArgs L, int k0)
             for (int s = 0; s < 4; ++s) {
                 if (s >= ks) break;
 #pragma unroll
-                for (int h = 0; h < 4; ++h) cr[h] = mfma4(av[h][s].x, uv[s].x, cr[h]);
+                for (int h = 0; h < 4; ++h) cr[h] = mfma4(uv[s].x, av[h][s].x, cr[h]);
 #pragma unroll
-                for (int h = 0; h < 4; ++h) ci[h] = mfma4(av[h][s].x, uv[s].y, ci[h]);
+                for (int h = 0; h < 4; ++h) ci[h] = mfma4(uv[s].x, av[h][s].y, ci[h]);
 #pragma unroll
-                for (int h = 0; h < 4; ++h) cr[h] = mfma4(-av[h][s].y, uv[s].y, cr[h]);
+                for (int h = 0; h < 4; ++h) cr[h] = mfma4(-uv[s].y, av[h][s].y, cr[h]);
 #pragma unroll
-                for (int h = 0; h < 4; ++h) ci[h] = mfma4(av[h][s].y, uv[s].x, ci[h]);
+                for (int h = 0; h < 4; ++h) ci[h] = mfma4(uv[s].y, av[h][s].x, ci[h]);
             }
-            if (cv_ok) {
 #pragma unroll
-                for (int h = 0; h < 4; ++h)
+            for (int h = 0; h < 4; ++h) {
+                const int ia = first + (rb + h * hs) * 16 + l16;
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int i = i0 + h * 16 + kq + 4 * r;
-                        if (i < N) {
-                            double2 v = cv[h][r];
-                            v.x -= cr[h][r];
-                            v.y -= ci[h][r];
-                            col[i] = v;
-                        }
+                for (int r = 0; r < 4; ++r)
+                    if (cok[r] && ia < N) {
+                        double2 v = cv[h][r];
+                        v.x -= cr[h][r];
+                        v.y -= ci[h][r];
+                        col[r][ia] = v;
                     }
             }
         }
+    }
+    if (t0 && first < N) {
+        __shared__ PanelLds sh;
+        __syncthreads();   // tile 0's update of the next panel's columns is visible
+        factor_panel(L, a, first, sh);
     }
 }
 
@@ -764,99 +785,113 @@ __global__ __launch_bounds__(64) void k_lu_dinv(LuArgs L, double2 *__restrict__ 
         for (int r = 0; r < 16; ++r) out[tid * 16 + r] = x[r];
     }
 }
-// Y <- U^{-1} Y by block back substitution.  A workgroup owns one 16-column block of Y
-// and walks up the 16-row blocks of U.  Step k: every wave forms X = inv(U_kk) Y_k, then
-// Y_i -= U_ik X for the blocks i < k, block i always by wave i % 4, two blocks per wave and
-// round with their loads issued together (the first round's with the loads of X's
-// operands).  The MFMA result layout (rows kq + 4 r of the lane's column) is the B-operand
-// layout of the update, so X stays in registers.  Wave 0 stores X after the barrier that
-// ends the step, when no wave reads Y_k any more.
-__global__ __launch_bounds__(256) void k_lu_back(LuArgs L, const double2 *__restrict__ dinv)
+// Y <- U^{-1} Y by block back substitution, one workgroup of BK_WAVES waves per 16-column
+// block of Y, that column of Y held in registers for the whole solve: wave w holds row
+// blocks w, w + BK_WAVES, ... (at most BK_SLOTS), each as MFMA accumulators (rows kq + 4 r of
+// the lane's column), so that the updates Y_i -= U_ik X accumulate into them in place.
+// Step k (k = nblk-1 .. 0): wave k % BK_WAVES forms X = inv(U_kk) Y_k from its registers,
+// stores it (the final rows of Y_k) and publishes it in LDS; after an LDS-only barrier every
+// wave subtracts U_ik X from its blocks i < k.  The U_ik operands are loaded at the start of
+// the step, before X is known, and the barrier does not wait for them; two waves per SIMD
+// cover each other's waits.  The blocks move down one register slot every BK_WAVES steps,
+// so that the block a step finishes is always its wave's slot 0.
+constexpr int BK_WAVES = 8, BK_SLOTS = 5;   // row blocks per column: N <= 640
+
+__device__ __forceinline__ void lds_barrier()
 {
-    constexpr int BH = 2;   // blocks per wave and round
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+__global__ __launch_bounds__(64 * BK_WAVES) void k_lu_back(LuArgs L, const double2 *__restrict__ dinv)
+{
+    __shared__ double xs[2][2][4][64];   // [step parity][re, im][r][lane]
     const int a = L.list[blockIdx.y];
     const int N = L.g.N, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nblk = (N + 15) / 16;
-    const int j0 = blockIdx.x * 16;
-    const int cw = min(16, N - j0);
+    const int j0 = blockIdx.x * 16, cw = min(16, N - j0);
     const int l16 = lane & 15, kq = lane >> 4;
     double2 *y = colp(L, a, N + j0 + min(l16, cw - 1));
-    for (int kb = nblk - 1; kb >= 0; --kb) {
-        const int r0 = kb * 16, rw = min(16, N - r0);
-        const int ks = (rw + 3) >> 2;
-        const double2 *inv = dinv + ((size_t)blockIdx.y * nblk + kb) * 256;
-        double2 iv[4], yb[4];
+    const int qtop = (nblk - 1) / BK_WAVES;
+    // slot i of group q (steps BK_WAVES q + BK_WAVES-1 .. BK_WAVES q) holds row block
+    // BK_WAVES (q - i) + wave
+    d4 br[BK_SLOTS], bi[BK_SLOTS];
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int k = s * 4 + kq;
-            iv[s] = inv[k * 16 + l16];   // inv(l16, k)
-            yb[s] = (k < rw && l16 < cw) ? y[r0 + k] : make_double2(0, 0);
+    for (int i = 0; i < BK_SLOTS; ++i) {
+        const int rb = BK_WAVES * (qtop - i) + wave;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = rb * 16 + kq + 4 * r;
+            const double2 v = (rb >= 0 && row < N && l16 < cw) ? y[row] : make_double2(0, 0);
+            br[i][r] = v.x;
+            bi[i][r] = v.y;
         }
-        double2 av[BH][4], yv[BH][4];
-        auto upd_load = [&](int round) {
+    }
+    for (int q = qtop; q >= 0; --q) {
 #pragma unroll
-            for (int h = 0; h < BH; ++h) {
-                const int rb = wave + 4 * h + 4 * BH * round, i0 = rb * 16;
-                const bool hv = rb < kb;
+        for (int wp = BK_WAVES - 1; wp >= 0; --wp) {
+            const int kb = BK_WAVES * q + wp;
+            if (kb >= nblk) continue;   // uniform
+            const int r0 = kb * 16, rw = min(16, N - r0), ks = (rw + 3) >> 2;
+            double2 au[BK_SLOTS][4];   // U(block of slot i, block kb)
+#pragma unroll
+            for (int i = 0; i < BK_SLOTS; ++i) {
+                const int rb = BK_WAVES * (q - i) + wave;
+                const bool hv = (i > 0 || wave < wp) && rb >= 0;
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
                     const int k = s * 4 + kq;
-                    av[h][s] = (hv && s < ks && k < rw) ? colp(L, a, r0 + k)[i0 + l16] : make_double2(0, 0);
+                    au[i][s] = (hv && s < ks && k < rw) ? colp(L, a, r0 + k)[rb * 16 + l16] : make_double2(0, 0);
+                }
+            }
+            const int par = kb & 1;
+            if (wave == wp) {
+                const double2 *inv = dinv + ((size_t)blockIdx.y * nblk + kb) * 256;
+                d4 xr = {0, 0, 0, 0}, xi = {0, 0, 0, 0};
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const double2 iv = inv[(s * 4 + kq) * 16 + l16];   // inv(l16, 4 s + kq)
+                    xr = mfma4(iv.x, br[0][s], xr);
+                    xi = mfma4(iv.x, bi[0][s], xi);
+                    xr = mfma4(-iv.y, bi[0][s], xr);
+                    xi = mfma4(iv.y, br[0][s], xi);
                 }
 #pragma unroll
-                for (int r = 0; r < 4; ++r) yv[h][r] = hv ? y[i0 + kq + 4 * r] : make_double2(0, 0);
+                for (int r = 0; r < 4; ++r) {
+                    xs[par][0][r][lane] = xr[r];
+                    xs[par][1][r][lane] = xi[r];
+                    if (l16 < cw && kq + 4 * r < rw) y[r0 + kq + 4 * r] = make_double2(xr[r], xi[r]);
+                }
             }
-        };
-        upd_load(0);
-        d4 xr = {0, 0, 0, 0}, xi = {0, 0, 0, 0};
+            lds_barrier();
+            d4 xr, xi;
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            xr = mfma4(iv[s].x, yb[s].x, xr);
-            xi = mfma4(iv[s].x, yb[s].y, xi);
-            xr = mfma4(-iv[s].y, yb[s].y, xr);
-            xi = mfma4(iv[s].y, yb[s].x, xi);
-        }
-        for (int round = 0; round * 4 * BH < kb; ++round) {
-            if (round > 0) upd_load(round);
-            d4 cr[BH], ci[BH];
-#pragma unroll
-            for (int h = 0; h < BH; ++h) {
-                cr[h] = d4{0, 0, 0, 0};
-                ci[h] = d4{0, 0, 0, 0};
+            for (int r = 0; r < 4; ++r) {
+                xr[r] = xs[par][0][r][lane];
+                xi[r] = xs[par][1][r][lane];
             }
 #pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                if (s >= ks) break;
+            for (int i = 0; i < BK_SLOTS; ++i) {
+                const int rb = BK_WAVES * (q - i) + wave;
+                if (!((i > 0 || wave < wp) && rb >= 0)) continue;   // uniform per wave
 #pragma unroll
-                for (int h = 0; h < BH; ++h) cr[h] = mfma4(av[h][s].x, xr[s], cr[h]);
-#pragma unroll
-                for (int h = 0; h < BH; ++h) ci[h] = mfma4(av[h][s].x, xi[s], ci[h]);
-#pragma unroll
-                for (int h = 0; h < BH; ++h) cr[h] = mfma4(-av[h][s].y, xi[s], cr[h]);
-#pragma unroll
-                for (int h = 0; h < BH; ++h) ci[h] = mfma4(av[h][s].y, xr[s], ci[h]);
-            }
-            if (l16 < cw) {
-#pragma unroll
-                for (int h = 0; h < BH; ++h) {
-                    const int rb = wave + 4 * h + 4 * BH * round, i0 = rb * 16;
-                    if (rb >= kb) continue;
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        double2 v = yv[h][r];
-                        v.x -= cr[h][r];
-                        v.y -= ci[h][r];
-                        y[i0 + kq + 4 * r] = v;
-                    }
+                for (int s = 0; s < 4; ++s) {
+                    if (s >= ks) break;
+                    br[i] = mfma4(-au[i][s].x, xr[s], br[i]);
+                    bi[i] = mfma4(-au[i][s].x, xi[s], bi[i]);
+                    br[i] = mfma4(au[i][s].y, xi[s], br[i]);
+                    bi[i] = mfma4(-au[i][s].y, xr[s], bi[i]);
                 }
             }
         }
-        __syncthreads();
-        if (wave == 0 && l16 < cw) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-                if (kq + 4 * r < rw) y[r0 + kq + 4 * r] = make_double2(xr[r], xi[r]);
+        for (int i = 0; i + 1 < BK_SLOTS; ++i) {
+            br[i] = br[i + 1];
+            bi[i] = bi[i + 1];
         }
+        br[BK_SLOTS - 1] = d4{0, 0, 0, 0};
+        bi[BK_SLOTS - 1] = d4{0, 0, 0, 0};
     }
 }
 
@@ -1285,7 +1320,7 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
                  ocs = take((size_t)g.A * (g.NP / 16) * g.NP * 8), oda = take((size_t)g.A * g.NP * 8),
                  oco = take((size_t)g.A * 8), oxi = take((size_t)g.A * 4), ore = take((size_t)g.A * 4),
                  opv = take((size_t)g.A * g.NP * 4), oli = take((size_t)g.A * 4), oal = take((size_t)g.A * 4),
-                 opm = take((size_t)g.A * LU_PERM * 4);
+                 opm = take((size_t)2 * g.A * LU_PERM * 4);
     const int nw = t.method == TRHEPD_RK4GS ? GS_MATS : magnus_mats(t.method);
     const size_t omg = nw ? take(nw * mb) : 0;
     const size_t ox = t.method == TRHEPD_RK4GS ? take((size_t)g.A * g.NP * sizeof(double2)) : 0;
@@ -1373,17 +1408,17 @@ static int lu_on(trhepd_ctx *ctx, hipStream_t st, LuArgs L, int npts)
 {
     if (npts == 0) return TRHEPD_OK;
     const int N = L.g.N;
-    static_assert(LU_RPT == 3 && LU_RPT * 256 >= 600, "panel rows per workgroup below the device path's beam limit");
+    static_assert(LU_RPT == 3 && LU_RPT * 256 >= 600 && BK_WAVES * BK_SLOTS * 16 >= 600, "LU panel rows per workgroup or back-substitution slots below the device path's beam limit");
     hipLaunchKernelGGL(k_lu_panel, dim3(npts), dim3(256), 0, st, L, 0);
     for (int k0 = 0; k0 < N; k0 += 16) {
         const int kw = std::min(16, N - k0);
-        const int ntile = (2 * N - k0 - kw + LU_TILE - 1) / LU_TILE;
-        hipLaunchKernelGGL(k_lu_step, dim3(ntile, npts), dim3(256), 0, st, L, k0);
-        if (k0 + 16 < N) hipLaunchKernelGGL(k_lu_panel, dim3(npts), dim3(256), 0, st, L, k0 + 16);
+        const int rest = 2 * N - (k0 + kw) - 16;   // columns right of tile 0
+        const int ntile = 1 + (rest > 0 ? (rest + LU_TILE - 1) / LU_TILE : 0);
+        hipLaunchKernelGGL(k_lu_step, dim3(ntile, npts), dim3(256), 0, st, L, k0);   // also panel k0 + 16
     }
     const int nblk = (N + 15) / 16;
     hipLaunchKernelGGL(k_lu_dinv, dim3(nblk, npts), dim3(64), 0, st, L, L.dinv);
-    hipLaunchKernelGGL(k_lu_back, dim3(nblk, npts), dim3(256), 0, st, L, (const double2 *)L.dinv);
+    hipLaunchKernelGGL(k_lu_back, dim3(nblk, npts), dim3(64 * BK_WAVES), 0, st, L, (const double2 *)L.dinv);
     TRY_HIP(ctx, hipGetLastError());
     return TRHEPD_OK;
 }
