@@ -1273,6 +1273,11 @@ struct Runner {
     MixedGeo mg;
     int8_t *qd = nullptr, *md = nullptr;
     int *eq = nullptr, *em = nullptr;
+    // per-row maxima of Q gathered by the stage kernel that wrote it (double-buffered);
+    // rmax_ok: rmax[rmax_cur] holds the current Q's, so its digit split needs one pass
+    unsigned long long *rmax[2] = {};
+    int rmax_cur = 0;
+    bool rmax_ok = false;
     int bn = 48;                // FP64 stage GEMM tile width (autotuned)
     // Magnus (int_method 1-3): per-angle work matrices (slice matrices, exponential blocks,
     // the carried rp)
@@ -1326,7 +1331,7 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
     const size_t ox = t.method == TRHEPD_RK4GS ? take((size_t)g.A * g.NP * sizeof(double2)) : 0;
     // the Magnus and rk4gs products are all per-angle: they run in FP64 in either precision
     R->mixed = ctx->precision == TRHEPD_PREC_MIXED && !dense_method(t.method);
-    size_t oqd = 0, omd = 0, oeq = 0, oem = 0;
+    size_t oqd = 0, omd = 0, oeq = 0, oem = 0, orm = 0;
     if (R->mixed) {
         // digit buffers sized for the widest padding any candidate tile needs
         const int s = rkn_coef(t.method).s;
@@ -1342,6 +1347,7 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
         omd = take(md * s);
         oeq = take((size_t)R->mg.RBP * 16 * 4);
         oem = take((size_t)cbp * 16 * 4 * s);
+        orm = take((size_t)2 * R->mg.RBP * 16 * 8);
     }
     int rc = dev_alloc(ctx, (void **)&R->d, off);
     if (rc) {
@@ -1373,6 +1379,8 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
         R->md = (int8_t *)(d + omd);
         R->eq = (int *)(d + oeq);
         R->em = (int *)(d + oem);
+        R->rmax[0] = (unsigned long long *)(d + orm);
+        R->rmax[1] = R->rmax[0] + (size_t)R->mg.RBP * 16;
     }
     hipError_t e = hipHostMalloc((void **)&R->h, (size_t)g.A * 8 + 64, hipHostMallocDefault);
     if (!e) e = hipEventCreateWithFlags(&R->evf, hipEventDisableTiming);
@@ -1615,6 +1623,7 @@ int runner_begin(trhepd_ctx *ctx, Batch &b, const Table &t)
                        R.g2c, R.gc);
     R.Q = R.Q0;
     R.Qn = R.Q1;
+    R.rmax_ok = false;
     if (t.method == TRHEPD_RK4GS) {   // invstrans, bcgs, x of power_est(-100) (surf_rk.f90:109-111)
         hipLaunchKernelGGL(k_gs_setup, eg, eb, 0, b.stream, g, (const double2 *)b.gamma, (const double2 *)b.fout, R.Q,
                            R.P, R.X);
@@ -1664,7 +1673,10 @@ static void stage_launch(trhepd_ctx *ctx, Batch &b, const Table &t, int mode, co
     Runner &R = *b.run;
     const Geo &g = R.g;
     if (R.mixed) {
-        launch_qsplit(b.stream, R.mg, R.Q, R.qd, R.eq);
+        // one pass over Q when the previous stage gathered its row maxima; either way the
+        // split clears the other buffer, which this stage's kernel fills for the next split
+        unsigned long long *nxt = R.rmax[R.rmax_cur ^ 1];
+        launch_qsplit(b.stream, R.mg, R.Q, R.qd, R.eq, R.rmax_ok ? R.rmax[R.rmax_cur] : nullptr, nxt);
         MixedStage M;
         M.g = R.mg;
         M.qd = R.qd;
@@ -1682,10 +1694,15 @@ static void stage_launch(trhepd_ctx *ctx, Batch &b, const Table &t, int mode, co
         M.mode = mode;
         M.colsum = R.colsum;
         M.diagabs = R.diagabs;
+        M.rmax = mode != 2 ? nxt : nullptr;
         prof_mark(ctx, b.stream);
         launch_stage_i8(b.stream, M);
         prof_mark(ctx, b.stream);
-        if (mode != 2) std::swap(R.Q, R.Qn);
+        if (mode != 2) {
+            std::swap(R.Q, R.Qn);
+            R.rmax_cur ^= 1;
+            R.rmax_ok = true;
+        }
         return;
     }
     const int ggrid = (int)((g.mat + 255) / 256);
@@ -1766,6 +1783,7 @@ int runner_slice_enqueue(trhepd_ctx *ctx, Batch &b, const Table &t, int l)
 {
     Runner &R = *b.run;
     const Geo &g = R.g;
+    R.rmax_ok = false;   // the slice starts by changing Q outside the stage kernels
     const RknCoef co = rkn_coef(t.method);
     const int s = co.s;
     const size_t ntot = g.mat * g.A;

@@ -203,13 +203,17 @@ struct MixedStage {
     const double *coef;
     int mode;
     double *colsum, *diagabs;
+    unsigned long long *rmax;   // mode 0/1: per-row maxima of Qn for the next split (bit patterns), or null
 };
 // digits of the off-diagonal slice matrices of `nst` consecutive table columns starting at
 // `col0`, into md[st] / em[st]
 void launch_msplit(hipStream_t st, const MixedGeo &g, const int *gmap, const VVi *col0, int nv, int nst,
                    int8_t *md, int *em);
-// digits of Q (row scales into eq)
-void launch_qsplit(hipStream_t st, const MixedGeo &g, const double2 *Q, int8_t *qd, int *eq);
+// digits of Q (row scales into eq).  vmax: Q's per-row maxima when the kernel that wrote Q
+// gathered them (one pass), else null (a pass of its own); clear: [RBP*16] zeroed for the
+// next stage's maxima, or null
+void launch_qsplit(hipStream_t st, const MixedGeo &g, const double2 *Q, int8_t *qd, int *eq,
+                   const unsigned long long *vmax = nullptr, unsigned long long *clear = nullptr);
 // P += alpha (Q M0 + Q diag(d)) from the digits, FP64 epilogue as the FP64 stage kernel
 void launch_stage_i8(hipStream_t st, const MixedStage &S);
 

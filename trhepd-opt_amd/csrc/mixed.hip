@@ -85,29 +85,38 @@ __host__ __device__ __forceinline__ size_t md_stage_bytes(const MixedGeo &g)
 
 // One workgroup splits one 16-vector block (16 rows of Q or 16 columns of M0) over all k.
 // Thread (c, g): vector c of the block, k groups g, g+16, ... of 16 consecutive k.  The
-// element accessor `get(c, k)` returns the complex entry (zero outside the matrix).
+// element accessor `get(c, k)` returns the complex entry (zero outside the matrix).  The
+// vectors' maxima (of cmax3 over k) come from `vmax` when the producer of the data gathered
+// them, else from a first pass over the block.
 template <int ND, class Get>
 __device__ __forceinline__ void split_block(const MixedGeo &g, Get get, int blk, int nblkp, bool vec_valid,
-                                            int8_t *__restrict__ out, int *__restrict__ eout)
+                                            int8_t *__restrict__ out, int *__restrict__ eout,
+                                            const unsigned long long *__restrict__ vmax)
 {
     const int t = threadIdx.x, c = t & 15, gq = t >> 4;
     const int ng = g.KC * 4;
-    __shared__ double smax[16][17];
-    double mx = 0.0;
-    if (vec_valid)
-        for (int grp = gq; grp < ng; grp += 16)
+    int e;
+    if (vmax) {
+        e = vec_valid ? scale_exp(__longlong_as_double((long long)vmax[blk * 16 + c])) : 0;
+        if (gq == 0) eout[blk * 16 + c] = e;
+    } else {
+        __shared__ double smax[16][17];
+        double mx = 0.0;
+        if (vec_valid)
+            for (int grp = gq; grp < ng; grp += 16)
 #pragma unroll 4
-            for (int kk = 0; kk < 16; ++kk) mx = fmax(mx, cmax3(get(c, grp * 16 + kk)));
-    smax[c][gq] = mx;
-    __syncthreads();
-    if (gq == 0) {
-        double m = 0.0;
-        for (int u = 0; u < 16; ++u) m = fmax(m, smax[c][u]);
-        smax[c][16] = m;
-        eout[blk * 16 + c] = scale_exp(m);
+                for (int kk = 0; kk < 16; ++kk) mx = fmax(mx, cmax3(get(c, grp * 16 + kk)));
+        smax[c][gq] = mx;
+        __syncthreads();
+        if (gq == 0) {
+            double m = 0.0;
+            for (int u = 0; u < 16; ++u) m = fmax(m, smax[c][u]);
+            smax[c][16] = m;
+            eout[blk * 16 + c] = scale_exp(m);
+        }
+        __syncthreads();
+        e = scale_exp(smax[c][16]);
     }
-    __syncthreads();
-    const int e = scale_exp(smax[c][16]);
     const size_t plane = (size_t)nblkp * g.KC * 64;   // v4i per digit plane
     v4i *o = (v4i *)out;
     for (int grp = gq; grp < ng; grp += 16) {
@@ -145,15 +154,17 @@ __device__ __forceinline__ void split_block(const MixedGeo &g, Get get, int blk,
 
 template <int ND>
 __global__ __launch_bounds__(256) void k_qsplit(MixedGeo g, const double2 *__restrict__ Q, int8_t *__restrict__ qd,
-                                                int *__restrict__ eq)
+                                                int *__restrict__ eq, const unsigned long long *__restrict__ vmax,
+                                                unsigned long long *__restrict__ clear)
 {
     const int rb = blockIdx.x;
+    if (clear && threadIdx.x < 16) clear[rb * 16 + threadIdx.x] = 0;
     const int c = threadIdx.x & 15;
     const int R = rb * 16 + c;
     const bool rv = rb < g.RB && (R % g.NP) < g.N;
     const double2 *q = Q + (size_t)(R / g.NP) * g.mat + (R % g.NP);
     auto get = [&](int, int k) -> double2 { return k < g.N ? q[(size_t)k * g.NP] : make_double2(0, 0); };
-    split_block<ND>(g, get, rb, g.RBP, rv, qd, eq);
+    split_block<ND>(g, get, rb, g.RBP, rv, qd, eq, vmax);
 }
 
 template <int ND>
@@ -169,7 +180,8 @@ __global__ __launch_bounds__(256) void k_msplit(MixedGeo g, const int *__restric
         const int s = gmap[k * g.N + j];
         return s > 0 ? w_upper(col[s - 1]) : w_lower(col[-s - 1]);
     };
-    split_block<ND>(g, get, cb, g.CBP, j < g.N, md + (size_t)st * md_stage_bytes(g), em + (size_t)st * g.CBP * 16);
+    split_block<ND>(g, get, cb, g.CBP, j < g.N, md + (size_t)st * md_stage_bytes(g), em + (size_t)st * g.CBP * 16,
+                    nullptr);
 }
 
 struct StageLaunch {
@@ -307,6 +319,7 @@ __global__ __launch_bounds__(64 * WR, 1) void k_stage_i8l(StageLaunch Lc)
     const int a = R / g.NP, i = R % g.NP;
     const double alpha = S.coef ? S.coef[a] : S.alpha;
     const int eqr = S.eq[R];
+    double qmax = 0.0;   // this lane's maximum of cmax3(Qn) (mode 0/1)
 #pragma unroll
     for (int cb = 0; cb < WC; ++cb) {
         double csum[4];
@@ -328,6 +341,7 @@ __global__ __launch_bounds__(64 * WR, 1) void k_stage_i8l(StageLaunch Lc)
                 if (S.mode != 2) {
                     const double2 qn = make_double2(q.x + S.beta * p.x, q.y + S.beta * p.y);
                     S.Qn[idx] = qn;
+                    qmax = fmax(qmax, cmax3(qn));
                     if (S.mode == 1) {
                         if (i == j) S.diagabs[(size_t)a * g.NP + j] = hypot(qn.x, qn.y);
                         else csum[r] = sqrt(qn.x * qn.x + qn.y * qn.y);
@@ -347,6 +361,12 @@ __global__ __launch_bounds__(64 * WR, 1) void k_stage_i8l(StageLaunch Lc)
                 if (l16 == 0 && j < g.NP) S.colsum[(size_t)(R >> 4) * g.NP + j] = v;
             }
         }
+    }
+    // the row maxima of Qn over this tile's columns, for the next stage's digit split
+    if (S.mode != 2 && S.rmax) {
+        qmax = fmax(qmax, __shfl_xor(qmax, 16));
+        qmax = fmax(qmax, __shfl_xor(qmax, 32));
+        if (kq == 0 && i < g.N) atomicMax(&S.rmax[R], (unsigned long long)__double_as_longlong(qmax));
     }
 }
 
@@ -394,12 +414,13 @@ void launch_msplit(hipStream_t st, const MixedGeo &g, const int *gmap, const VVi
     }
 }
 
-void launch_qsplit(hipStream_t st, const MixedGeo &g, const double2 *Q, int8_t *qd, int *eq)
+void launch_qsplit(hipStream_t st, const MixedGeo &g, const double2 *Q, int8_t *qd, int *eq,
+                   const unsigned long long *vmax, unsigned long long *clear)
 {
     switch (g.ND) {
-    case 5: hipLaunchKernelGGL(k_qsplit<5>, dim3(g.RBP), dim3(256), 0, st, g, Q, qd, eq); break;
-    case 6: hipLaunchKernelGGL(k_qsplit<6>, dim3(g.RBP), dim3(256), 0, st, g, Q, qd, eq); break;
-    default: hipLaunchKernelGGL(k_qsplit<7>, dim3(g.RBP), dim3(256), 0, st, g, Q, qd, eq); break;
+    case 5: hipLaunchKernelGGL(k_qsplit<5>, dim3(g.RBP), dim3(256), 0, st, g, Q, qd, eq, vmax, clear); break;
+    case 6: hipLaunchKernelGGL(k_qsplit<6>, dim3(g.RBP), dim3(256), 0, st, g, Q, qd, eq, vmax, clear); break;
+    default: hipLaunchKernelGGL(k_qsplit<7>, dim3(g.RBP), dim3(256), 0, st, g, Q, qd, eq, vmax, clear); break;
     }
 }
 
