@@ -434,6 +434,15 @@ __device__ __forceinline__ double2 *colp(const LuArgs &L, int a, int c)
 // that two barriers per column suffice.  Writes the factored panel and the pivots back.
 constexpr int LU_RPT = 3;   // rows per thread: panels of up to 768 rows
 
+// Workgroup barrier ordering LDS only: it does not wait for the wave's global loads and
+// stores in flight, as a full __syncthreads() does on this target.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 struct PanelLds {
     double2 prow[2][16], crow[2][16];
     double wbest[2][4];
@@ -457,8 +466,8 @@ template <class F> __device__ __forceinline__ void rows3(const PanelRows &v, F f
 
 // column C of the panel (a template, so that every column index is a constant)
 template <int C>
-__device__ __forceinline__ void panel_column(const LuArgs &L, int a, int k0, int kw, int rows, const PanelRows &v,
-                                             int (&oi)[LU_RPT], PanelLds &sh)
+__device__ __forceinline__ void panel_column(int kw, int rows, const PanelRows &v, int (&oi)[LU_RPT], int (&pv)[16],
+                                             bool &sing, PanelLds &sh)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     constexpr int s = C & 1;
@@ -486,19 +495,28 @@ __device__ __forceinline__ void panel_column(const LuArgs &L, int a, int k0, int
         sh.wbest[s][wave] = best;
         sh.wbi[s][wave] = bi;
     }
-    __syncthreads();
-    best = sh.wbest[s][0];
-    bi = sh.wbi[s][0];
-    for (int w = 1; w < 4; ++w)
-        if (sh.wbest[s][w] > best || (sh.wbest[s][w] == best && sh.wbi[s][w] < bi)) {
-            best = sh.wbest[s][w];
-            bi = sh.wbi[s][w];
+    lds_barrier();
+    {
+        // the four waves' results, read together, combined without branches
+        double wb[4];
+        int wi[4];
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            wb[w] = sh.wbest[s][w];
+            wi[w] = sh.wbi[s][w];
         }
-    if (bi >= rows) bi = C;   // no candidate compares (NaN column): no interchange
-    if (tid == 0) {
-        L.piv[(size_t)a * L.g.NP + k0 + C] = k0 + bi;
-        if (!(best > 0.0)) *L.flag = 1;
+        best = wb[0];
+        bi = wi[0];
+#pragma unroll
+        for (int w = 1; w < 4; ++w) {
+            const bool better = wb[w] > best || (wb[w] == best && wi[w] < bi);
+            best = better ? wb[w] : best;
+            bi = better ? wi[w] : bi;
+        }
     }
+    if (bi >= rows) bi = C;   // no candidate compares (NaN column): no interchange
+    pv[C] = bi;   // written to memory after the last column, so no barrier waits for a store
+    if (!(best > 0.0)) sing = true;
     // publish the pivot row; row C (thread C's first row) goes to the pivot row's owner
     rows3(v, [&](auto J, double2 (&x)[16]) {
         if (tid + 256 * J == bi) {
@@ -512,7 +530,7 @@ __device__ __forceinline__ void panel_column(const LuArgs &L, int a, int k0, int
         for (int cc = 0; cc < 16; ++cc) sh.crow[s][cc] = v.r0[cc];
         sh.cidx[s] = oi[0];
     }
-    __syncthreads();
+    lds_barrier();
     if (bi != C) {
         if (tid == C) {
 #pragma unroll
@@ -531,32 +549,37 @@ __device__ __forceinline__ void panel_column(const LuArgs &L, int a, int k0, int
     const bool nz = (pk.x != 0.0 || pk.y != 0.0);
     const double2 rp = nz ? cdiv(make_double2(1, 0), pk) : make_double2(0, 0);
     const double2 *u = sh.prow[s];
+    // Each row below C is scaled and updated; for the other rows the multiplier is zero, so
+    // that the update is the same straight-line code for every row and leaves them exactly
+    // as they are.  Columns beyond kw are zero in every row and stay zero.
+    double2 l[LU_RPT];
     rows3(v, [&](auto J, double2 (&x)[16]) {
         const int r = tid + 256 * J;
-        if (r > C && r < rows) {
-            double2 l = x[C];
-            if (nz) {
-                l = cmul(l, rp);
-                x[C] = l;
-            }
-#pragma unroll
-            for (int cc = C + 1; cc < 16; ++cc) {
-                if (cc < kw) {
-                    x[cc].x -= l.x * u[cc].x - l.y * u[cc].y;
-                    x[cc].y -= l.x * u[cc].y + l.y * u[cc].x;
-                }
-            }
-        }
+        const bool below = r > C && r < rows;
+        l[J] = below ? x[C] : make_double2(0, 0);
+        if (nz) l[J] = cmul(l[J], rp);
+        if (below) x[C] = l[J];
     });
+    const bool third = rows > 512;   // uniform: whether any thread has a third row
+#pragma unroll
+    for (int cc = C + 1; cc < 16; ++cc) {
+        const double2 uc = u[cc];
+        rows3(v, [&](auto J, double2 (&x)[16]) {
+            if (J < 2 || third) {
+                x[cc].x -= l[J].x * uc.x - l[J].y * uc.y;
+                x[cc].y -= l[J].x * uc.y + l[J].y * uc.x;
+            }
+        });
+    }
 }
 
 template <int C>
-__device__ __forceinline__ void panel_columns(const LuArgs &L, int a, int k0, int kw, int rows, const PanelRows &v,
-                                              int (&oi)[LU_RPT], PanelLds &sh)
+__device__ __forceinline__ void panel_columns(int kw, int rows, const PanelRows &v, int (&oi)[LU_RPT], int (&pv)[16],
+                                              bool &sing, PanelLds &sh)
 {
     // kw is uniform, so every thread takes the same branches to the barriers
-    if (C < kw) panel_column<C>(L, a, k0, kw, rows, v, oi, sh);
-    if constexpr (C + 1 < 16) panel_columns<C + 1>(L, a, k0, kw, rows, v, oi, sh);
+    if (C < kw) panel_column<C>(kw, rows, v, oi, pv, sing, sh);
+    if constexpr (C + 1 < 16) panel_columns<C + 1>(kw, rows, v, oi, pv, sing, sh);
 }
 
 // Factors panel k0 of angle a with all 256 threads of the workgroup (see k_lu_panel).
@@ -575,7 +598,15 @@ __device__ __forceinline__ void factor_panel(const LuArgs &L, int a, int k0, Pan
 #pragma unroll
     for (int j = 0; j < LU_RPT; ++j) oi[j] = tid + 256 * j;
     if (tid == 0) sh.ntouch = kw;
-    panel_columns<0>(L, a, k0, kw, rows, v, oi, sh);
+    int pv[16];
+    bool sing = false;
+    panel_columns<0>(kw, rows, v, oi, pv, sing, sh);
+    if (tid == 0) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c)
+            if (c < kw) L.piv[(size_t)a * L.g.NP + k0 + c] = k0 + pv[c];
+        if (sing) *L.flag = 1;
+    }
     // the interchanges as one permutation for the trailing updates: the panel's own rows
     // take slots 0..kw-1, every other row whose content changed one of the following slots
     int *pm = perm_rec(L, k0, a);
@@ -594,7 +625,7 @@ __device__ __forceinline__ void factor_panel(const LuArgs &L, int a, int k0, Pan
             }
         }
     });
-    __syncthreads();
+    lds_barrier();   // the slot counter is complete
     if (tid == 0) pm[0] = sh.ntouch;
 }
 
@@ -796,13 +827,6 @@ __global__ __launch_bounds__(64) void k_lu_dinv(LuArgs L, double2 *__restrict__ 
 // cover each other's waits.  The blocks move down one register slot every BK_WAVES steps,
 // so that the block a step finishes is always its wave's slot 0.
 constexpr int BK_WAVES = 8, BK_SLOTS = 5;   // row blocks per column: N <= 640
-
-__device__ __forceinline__ void lds_barrier()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
 
 __global__ __launch_bounds__(64 * BK_WAVES) void k_lu_back(LuArgs L, const double2 *__restrict__ dinv)
 {
