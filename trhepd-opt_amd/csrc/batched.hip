@@ -231,7 +231,7 @@ struct StageArgs {
 };
 
 template <int BN>
-__global__ __launch_bounds__(256) void k_stage_gemm(StageArgs S)
+__global__ __launch_bounds__(256, BN <= 48 ? 2 : 1) void k_stage_gemm(StageArgs S)
 {
     constexpr int NB = BN / 16;
     const Geo g = S.g;
@@ -312,6 +312,11 @@ __global__ __launch_bounds__(256) void k_stage_gemm(StageArgs S)
         for (int cb = 0; cb < NB; ++cb) bv[cb] = bn[cb];
     }
     const double2 w0 = w_upper(S.col[0]);
+    // Epilogue.  The Q, P and conj(gamma^2) loads of EB column blocks are issued together
+    // (clamped to the matrix, so unconditional), then their updates run: one memory round
+    // trip per EB blocks rather than one per element.  EB = 2 where the registers allow two
+    // waves per SIMD, else 1.
+    constexpr int EB = NB <= 2 ? 2 : 1;
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb) {
         if (!rvalid[rb]) continue;
@@ -319,38 +324,55 @@ __global__ __launch_bounds__(256) void k_stage_gemm(StageArgs S)
         const int a = Rb / g.NP, ib = Rb % g.NP;
         const double alpha = S.coef ? S.coef[a] : S.alpha;
 #pragma unroll
-        for (int cb = 0; cb < NB; ++cb) {
-            const int j = c0 + cb * 16 + l16;
-            double csum = 0.0;
-            if (j < g.N) {
-                const double2 c2 = S.g2c[(size_t)a * g.NP + j];
-                const double2 dj = make_double2(w0.x - c2.x, w0.y - c2.y);
+        for (int cp = 0; cp < NB; cp += EB) {
+            double2 qv[EB][4], pv[EB][4], c2v[EB];
+#pragma unroll
+            for (int e = 0; e < EB; ++e) {
+                if (cp + e >= NB) continue;
+                const int jj = min(c0 + (cp + e) * 16 + l16, g.NP - 1);
+                c2v[e] = S.g2c[(size_t)a * g.NP + jj];
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int i = ib + kq + 4 * r;
-                    if (i < g.N) {
-                        const size_t idx = (size_t)a * g.mat + i + (size_t)j * g.NP;
-                        const double2 q = S.Q[idx];
-                        const double2 qd = cmul(q, dj);
-                        double2 p = S.P[idx];
-                        p.x += alpha * (cr[rb][cb][r] + qd.x);
-                        p.y += alpha * (ci[rb][cb][r] + qd.y);
-                        S.P[idx] = p;
-                        if (S.mode != 2) {
-                            const double2 qn = make_double2(q.x + S.beta * p.x, q.y + S.beta * p.y);
-                            S.Qn[idx] = qn;
-                            if (S.mode == 1) {
-                                if (i == j) S.diagabs[(size_t)a * g.NP + j] = hypot(qn.x, qn.y);
-                                else csum += sqrt(qn.x * qn.x + qn.y * qn.y);
+                    const size_t idx = (size_t)a * g.mat + (ib + kq + 4 * r) + (size_t)jj * g.NP;
+                    qv[e][r] = S.Q[idx];
+                    pv[e][r] = S.P[idx];
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < EB; ++e) {
+                const int cb = cp + e;
+                if (cb >= NB) continue;
+                const int j = c0 + cb * 16 + l16;
+                double csum = 0.0;
+                if (j < g.N) {
+                    const double2 dj = make_double2(w0.x - c2v[e].x, w0.y - c2v[e].y);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int i = ib + kq + 4 * r;
+                        if (i < g.N) {
+                            const size_t idx = (size_t)a * g.mat + i + (size_t)j * g.NP;
+                            const double2 q = qv[e][r];
+                            const double2 qd = cmul(q, dj);
+                            double2 p = pv[e][r];
+                            p.x += alpha * (cr[rb][cb][r] + qd.x);
+                            p.y += alpha * (ci[rb][cb][r] + qd.y);
+                            S.P[idx] = p;
+                            if (S.mode != 2) {
+                                const double2 qn = make_double2(q.x + S.beta * p.x, q.y + S.beta * p.y);
+                                S.Qn[idx] = qn;
+                                if (S.mode == 1) {
+                                    if (i == j) S.diagabs[(size_t)a * g.NP + j] = hypot(qn.x, qn.y);
+                                    else csum += sqrt(qn.x * qn.x + qn.y * qn.y);
+                                }
                             }
                         }
                     }
                 }
-            }
-            if (S.mode == 1) {
-                csum += __shfl_xor(csum, 16);
-                csum += __shfl_xor(csum, 32);
-                if (kq == 0 && j < g.NP) S.colsum[(size_t)(Rb >> 4) * g.NP + j] = csum;
+                if (S.mode == 1) {
+                    csum += __shfl_xor(csum, 16);
+                    csum += __shfl_xor(csum, 32);
+                    if (kq == 0 && j < g.NP) S.colsum[(size_t)(Rb >> 4) * g.NP + j] = csum;
+                }
             }
         }
     }

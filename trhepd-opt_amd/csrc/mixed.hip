@@ -320,45 +320,67 @@ __global__ __launch_bounds__(64 * WR, 1) void k_stage_i8l(StageLaunch Lc)
     const double alpha = S.coef ? S.coef[a] : S.alpha;
     const int eqr = S.eq[R];
     double qmax = 0.0;   // this lane's maximum of cmax3(Qn) (mode 0/1)
+    // The Q, P, conj(gamma^2) and column-scale loads of two column blocks are issued together
+    // (clamped to the matrix, so unconditional), then their updates run: one memory round
+    // trip per pair of blocks rather than one per element.
 #pragma unroll
-    for (int cb = 0; cb < WC; ++cb) {
-        double csum[4];
+    for (int cp = 0; cp < WC; cp += 2) {
+        double2 qv[2][4], pv[2][4], c2v[2][4];
+        int emv[2][4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            csum[r] = 0.0;
-            const int j = (cb0 + cb) * 16 + 4 * kq + r;
-            if (i < g.N && j < g.N) {
-                const double sc = ldexp(1.0 / (254.0 * 254.0), eqr + S.em[j]);
-                const double2 c2 = S.g2c[(size_t)a * g.NP + j];
-                const double2 dj = make_double2(w0.x - c2.x, w0.y - c2.y);
-                const size_t idx = (size_t)a * g.mat + i + (size_t)j * g.NP;
-                const double2 q = S.Q[idx];
-                const double2 qd = cmul(q, dj);
-                double2 p = S.P[idx];
-                p.x += alpha * (re[cb][r] * sc + qd.x);
-                p.y += alpha * (im[cb][r] * sc + qd.y);
-                S.P[idx] = p;
-                if (S.mode != 2) {
-                    const double2 qn = make_double2(q.x + S.beta * p.x, q.y + S.beta * p.y);
-                    S.Qn[idx] = qn;
-                    qmax = fmax(qmax, cmax3(qn));
-                    if (S.mode == 1) {
-                        if (i == j) S.diagabs[(size_t)a * g.NP + j] = hypot(qn.x, qn.y);
-                        else csum[r] = sqrt(qn.x * qn.x + qn.y * qn.y);
+        for (int e = 0; e < 2; ++e) {
+            if (cp + e >= WC) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int j = (cb0 + cp + e) * 16 + 4 * kq + r, jj = min(j, g.NP - 1);
+                const size_t idx = (size_t)a * g.mat + i + (size_t)jj * g.NP;
+                qv[e][r] = S.Q[idx];
+                pv[e][r] = S.P[idx];
+                c2v[e][r] = S.g2c[(size_t)a * g.NP + jj];
+                emv[e][r] = S.em[j];
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int cb = cp + e;
+            if (cb >= WC) continue;
+            double csum[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                csum[r] = 0.0;
+                const int j = (cb0 + cb) * 16 + 4 * kq + r;
+                if (i < g.N && j < g.N) {
+                    const double sc = ldexp(1.0 / (254.0 * 254.0), eqr + emv[e][r]);
+                    const double2 dj = make_double2(w0.x - c2v[e][r].x, w0.y - c2v[e][r].y);
+                    const size_t idx = (size_t)a * g.mat + i + (size_t)j * g.NP;
+                    const double2 q = qv[e][r];
+                    const double2 qd = cmul(q, dj);
+                    double2 p = pv[e][r];
+                    p.x += alpha * (re[cb][r] * sc + qd.x);
+                    p.y += alpha * (im[cb][r] * sc + qd.y);
+                    S.P[idx] = p;
+                    if (S.mode != 2) {
+                        const double2 qn = make_double2(q.x + S.beta * p.x, q.y + S.beta * p.y);
+                        S.Qn[idx] = qn;
+                        qmax = fmax(qmax, cmax3(qn));
+                        if (S.mode == 1) {
+                            if (i == j) S.diagabs[(size_t)a * g.NP + j] = hypot(qn.x, qn.y);
+                            else csum[r] = sqrt(qn.x * qn.x + qn.y * qn.y);
+                        }
                     }
                 }
             }
-        }
-        if (S.mode == 1) {
+            if (S.mode == 1) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                double v = csum[r];
-                v += __shfl_xor(v, 1);
-                v += __shfl_xor(v, 2);
-                v += __shfl_xor(v, 4);
-                v += __shfl_xor(v, 8);
-                const int j = (cb0 + cb) * 16 + 4 * kq + r;
-                if (l16 == 0 && j < g.NP) S.colsum[(size_t)(R >> 4) * g.NP + j] = v;
+                for (int r = 0; r < 4; ++r) {
+                    double v = csum[r];
+                    v += __shfl_xor(v, 1);
+                    v += __shfl_xor(v, 2);
+                    v += __shfl_xor(v, 4);
+                    v += __shfl_xor(v, 8);
+                    const int j = (cb0 + cb) * 16 + 4 * kq + r;
+                    if (l16 == 0 && j < g.NP) S.colsum[(size_t)(R >> 4) * g.NP + j] = v;
+                }
             }
         }
     }
