@@ -388,8 +388,15 @@ __global__ void k_cond(Geo g, const double *__restrict__ colsum, const double *_
     const int nblk = g.NP / 16;
     double mn = 1e308, mx = -1e308;
     for (int j = tid; j < g.N; j += blockDim.x) {
+        // the column's partial sums are loaded together, then added in block order
+        constexpr int MAXB = 40;   // NP <= 640
+        double part[MAXB];
+#pragma unroll
+        for (int b = 0; b < MAXB; ++b) part[b] = b < nblk ? colsum[((size_t)a * nblk + b) * g.NP + j] : 0.0;
         double t = 0.0;
-        for (int b = 0; b < nblk; ++b) t += colsum[((size_t)a * nblk + b) * g.NP + j];
+#pragma unroll
+        for (int b = 0; b < MAXB; ++b)
+            if (b < nblk) t += part[b];
         const double dj = diagabs[(size_t)a * g.NP + j];
         mn = fmin(mn, dj - t);
         mx = fmax(mx, dj + t);
@@ -995,15 +1002,23 @@ __global__ __launch_bounds__(256) void k_bmm(BmmArgs S)
     const double sgn = S.conjA ? -1.0 : 1.0;
     const double2 *__restrict__ Bb = S.B + (size_t)a * g.mat + (size_t)(bj * 16 + l16) * g.NP;
     d4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
-    for (int k0 = 0; k0 < g.NP; k0 += 4) {
-        const int k = k0 + kq;
-        double2 av = Ab[(size_t)k * ks];
-        av.y *= sgn;
-        const double2 bv = Bb[k];
-        cr = mfma4(av.x, bv.x, cr);
-        ci = mfma4(av.x, bv.y, ci);
-        cr = mfma4(-av.y, bv.y, cr);
-        ci = mfma4(av.y, bv.x, ci);
+    // 16 k per iteration: the operands of four MFMA steps are loaded together
+    for (int k0 = 0; k0 < g.NP; k0 += 16) {
+        double2 av[4], bv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int k = k0 + 4 * u + kq;
+            av[u] = Ab[(size_t)k * ks];
+            bv[u] = Bb[k];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            av[u].y *= sgn;
+            cr = mfma4(av[u].x, bv[u].x, cr);
+            ci = mfma4(av[u].x, bv[u].y, ci);
+            cr = mfma4(-av[u].y, bv[u].y, cr);
+            ci = mfma4(av[u].y, bv[u].x, ci);
+        }
     }
     const int j = bj * 16 + l16;
 #pragma unroll
