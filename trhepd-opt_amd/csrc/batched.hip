@@ -275,21 +275,19 @@ __global__ __launch_bounds__(256, BN <= 48 ? 2 : 1) void k_stage_gemm(StageArgs 
         jb[cb] = c0 + cb * 16 + l16;
         jv[cb] = jb[cb] < g.NP;
     }
-    // operands of k-step s+1 are loaded while the matrix cores work on step s; the four
-    // real products of each complex block are issued term by term so that consecutive
-    // MFMAs never depend on each other
-    double2 av[2], bv[NB];
+    // operands of k-step s+2 are loaded while the matrix cores work on step s (three
+    // register sets rotate with the loop unrolled by three, so no set is copied while its
+    // loads are in flight; past the last step the loads repeat it, unused); the four real
+    // products of each complex block are issued term by term so that consecutive MFMAs never
+    // depend on each other
+    auto load = [&](int st, double2 (&av)[2], double2 (&bv)[NB]) {
+        const int k = min(st, ksteps - 1) * 4 + kq;
 #pragma unroll
-    for (int rb = 0; rb < 2; ++rb) av[rb] = Qp[abase[rb] + (size_t)kq * g.NP];
+        for (int rb = 0; rb < 2; ++rb) av[rb] = Qp[abase[rb] + (size_t)k * g.NP];
 #pragma unroll
-    for (int cb = 0; cb < NB; ++cb) bv[cb] = jv[cb] ? Mp[(size_t)kq * g.NP + jb[cb]] : make_double2(0, 0);
-    for (int s = 0; s < ksteps; ++s) {
-        double2 an[2], bn[NB];
-        const int kn = (s + 1 < ksteps ? s + 1 : s) * 4 + kq;
-#pragma unroll
-        for (int rb = 0; rb < 2; ++rb) an[rb] = Qp[abase[rb] + (size_t)kn * g.NP];
-#pragma unroll
-        for (int cb = 0; cb < NB; ++cb) bn[cb] = jv[cb] ? Mp[(size_t)kn * g.NP + jb[cb]] : make_double2(0, 0);
+        for (int cb = 0; cb < NB; ++cb) bv[cb] = jv[cb] ? Mp[(size_t)k * g.NP + jb[cb]] : make_double2(0, 0);
+    };
+    auto step = [&](const double2 (&av)[2], const double2 (&bv)[NB]) {
 #pragma unroll
         for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
@@ -306,10 +304,17 @@ __global__ __launch_bounds__(256, BN <= 48 ? 2 : 1) void k_stage_gemm(StageArgs 
         for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
             for (int cb = 0; cb < NB; ++cb) ci[rb][cb] = mfma4(av[rb].y, bv[cb].x, ci[rb][cb]);
-#pragma unroll
-        for (int rb = 0; rb < 2; ++rb) av[rb] = an[rb];
-#pragma unroll
-        for (int cb = 0; cb < NB; ++cb) bv[cb] = bn[cb];
+    };
+    double2 a0[2], b0[NB], a1[2], b1[NB], a2[2], b2[NB];
+    load(0, a0, b0);
+    load(1, a1, b1);
+    for (int s = 0; s < ksteps; s += 3) {
+        load(s + 2, a2, b2);
+        step(a0, b0);
+        load(s + 3, a0, b0);
+        if (s + 1 < ksteps) step(a1, b1);
+        load(s + 4, a1, b1);
+        if (s + 2 < ksteps) step(a2, b2);
     }
     const double2 w0 = w_upper(S.col[0]);
     // Epilogue.  The Q, P and conj(gamma^2) loads of EB column blocks are issued together
