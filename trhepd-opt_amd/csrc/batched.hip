@@ -275,8 +275,8 @@ __global__ __launch_bounds__(256, BN <= 48 ? 2 : 1) void k_stage_gemm(StageArgs 
         jb[cb] = c0 + cb * 16 + l16;
         jv[cb] = jb[cb] < g.NP;
     }
-    // operands of k-step s+2 are loaded while the matrix cores work on step s (three
-    // register sets rotate with the loop unrolled by three, so no set is copied while its
+    // operands of k-step s+3 are loaded while the matrix cores work on step s (four
+    // register sets rotate with the loop unrolled by four, so no set is copied while its
     // loads are in flight; past the last step the loads repeat it, unused); the four real
     // products of each complex block are issued term by term so that consecutive MFMAs never
     // depend on each other
@@ -305,16 +305,19 @@ __global__ __launch_bounds__(256, BN <= 48 ? 2 : 1) void k_stage_gemm(StageArgs 
 #pragma unroll
             for (int cb = 0; cb < NB; ++cb) ci[rb][cb] = mfma4(av[rb].y, bv[cb].x, ci[rb][cb]);
     };
-    double2 a0[2], b0[NB], a1[2], b1[NB], a2[2], b2[NB];
+    double2 a0[2], b0[NB], a1[2], b1[NB], a2[2], b2[NB], a3[2], b3[NB];
     load(0, a0, b0);
     load(1, a1, b1);
-    for (int s = 0; s < ksteps; s += 3) {
-        load(s + 2, a2, b2);
+    load(2, a2, b2);
+    for (int s = 0; s < ksteps; s += 4) {
+        load(s + 3, a3, b3);
         step(a0, b0);
-        load(s + 3, a0, b0);
+        load(s + 4, a0, b0);
         if (s + 1 < ksteps) step(a1, b1);
-        load(s + 4, a1, b1);
+        load(s + 5, a1, b1);
         if (s + 2 < ksteps) step(a2, b2);
+        load(s + 6, a2, b2);
+        if (s + 3 < ksteps) step(a3, b3);
     }
     const double2 w0 = w_upper(S.col[0]);
     // Epilogue.  The Q, P and conj(gamma^2) loads of EB column blocks are issued together
