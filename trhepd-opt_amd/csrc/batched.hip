@@ -503,7 +503,7 @@ template <class F> __device__ __forceinline__ void rows3(const PanelRows &v, F f
 
 // column C of the panel (a template, so that every column index is a constant)
 template <int C>
-__device__ __forceinline__ void panel_column(int kw, int rows, const PanelRows &v, int (&oi)[LU_RPT], int (&pv)[16],
+__device__ __forceinline__ void panel_column(int rows, const PanelRows &v, int (&oi)[LU_RPT], int (&pv)[16],
                                              bool &sing, PanelLds &sh)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -615,7 +615,7 @@ __device__ __forceinline__ void panel_columns(int kw, int rows, const PanelRows 
                                               bool &sing, PanelLds &sh)
 {
     // kw is uniform, so every thread takes the same branches to the barriers
-    if (C < kw) panel_column<C>(kw, rows, v, oi, pv, sing, sh);
+    if (C < kw) panel_column<C>(rows, v, oi, pv, sing, sh);
     if constexpr (C + 1 < 16) panel_columns<C + 1>(kw, rows, v, oi, pv, sing, sh);
 }
 
