@@ -73,3 +73,65 @@ def test_c3_periodic_schedule_vs_reference_rule(engine):
     l2 = max(np.linalg.norm(out[-1][0][i] - out[0][0][i]) / np.linalg.norm(out[0][0][i]) for i in range(len(sel)))
     print("renorms reference rule %d, periodic %d; rel %.3e L2 %.3e" % (out[0][1], out[-1][1], err, l2))
     assert l2 < 1e-10 and err < 1e-8
+
+
+@pytest.mark.parametrize("nb", [49, 64, 600])
+def test_edge_beam_counts_renormalisation_invariance(engine, nb):
+    """The batched path at its smallest N, at an N with no row padding (NP = N) and at the
+    largest N (the LU panel's third row per thread, 38 back-substitution blocks): the
+    renormalisation is an exact change of variables, so a renormalisation after every
+    slice, every fourth slice and the reference's Gershgorin rule give the same curve to the
+    FP64 bar (a mistake in the LU would not cancel between the schedules)."""
+    print("N", nb, flush=True)
+    rng = np.random.default_rng([11, nb])
+    bd = synth.bulk_deck(nb, 0.5, 0.8 + 1e-6, 0.1)
+    st = synth.Structure(bd, synth.surf_deck(rng, [14, 8], 3, 6.0), driver.bulk_header(bd))
+    blocks = synth.bulk_reflection(engine, st, layers=1)
+    job = driver.SurfaceJob(engine, st.bp, st.surf, 0.04, lib.SP4)
+    out = {}
+    try:
+        G, F, W, A = job.inputs(0, blocks)
+        assert job.nb0 >= 1 and len(A) == 4
+        for period in (1, 4, 0):
+            engine.set_renorm_period(period)
+            _, f2 = engine.propagate_batch(job.table_id, job.iv, G, F, job.nb0, W)
+            out[period] = (f2, engine.last_stats()["n_renorm"])
+    finally:
+        engine.set_renorm_period(0)
+        job.release()
+    ref = out[0][0]
+    for period in (1, 4):
+        f2, nren = out[period]
+        err = max(O.rel_err(f2[i], ref[i]) for i in range(len(A)))
+        l2 = max(np.linalg.norm(f2[i] - ref[i]) / np.linalg.norm(ref[i]) for i in range(len(A)))
+        print("period %d: %d renormalisations (rule: %d); rel %.3e L2 %.3e" % (period, nren, out[0][1], err, l2))
+        assert np.all(np.isfinite(f2)) and l2 < 1e-10 and err < 1e-8
+    assert out[1][1] > out[4][1]
+
+
+@pytest.mark.parametrize("nb", [49, 600])
+def test_mixed_edge_beam_counts_vs_fp64(engine, nb):
+    """The int8 digit path at the smallest and the largest batched N (column-block and k
+    padding of the digit fragments) against the FP64 path, at 6 digits: within the mixed
+    bar of 1e-6 per intensity."""
+    print("N", nb, flush=True)
+    rng = np.random.default_rng([12, nb])
+    bd = synth.bulk_deck(nb, 0.5, 0.8 + 1e-6, 0.1)
+    st = synth.Structure(bd, synth.surf_deck(rng, [14, 8], 3, 6.0), driver.bulk_header(bd))
+    blocks = synth.bulk_reflection(engine, st, layers=1)
+    out = {}
+    for prec in (lib.PREC_FP64, lib.PREC_MIXED):
+        with lib.Engine(0, precision=prec) as eng:
+            eng.set_renorm_period(-1)
+            if prec == lib.PREC_MIXED:
+                eng.set_mixed_digits(6)
+            job = driver.SurfaceJob(eng, st.bp, st.surf, 0.04, lib.SP4)
+            try:
+                G, F, W, A = job.inputs(0, blocks)
+                out[prec] = eng.propagate_batch(job.table_id, job.iv, G, F, job.nb0, W)[1]
+            finally:
+                job.release()
+    f64, fmx = out[lib.PREC_FP64], out[lib.PREC_MIXED]
+    err = max(O.rel_err(fmx[i], f64[i]) for i in range(len(f64)))
+    print("mixed vs fp64 %.3e" % err)
+    assert np.all(np.isfinite(fmx)) and err < 1e-6
