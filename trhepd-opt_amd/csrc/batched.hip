@@ -770,11 +770,15 @@ __global__ __launch_bounds__(256, 2) void k_lu_step(LuArgs L, int k0)
             cok[r] = c < cw;
             col[r] = colp(L, a, c0 + (cok[r] ? c : 0));
         }
-        for (int rb = t0 ? wave : 0; rb < nrb; rb += 4 * hs) {
-            double2 av[4][4], cv[4][4];
+        // iteration it covers row blocks rs + 2 hs it and rs + 2 hs it + hs; the operands of
+        // iteration it + 1 are loaded while iteration it runs (two register sets rotate with
+        // the loop unrolled by two; past the last iteration every load is predicated off)
+        const int rs = t0 ? wave : 0;
+        const int niter = nrb > rs ? (nrb - rs + 2 * hs - 1) / (2 * hs) : 0;
+        auto load = [&](int it, double2 (&av)[2][4], double2 (&cv)[2][4]) {
 #pragma unroll
-            for (int h = 0; h < 4; ++h) {
-                const int ia = first + (rb + h * hs) * 16 + l16;
+            for (int h = 0; h < 2; ++h) {
+                const int ia = first + (rs + it * 2 * hs + h * hs) * 16 + l16;
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
                     const int k = s * 4 + kq;
@@ -783,9 +787,11 @@ __global__ __launch_bounds__(256, 2) void k_lu_step(LuArgs L, int k0)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) cv[h][r] = (cok[r] && ia < N) ? col[r][ia] : make_double2(0, 0);
             }
-            d4 cr[4], ci[4];
+        };
+        auto update = [&](int it, const double2 (&av)[2][4], const double2 (&cv)[2][4]) {
+            d4 cr[2], ci[2];
 #pragma unroll
-            for (int h = 0; h < 4; ++h) {
+            for (int h = 0; h < 2; ++h) {
                 cr[h] = d4{0, 0, 0, 0};
                 ci[h] = d4{0, 0, 0, 0};
             }
@@ -793,17 +799,17 @@ __global__ __launch_bounds__(256, 2) void k_lu_step(LuArgs L, int k0)
             for (int s = 0; s < 4; ++s) {
                 if (s >= ks) break;
 #pragma unroll
-                for (int h = 0; h < 4; ++h) cr[h] = mfma4(uv[s].x, av[h][s].x, cr[h]);
+                for (int h = 0; h < 2; ++h) cr[h] = mfma4(uv[s].x, av[h][s].x, cr[h]);
 #pragma unroll
-                for (int h = 0; h < 4; ++h) ci[h] = mfma4(uv[s].x, av[h][s].y, ci[h]);
+                for (int h = 0; h < 2; ++h) ci[h] = mfma4(uv[s].x, av[h][s].y, ci[h]);
 #pragma unroll
-                for (int h = 0; h < 4; ++h) cr[h] = mfma4(-uv[s].y, av[h][s].y, cr[h]);
+                for (int h = 0; h < 2; ++h) cr[h] = mfma4(-uv[s].y, av[h][s].y, cr[h]);
 #pragma unroll
-                for (int h = 0; h < 4; ++h) ci[h] = mfma4(uv[s].y, av[h][s].x, ci[h]);
+                for (int h = 0; h < 2; ++h) ci[h] = mfma4(uv[s].y, av[h][s].x, ci[h]);
             }
 #pragma unroll
-            for (int h = 0; h < 4; ++h) {
-                const int ia = first + (rb + h * hs) * 16 + l16;
+            for (int h = 0; h < 2; ++h) {
+                const int ia = first + (rs + it * 2 * hs + h * hs) * 16 + l16;
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
                     if (cok[r] && ia < N) {
@@ -813,6 +819,14 @@ __global__ __launch_bounds__(256, 2) void k_lu_step(LuArgs L, int k0)
                         col[r][ia] = v;
                     }
             }
+        };
+        double2 a0[2][4], c0v[2][4], a1[2][4], c1v[2][4];
+        load(0, a0, c0v);
+        for (int it = 0; it < niter; it += 2) {
+            load(it + 1, a1, c1v);
+            update(it, a0, c0v);
+            load(it + 2, a0, c0v);
+            if (it + 1 < niter) update(it + 1, a1, c1v);
         }
     }
     if (t0 && first < N) {
