@@ -1173,62 +1173,191 @@ __global__ void k_gs_setup(Geo g, const double2 *__restrict__ gamma, const doubl
     }
 }
 
-// zpotrf 'U' of the listed angles' C = U^H U in place, in the order of the reference's
-// column loop; one workgroup per angle
-__global__ __launch_bounds__(256) void k_chol(Geo g, const int *__restrict__ list, double2 *__restrict__ C,
-                                              int *__restrict__ counters, int count)
+// Blocked Cholesky C = R^H R (upper R written over the upper triangle of C; below the
+// diagonal C is left as scratch), right-looking over 16-column blocks.  Step kb:
+// k_chol_panel factors the diagonal block (every workgroup redundantly, in LDS) and solves
+// the row panel R(kb, c) = R_kk^-H C(kb, c) for its 64 columns c, one column per thread;
+// k_chol_update then subtracts R(kb, i)^H R(kb, j) from the trailing blocks i <= j on the
+// matrix cores.  The factored diagonal block goes to `diag` ([list][block][col][row]), not
+// into C, since other workgroups of the same launch still read C's block; k_chol_diag puts
+// all of them into C at the end.
+__global__ __launch_bounds__(256) void k_chol_panel(Geo g, const int *__restrict__ list, double2 *__restrict__ C, int kb,
+                                                    double2 *__restrict__ diag, int *__restrict__ counters, int count)
 {
-    __shared__ double sd;
-    const int a = list[blockIdx.x];
+    __shared__ double2 Rk[16][17];
+    const int a = list[blockIdx.y], tid = threadIdx.x;
+    const int N = g.N, NP = g.NP, k0 = kb * 16, kw = min(16, N - k0);
     double2 *c = C + (size_t)a * g.mat;
-    const int N = g.N, NP = g.NP;
-    for (int j = 0; j < N; ++j) {
-        if (threadIdx.x == 0) {
-            double d = c[j + (size_t)j * NP].x;
-            for (int r = 0; r < j; ++r) {
-                const double2 u = c[r + (size_t)j * NP];
-                d -= u.x * u.x + u.y * u.y;
-            }
-            d = sqrt(d);
-            c[j + (size_t)j * NP] = make_double2(d, 0.0);
-            sd = d;
+    {
+        const int r = tid & 15, cc = tid >> 4;
+        Rk[r][cc] = (r <= cc && cc < kw) ? c[(size_t)(k0 + cc) * NP + k0 + r] : make_double2(0, 0);
+    }
+    __syncthreads();
+    // the 16 x 16 diagonal block, column by column: thread cc < 16 owns column cc
+    for (int j = 0; j < kw; ++j) {
+        if (tid == 0) {
+            double d = Rk[j][j].x;
+            for (int r = 0; r < j; ++r) d -= Rk[r][j].x * Rk[r][j].x + Rk[r][j].y * Rk[r][j].y;
+            Rk[j][j] = make_double2(sqrt(d), 0.0);
         }
         __syncthreads();
-        const double d = sd;
-        for (int k = j + 1 + threadIdx.x; k < N; k += blockDim.x) {
-            double2 acc = c[j + (size_t)k * NP];
+        if (tid > j && tid < kw) {
+            double2 acc = Rk[j][tid];
             for (int r = 0; r < j; ++r) {
-                const double2 u = c[r + (size_t)j * NP], v = c[r + (size_t)k * NP];
-                // conj(u) v
-                acc.x -= u.x * v.x + u.y * v.y;
+                const double2 u = Rk[r][j], v = Rk[r][tid];
+                acc.x -= u.x * v.x + u.y * v.y;   // conj(u) v
                 acc.y -= u.x * v.y - u.y * v.x;
             }
-            c[j + (size_t)k * NP] = make_double2(acc.x / d, acc.y / d);
+            const double d = Rk[j][j].x;
+            Rk[j][tid] = make_double2(acc.x / d, acc.y / d);
         }
         __syncthreads();
     }
-    if (count && threadIdx.x == 0) atomicAdd(&counters[0], 1);
+    if (blockIdx.x == 0) {
+        const int r = tid & 15, cc = tid >> 4;
+        diag[((size_t)blockIdx.y * (NP / 16) + kb) * 256 + cc * 16 + r] = Rk[r][cc];
+        if (kb == 0 && count && tid == 0) atomicAdd(&counters[0], 1);
+    }
+    // the row panel: columns k0 + 16 + 64 blockIdx.x + tid (tid < 64)
+    const int col = k0 + 16 + blockIdx.x * 64 + tid;
+    if (tid < 64 && col < N) {
+        double2 *cp = c + (size_t)col * NP + k0;
+        double2 x[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            if (j < kw) {
+                double2 v = cp[j];
+#pragma unroll
+                for (int r = 0; r < j; ++r) {
+                    const double2 u = Rk[r][j];   // R_kk^H (j, r) = conj(R_kk(r, j))
+                    v.x -= u.x * x[r].x + u.y * x[r].y;
+                    v.y -= u.x * x[r].y - u.y * x[r].x;
+                }
+                const double d = Rk[j][j].x;
+                x[j] = make_double2(v.x / d, v.y / d);
+                cp[j] = x[j];
+            }
+        }
+    }
 }
 
-// ztrsm 'R','U','N','N': X <- X U^-1 for the listed angles; one thread per row, X = q or p
-__global__ __launch_bounds__(256) void k_trsm_r(Geo g, const int *__restrict__ list, const double2 *__restrict__ C,
-                                                double2 *__restrict__ X)
+// the factored diagonal blocks into the upper triangle of C (grid: blocks, angles)
+__global__ __launch_bounds__(256) void k_chol_diag(Geo g, const int *__restrict__ list, double2 *__restrict__ C,
+                                                   const double2 *__restrict__ diag)
+{
+    const int a = list[blockIdx.y], kb = blockIdx.x, tid = threadIdx.x;
+    const int r = tid & 15, cc = tid >> 4, k0 = kb * 16;
+    if (r <= cc && k0 + cc < g.N)
+        C[(size_t)a * g.mat + (size_t)(k0 + cc) * g.NP + k0 + r] = diag[((size_t)blockIdx.y * gridDim.x + kb) * 256 + cc * 16 + r];
+}
+
+// C(i, j) -= sum_r conj(R(k0 + r, i)) R(k0 + r, j) for the trailing 16 x 16 blocks with
+// i-block <= j-block: one wave per block (grid: j-blocks, groups of four i-blocks, angles)
+__global__ __launch_bounds__(256) void k_chol_update(Geo g, const int *__restrict__ list, double2 *__restrict__ C, int kb)
+{
+    const int a = list[blockIdx.z];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l16 = lane & 15, kq = lane >> 4;
+    const int N = g.N, NP = g.NP, k0 = kb * 16, kw = min(16, N - k0);
+    const int b0 = kb + 1;
+    const int bj = b0 + blockIdx.x, bi = b0 + blockIdx.y * 4 + wave;
+    if (bi > bj) return;
+    double2 *c = C + (size_t)a * g.mat;
+    const int ks = (kw + 3) >> 2;
+    d4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+    double2 av[4], bv[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int k = 4 * s + kq;
+        av[s] = (s < ks && k < kw) ? c[(size_t)(bi * 16 + l16) * NP + k0 + k] : make_double2(0, 0);
+        bv[s] = (s < ks && k < kw) ? c[(size_t)(bj * 16 + l16) * NP + k0 + k] : make_double2(0, 0);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        // conj(a) b = (ar br + ai bi) + i (ar bi - ai br)
+        cr = mfma4(av[s].x, bv[s].x, cr);
+        cr = mfma4(av[s].y, bv[s].y, cr);
+        ci = mfma4(av[s].x, bv[s].y, ci);
+        ci = mfma4(-av[s].y, bv[s].x, ci);
+    }
+    const int j = bj * 16 + l16;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = bi * 16 + kq + 4 * r;
+        if (i < N && j < N) {
+            double2 *e = c + (size_t)j * NP + i;
+            double2 v = *e;
+            v.x -= cr[r];
+            v.y -= ci[r];
+            *e = v;
+        }
+    }
+}
+
+// ztrsm 'R','U','N','N' by blocks: X <- X U^-1 with one wave per 16-row block of X, walking
+// the 16-column blocks jb: T = X_jb - sum_{ib<jb} X_ib U_(ib,jb), then X_jb = T inv(U_(jb,jb))
+// (the inverses of the diagonal blocks from k_lu_dinv).  The products are formed transposed
+// (U^T X^T), so that a lane's results are row l16 of columns kq + 4 r (16 lanes cover 16
+// consecutive rows of a column) and are, register for register, the operand of the
+// multiplication by the inverse.  Each wave reads back only blocks it has itself finished.
+__global__ __launch_bounds__(256) void k_trsm_rb(Geo g, const int *__restrict__ list, const double2 *__restrict__ C,
+                                                 const double2 *__restrict__ dinv, double2 *__restrict__ X)
 {
     const int a = list[blockIdx.y];
-    const int rrow = blockIdx.x * blockDim.x + threadIdx.x;
-    if (rrow >= g.N) return;
-    const double2 *u = C + (size_t)a * g.mat;
-    double2 *x = X + (size_t)a * g.mat + rrow;
-    const int NP = g.NP;
-    for (int j = 0; j < g.N; ++j) {
-        double2 v = x[(size_t)j * NP];
-        for (int k = 0; k < j; ++k) {
-            const double2 xk = x[(size_t)k * NP], ukj = u[k + (size_t)j * NP];
-            v.x -= xk.x * ukj.x - xk.y * ukj.y;
-            v.y -= xk.x * ukj.y + xk.y * ukj.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l16 = lane & 15, kq = lane >> 4;
+    const int nblk = g.NP / 16;
+    const int rb = blockIdx.x * 4 + wave;
+    if (rb >= nblk) return;
+    const size_t NP = g.NP;
+    const double2 *__restrict__ u = C + (size_t)a * g.mat;
+    double2 *x = X + (size_t)a * g.mat + rb * 16 + l16;   // X(rb 16 + l16, c) at x[c NP]
+    const double2 *inv0 = dinv + (size_t)blockIdx.y * nblk * 256;
+    for (int jb = 0; jb < nblk; ++jb) {
+        d4 tr, ti;   // T(l16, jb 16 + kq + 4 r)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const double2 v = x[(size_t)(jb * 16 + kq + 4 * r) * NP];
+            tr[r] = v.x;
+            ti[r] = v.y;
         }
-        const double d = u[j + (size_t)j * NP].x;
-        x[(size_t)j * NP] = make_double2(v.x / d, v.y / d);
+        for (int ib0 = 0; ib0 < jb; ib0 += 4) {
+            double2 av[4][4], bv[4][4];   // U(ib 16 + 4 s + kq, jb 16 + l16), X(l16, ib 16 + 4 s + kq)
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                const int ib = ib0 + h;
+                const bool hv = ib < jb;
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const int k = ib * 16 + 4 * s + kq;
+                    av[h][s] = hv ? u[(size_t)(jb * 16 + l16) * NP + k] : make_double2(0, 0);
+                    bv[h][s] = hv ? x[(size_t)k * NP] : make_double2(0, 0);
+                }
+            }
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                if (ib0 + h >= jb) break;
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    tr = mfma4(-av[h][s].x, bv[h][s].x, tr);
+                    tr = mfma4(av[h][s].y, bv[h][s].y, tr);
+                    ti = mfma4(-av[h][s].x, bv[h][s].y, ti);
+                    ti = mfma4(-av[h][s].y, bv[h][s].x, ti);
+                }
+            }
+        }
+        const double2 *inv = inv0 + (size_t)jb * 256;
+        d4 xr = {0, 0, 0, 0}, xi = {0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const double2 iv = inv[l16 * 16 + 4 * s + kq];   // inv(4 s + kq, l16)
+            xr = mfma4(iv.x, tr[s], xr);
+            xr = mfma4(-iv.y, ti[s], xr);
+            xi = mfma4(iv.y, tr[s], xi);
+            xi = mfma4(iv.x, ti[s], xi);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) x[(size_t)(jb * 16 + kq + 4 * r) * NP] = make_double2(xr[r], xi[r]);
     }
 }
 
@@ -1649,10 +1778,23 @@ static void cqr(Batch &b, const int *list, int npts, bool count)
     double2 *C = R.W[GS_MATS - 1];
     bmm(b, R.Q, R.Q, C, 1.0, 0.0, list, npts, 1);
     bmm(b, R.P, R.P, C, 1.0, 1.0, list, npts, 1);
-    hipLaunchKernelGGL(k_chol, dim3(npts), dim3(256), 0, b.stream, g, list, C, b.cnt, count ? 1 : 0);
-    const dim3 tg((unsigned)((g.N + 255) / 256), npts);
-    hipLaunchKernelGGL(k_trsm_r, tg, dim3(256), 0, b.stream, g, list, (const double2 *)C, R.Q);
-    hipLaunchKernelGGL(k_trsm_r, tg, dim3(256), 0, b.stream, g, list, (const double2 *)C, R.P);
+    const int nblk = g.NP / 16;
+    for (int kb = 0; kb < nblk; ++kb) {
+        const int rest = g.N - 16 * (kb + 1);   // columns right of the diagonal block
+        const int ntile = rest > 0 ? (rest + 63) / 64 : 1;
+        hipLaunchKernelGGL(k_chol_panel, dim3(ntile, npts), dim3(256), 0, b.stream, g, list, C, kb, R.dinv, b.cnt,
+                           count ? 1 : 0);
+        const int m = nblk - kb - 1;
+        if (m > 0)
+            hipLaunchKernelGGL(k_chol_update, dim3(m, (m + 3) / 4, npts), dim3(256), 0, b.stream, g, list, C, kb);
+    }
+    hipLaunchKernelGGL(k_chol_diag, dim3(nblk, npts), dim3(256), 0, b.stream, g, list, C, (const double2 *)R.dinv);
+    // [q; p] <- [q; p] R^-1: inverses of R's diagonal blocks, then the blocked solves
+    LuArgs L{g, C, nullptr, nullptr, list, nullptr, R.dinv, nullptr};
+    hipLaunchKernelGGL(k_lu_dinv, dim3(nblk, npts), dim3(64), 0, b.stream, L, R.dinv);
+    const dim3 tg((unsigned)((nblk + 3) / 4), npts);
+    hipLaunchKernelGGL(k_trsm_rb, tg, dim3(256), 0, b.stream, g, list, (const double2 *)C, (const double2 *)R.dinv, R.Q);
+    hipLaunchKernelGGL(k_trsm_rb, tg, dim3(256), 0, b.stream, g, list, (const double2 *)C, (const double2 *)R.dinv, R.P);
 }
 
 static void build_mf(Batch &b, const VVi *col, double2 *M)
