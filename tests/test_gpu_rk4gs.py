@@ -58,6 +58,25 @@ def test_rk4gs_periodic_matches_rule(engine):
     assert e < 1e-10
 
 
+def test_rk4gs_n521_vs_oracle(engine):
+    """N = 521 at dz 0.1 (99 slices, 52 renormalisations in the oracle): every Cholesky QR
+    runs the blocked factorisation over 33 column blocks with several row-panel tiles per
+    step.  RK4 is stable at this dz (a one-ulp change of the potential table moves the
+    oracle's curve by 1e-12 per beam), so the N = 521 FP64 bar applies: 1e-8 per beam,
+    1e-10 normalised L2, and the same renormalisation count."""
+    print("case test521 rk4gs 0.1", flush=True)
+    surf = O.load_case(os.path.join(REF, "test521"), 0.1, 8)
+    f2, st = _device(engine, surf)
+    stats = []
+    rows = O.surface_curve(surf, stats=stats)
+    ren = int(sum(s[0] for s in stats))
+    e = O.rel_err(f2[0], rows[0][1])
+    l2 = np.linalg.norm(f2[0] - rows[0][1]) / np.linalg.norm(rows[0][1])
+    print("rel %.3e L2 %.3e, renorms device %d oracle %d" % (e, l2, st["n_renorm"], ren))
+    assert e < 1e-8 and l2 < 1e-10
+    assert st["n_renorm"] == ren
+
+
 @pytest.mark.parametrize("method", [8, 2])
 def test_dense_methods_n97_vs_oracle(engine, method):
     """A seeded synthetic structure with N = 97 at dz 0.02 (4 of its angles)."""

@@ -1363,20 +1363,163 @@ __global__ __launch_bounds__(256) void k_trsm_rb(Geo g, const int *__restrict__ 
 
 // power_est / power_step (matcomp.f90:530-594) for every angle, x carried between slices:
 // y = [q x; p x], z = q^H y1 + p^H y2, d = |x^H z|, x = z / |z|, until |z - x| < 1e-2 or
-// d grows by less than 1e-2 d, at most N steps.  Sets ren[a] = d > 1e2.
+// d grows by less than 1e-2 d, at most N steps.  The first steps of a slice run spread over
+// 16-row blocks of [q; p] (k_pw_rows: y on the block's rows and the block's share of z,
+// many workgroups per angle; k_pw_fin: z summed over the blocks, the normalisation and the
+// stopping test, one workgroup per angle); k_power goes on from where they stopped, one
+// workgroup per angle, when the estimate has not converged in those steps.
+struct PwState {
+    double d0, d;   // the previous and the latest estimate
+    int it, done;   // steps taken; 1 once the stopping test has passed
+};
+constexpr int PW_FAST = 3;   // block-split steps per slice
+constexpr int PW_MAXN = 640;
+
+// one step's first half for row block blockIdx.x of angle blockIdx.y: y1 = q x, y2 = p x on
+// the block's 16 rows, then part[a][w][j] = sum over those rows of conj(q(i,j)) y1(i) +
+// conj(p(i,j)) y2(i).  Thread (r, cg): row r of the block, columns cg, cg + 16, ...
+__global__ __launch_bounds__(256) void k_pw_rows(Geo g, const double2 *__restrict__ Q, const double2 *__restrict__ P,
+                                                 const double2 *__restrict__ X, double2 *__restrict__ part,
+                                                 const PwState *__restrict__ pst, int k)
+{
+    const int a = blockIdx.y, w = blockIdx.x, tid = threadIdx.x;
+    if (k > 0 && pst[a].done) return;
+    __shared__ double2 sx[PW_MAXN];
+    __shared__ double2 red[2][16][17];
+    __shared__ double2 yb[2][16];
+    const int N = g.N, NP = g.NP;
+    const int r = tid & 15, cg = tid >> 4, i = w * 16 + r;
+    const double2 *q = Q + (size_t)a * g.mat, *p = P + (size_t)a * g.mat;
+    for (int j = tid; j < N; j += 256) sx[j] = X[(size_t)a * NP + j];
+    __syncthreads();
+    double2 s1 = make_double2(0, 0), s2 = make_double2(0, 0);
+    if (i < N)
+        for (int j = cg; j < N; j += 16) {
+            const double2 xj = sx[j], qv = q[i + (size_t)j * NP], pv = p[i + (size_t)j * NP];
+            s1.x += qv.x * xj.x - qv.y * xj.y;
+            s1.y += qv.x * xj.y + qv.y * xj.x;
+            s2.x += pv.x * xj.x - pv.y * xj.y;
+            s2.y += pv.x * xj.y + pv.y * xj.x;
+        }
+    red[0][cg][r] = s1;
+    red[1][cg][r] = s2;
+    __syncthreads();
+    if (tid < 32) {
+        const int m = tid >> 4, rr = tid & 15;
+        double2 s = make_double2(0, 0);
+        for (int c = 0; c < 16; ++c) {
+            s.x += red[m][c][rr].x;
+            s.y += red[m][c][rr].y;
+        }
+        yb[m][rr] = s;
+    }
+    __syncthreads();
+    const double2 y1 = i < N ? yb[0][r] : make_double2(0, 0), y2 = i < N ? yb[1][r] : make_double2(0, 0);
+    double2 *out = part + ((size_t)a * gridDim.x + w) * NP;
+    const int nt = (N + 15) / 16;   // the same trip count in every lane: the sums below shuffle
+    for (int t = 0; t < nt; ++t) {
+        const int j = cg + 16 * t;
+        double sx_ = 0.0, sy_ = 0.0;
+        if (i < N && j < N) {
+            const double2 qv = q[i + (size_t)j * NP], pv = p[i + (size_t)j * NP];
+            sx_ = qv.x * y1.x + qv.y * y1.y + pv.x * y2.x + pv.y * y2.y;
+            sy_ = qv.x * y1.y - qv.y * y1.x + pv.x * y2.y - pv.y * y2.x;
+        }
+        // sum over the 16 rows: lanes r of one column group (xor offsets below 16)
+        for (int o = 8; o > 0; o >>= 1) {
+            sx_ += __shfl_xor(sx_, o);
+            sy_ += __shfl_xor(sy_, o);
+        }
+        if (r == 0 && j < N) out[j] = make_double2(sx_, sy_);
+    }
+}
+
+// one step's second half, one workgroup per angle: z = the sum of the blocks' parts (in block
+// order), d = |x^H z|, x = z / |z|, the stopping test; the state records where it stands
+__global__ __launch_bounds__(256) void k_pw_fin(Geo g, const double2 *__restrict__ part, int nw,
+                                                double2 *__restrict__ X, PwState *__restrict__ pst, int k)
+{
+    const int a = blockIdx.x, tid = threadIdx.x, N = g.N, NP = g.NP;
+    if (k > 0 && pst[a].done) return;
+    __shared__ double red[3][256];
+    __shared__ double2 z[PW_MAXN];
+    double2 *x = X + (size_t)a * NP;
+    double dn = 0.0, crx = 0.0, cry = 0.0;
+    for (int j = tid; j < N; j += 256) {
+        double2 s = make_double2(0, 0);
+        for (int w = 0; w < nw; ++w) {
+            const double2 v = part[((size_t)a * nw + w) * NP + j];
+            s.x += v.x;
+            s.y += v.y;
+        }
+        z[j] = s;
+        dn += s.x * s.x + s.y * s.y;
+        const double2 xj = x[j];   // conj(x) z
+        crx += xj.x * s.x + xj.y * s.y;
+        cry += xj.x * s.y - xj.y * s.x;
+    }
+    red[0][tid] = dn;
+    red[1][tid] = crx;
+    red[2][tid] = cry;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o)
+            for (int c = 0; c < 3; ++c) red[c][tid] += red[c][tid + o];
+        __syncthreads();
+    }
+    const double d = hypot(red[1][0], red[2][0]);
+    const double inv = 1.0 / sqrt(red[0][0]);
+    __syncthreads();
+    double dp = 0.0;
+    for (int j = tid; j < N; j += 256) {
+        const double2 zz = make_double2(z[j].x * inv, z[j].y * inv), xj = x[j];
+        const double cx = zz.x - xj.x, cy = zz.y - xj.y;
+        dp += cx * cx + cy * cy;
+        x[j] = zz;
+    }
+    red[0][tid] = dp;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) red[0][tid] += red[0][tid + o];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        dp = sqrt(red[0][0]);
+        const double d0 = k == 0 ? -1.0 : pst[a].d0;
+        const int stop = dp < 1e-2 || d - d0 < 1e-2 * d;
+        PwState st;
+        st.d0 = stop ? d0 : d;
+        st.d = d;
+        st.it = k + 1;
+        st.done = stop;
+        pst[a] = st;
+    }
+}
+
+// the remaining steps (from pst's state after `fast` block-split steps; none when fast is 0)
+// and the decision ren[a] = d > 1e2
 __global__ __launch_bounds__(256) void k_power(Geo g, const double2 *__restrict__ Q, const double2 *__restrict__ P,
                                                double2 *__restrict__ X, int *__restrict__ ren,
-                                               int *__restrict__ xid)
+                                               int *__restrict__ xid, const PwState *__restrict__ pst, int fast)
 {
     extern __shared__ double2 sh[];   // x[NP], y[2 NP], z[NP]
     __shared__ double red[3][256];
     const int a = blockIdx.x, tid = threadIdx.x, nt = blockDim.x, N = g.N, NP = g.NP;
     double2 *x = sh, *y = sh + NP, *z = sh + 3 * NP;
     const double2 *q = Q + (size_t)a * g.mat, *p = P + (size_t)a * g.mat;
-    for (int i = tid; i < N; i += nt) x[i] = X[(size_t)a * NP + i];
-    __syncthreads();
     double d = -1.0, d0 = -1.0;
-    for (int it = 0; it < N; ++it) {
+    int it0 = 0;
+    if (fast > 0) {
+        const PwState st = pst[a];
+        d = st.d;
+        d0 = st.d0;
+        it0 = st.done ? N : st.it;
+    }
+    if (it0 < N) {
+        for (int i = tid; i < N; i += nt) x[i] = X[(size_t)a * NP + i];
+        __syncthreads();
+    }
+    for (int it = it0; it < N; ++it) {
         for (int i = tid; i < N; i += nt) {
             double2 s1 = make_double2(0, 0), s2 = make_double2(0, 0);
             for (int j = 0; j < N; ++j) {
@@ -1439,7 +1582,8 @@ __global__ __launch_bounds__(256) void k_power(Geo g, const double2 *__restrict_
         if (dp < 1e-2 || d - d0 < 1e-2 * d) break;
         d0 = d;
     }
-    for (int i = tid; i < N; i += nt) X[(size_t)a * NP + i] = x[i];
+    if (it0 < N)
+        for (int i = tid; i < N; i += nt) X[(size_t)a * NP + i] = x[i];
     if (tid == 0) {
         ren[a] = d > 1e2;
         xid[a] = d > 1e2;
@@ -1495,6 +1639,8 @@ struct Runner {
     // the carried rp)
     double2 *W[10] = {};
     double2 *X = nullptr;       // rk4_intgs: power-iteration vector per angle
+    double2 *xpart = nullptr;   // rk4_intgs: the row blocks' shares of z, [A][NP/16][NP]
+    PwState *pst = nullptr;     // rk4_intgs: power-iteration state per angle
     LuArgs lu(const int *lst) const { return LuArgs{g, Q, P, piv, lst, nullptr, dinv, perm}; }
 };
 
@@ -1540,7 +1686,10 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
                  opm = take((size_t)2 * g.A * LU_PERM * 4);
     const int nw = t.method == TRHEPD_RK4GS ? GS_MATS : magnus_mats(t.method);
     const size_t omg = nw ? take(nw * mb) : 0;
-    const size_t ox = t.method == TRHEPD_RK4GS ? take((size_t)g.A * g.NP * sizeof(double2)) : 0;
+    const bool gs = t.method == TRHEPD_RK4GS;
+    const size_t ox = gs ? take((size_t)g.A * g.NP * sizeof(double2)) : 0,
+                 oxp = gs ? take((size_t)g.A * (g.NP / 16) * g.NP * sizeof(double2)) : 0,
+                 ops = gs ? take((size_t)g.A * sizeof(PwState)) : 0;
     // the Magnus and rk4gs products are all per-angle: they run in FP64 in either precision
     R->mixed = ctx->precision == TRHEPD_PREC_MIXED && !dense_method(t.method);
     size_t oqd = 0, omd = 0, oeq = 0, oem = 0, orm = 0;
@@ -1585,7 +1734,11 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
     R->list = (int *)(d + oli);
     R->all = (int *)(d + oal);
     for (int w = 0; w < nw; ++w) R->W[w] = (double2 *)(d + omg + w * mb);
-    if (t.method == TRHEPD_RK4GS) R->X = (double2 *)(d + ox);
+    if (gs) {
+        R->X = (double2 *)(d + ox);
+        R->xpart = (double2 *)(d + oxp);
+        R->pst = (PwState *)(d + ops);
+    }
     if (R->mixed) {
         R->qd = (int8_t *)(d + oqd);
         R->md = (int8_t *)(d + omd);
@@ -2026,8 +2179,16 @@ int runner_slice_enqueue(trhepd_ctx *ctx, Batch &b, const Table &t, int l)
         const int period = batch_period(ctx, b, t);
         R.forced = period > 0 ? (l % period == 0 ? 1 : 0) : -1;
         if (R.forced < 0) {
+            const int nw = g.NP / 16, fast = std::min(PW_FAST, g.N);
+            for (int k = 0; k < fast; ++k) {
+                hipLaunchKernelGGL(k_pw_rows, dim3(nw, g.A), dim3(256), 0, b.stream, g, (const double2 *)R.Q,
+                                   (const double2 *)R.P, (const double2 *)R.X, R.xpart, (const PwState *)R.pst, k);
+                hipLaunchKernelGGL(k_pw_fin, dim3(g.A), dim3(256), 0, b.stream, g, (const double2 *)R.xpart, nw, R.X,
+                                   R.pst, k);
+            }
             hipLaunchKernelGGL(k_power, dim3(g.A), dim3(256), (size_t)4 * g.NP * sizeof(double2), b.stream, g,
-                               (const double2 *)R.Q, (const double2 *)R.P, R.X, R.ren, R.xid);
+                               (const double2 *)R.Q, (const double2 *)R.P, R.X, R.ren, R.xid,
+                               (const PwState *)R.pst, fast);
             TRY_HIP(ctx, hipMemcpyAsync(R.h, R.ren, g.A * 4, hipMemcpyDeviceToHost, b.stream));
             TRY_HIP(ctx, hipEventRecord(R.evf, b.stream));
         }
