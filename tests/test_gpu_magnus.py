@@ -75,3 +75,27 @@ def test_magnus_in_mixed_context_is_fp64(engine, mixed_engine):
     a, _ = _device(engine, surf, 2)
     b, _ = _device(mixed_engine, surf, 2)
     assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("method", [2, 8])
+def test_dense_methods_many_angles_n521_same_as_single(engine, method):
+    """Eight angles at N = 521 in one batch: every step of these methods is per angle (the
+    products, the renormalisation decision and factorisation), so each angle of the batch
+    gives, bit for bit, the intensities it gives alone (checked to 1e-12)."""
+    surf = O.load_case(os.path.join(REF, "test521"), 0.68, method)
+    a, G, F, W = _inputs(surf)
+    scale = np.linspace(0.999, 1.001, 8)
+    Gb = np.array([G[0] * s for s in scale])
+    Fb = np.array([F[0]] * 8)
+    Wb = np.array([W[0]] * 8)
+    tid = engine.table_build(surf.pot, surf.dz, surf.ns, method)
+    try:
+        _, f2b = engine.propagate_batch(tid, surf.iv, Gb, Fb, surf.nb0, Wb)
+        singles = [engine.propagate_batch(tid, surf.iv, Gb[i:i + 1], Fb[i:i + 1], surf.nb0, Wb[i:i + 1])[1][0]
+                   for i in (0, 3, 7)]
+    finally:
+        engine.table_release(tid)
+    for k, i in enumerate((0, 3, 7)):
+        print("method %d angle %d: rel %.3e, bit-identical %s" % (method, i, O.rel_err(f2b[i], singles[k]),
+                                                                   np.array_equal(f2b[i], singles[k])))
+        assert O.rel_err(f2b[i], singles[k]) < 1e-12
