@@ -217,7 +217,7 @@ int trhepd_ctx_create(const trhepd_opts *opts, trhepd_ctx **out)
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
     if (e != hipSuccess) {
-        if (c->stream) hipStreamDestroy(c->stream);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return TRHEPD_E_HIP;
     }
@@ -228,8 +228,8 @@ int trhepd_ctx_create(const trhepd_opts *opts, trhepd_ctx **out)
 int trhepd_ctx_destroy(trhepd_ctx *ctx)
 {
     if (!ctx) return TRHEPD_E_ARG;
-    hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
     server_destroy(ctx);
     for (GraphRec &gr : ctx->graphs) {
         (void)hipGraphExecDestroy(gr.exec);
@@ -246,9 +246,9 @@ int trhepd_ctx_destroy(trhepd_ctx *ctx)
     }
     for (hipEvent_t e : ctx->prof.ev) (void)hipEventDestroy(e);
     for (DevBuf &b : ctx->bufs) dev_free(ctx, b.p, b.bytes);
-    hipEventDestroy(ctx->ev0);
-    hipEventDestroy(ctx->ev1);
-    hipStreamDestroy(ctx->stream);
+    (void)hipEventDestroy(ctx->ev0);
+    (void)hipEventDestroy(ctx->ev1);
+    (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return TRHEPD_OK;
 }

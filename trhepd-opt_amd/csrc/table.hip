@@ -244,7 +244,7 @@ int launch_table(trhepd_ctx *ctx, const trhepd_potential *pot, Table &t)
     const size_t tbytes = (size_t)t.ncols * pot->nv * sizeof(VVi);
     rc = dev_alloc(ctx, (void **)&t.d, tbytes);
     if (rc) {
-        hipStreamSynchronize(ctx->stream);
+        (void)hipStreamSynchronize(ctx->stream);
         dev_free(ctx, dd, off);
         return rc;
     }
