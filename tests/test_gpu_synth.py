@@ -109,6 +109,40 @@ def test_edge_beam_counts_renormalisation_invariance(engine, nb):
     assert out[1][1] > out[4][1]
 
 
+@pytest.mark.parametrize("nb", [64, 600])
+@pytest.mark.parametrize("method", [lib.M2, lib.RK4GS])
+def test_dense_methods_edge_beam_counts_renormalisation_invariance(engine, nb, method):
+    """m2 (Gershgorin rule) and rk4gs (power-estimate rule, Cholesky-QR renormalisation) at
+    an N with no padding and at the largest N (38 column blocks in the blocked Cholesky and
+    triangular solves, 38 row blocks in the split power estimate): renormalising after every
+    slice, every fourth slice and by the method's own rule gives the same curve to the FP64
+    bar."""
+    print("N", nb, "method", method, flush=True)
+    rng = np.random.default_rng([13, nb])
+    bd = synth.bulk_deck(nb, 0.5, 0.8 + 1e-6, 0.1)
+    st = synth.Structure(bd, synth.surf_deck(rng, [14, 8], 3, 6.0), driver.bulk_header(bd))
+    blocks = synth.bulk_reflection(engine, st, layers=1)
+    job = driver.SurfaceJob(engine, st.bp, st.surf, 0.04, method)
+    out = {}
+    try:
+        G, F, W, A = job.inputs(0, blocks)
+        for period in (1, 4, 0):
+            engine.set_renorm_period(period)
+            _, f2 = engine.propagate_batch(job.table_id, job.iv, G, F, job.nb0, W)
+            out[period] = (f2, engine.last_stats()["n_renorm"])
+    finally:
+        engine.set_renorm_period(0)
+        job.release()
+    ref = out[0][0]
+    for period in (1, 4):
+        f2, nren = out[period]
+        err = max(O.rel_err(f2[i], ref[i]) for i in range(len(A)))
+        l2 = max(np.linalg.norm(f2[i] - ref[i]) / np.linalg.norm(ref[i]) for i in range(len(A)))
+        print("period %d: %d renormalisations (rule: %d); rel %.3e L2 %.3e" % (period, nren, out[0][1], err, l2))
+        assert np.all(np.isfinite(f2)) and l2 < 1e-10 and err < 1e-8
+    assert out[1][1] > out[4][1]
+
+
 @pytest.mark.parametrize("nb", [49, 600])
 def test_mixed_edge_beam_counts_vs_fp64(engine, nb):
     """The int8 digit path at the smallest and the largest batched N (column-block and k
