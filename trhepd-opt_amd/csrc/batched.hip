@@ -1086,6 +1086,45 @@ __global__ void k_lin(LinArgs L)
     L.out[e] = v;
 }
 
+// Up to four k_lin outputs in one pass over the elements, formed in order: a term naming an
+// earlier output of the set takes that output's new value (what a later k_lin would load),
+// any other term the memory as it was before the set; the outputs are stored after all are
+// formed, so an output may also be an input.  The arithmetic of every output is k_lin's.
+struct LinSet {
+    int nout;
+    LinArgs o[4];
+};
+
+__global__ void k_lin_set(Geo g, LinSet S)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.mat * g.A) return;
+    const int r = (int)(e % g.mat), i = r % g.NP, j = r / g.NP;
+    const bool in = i < g.N && j < g.N;
+    double2 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        v[u] = make_double2(0, 0);
+        if (u >= S.nout || !in) continue;
+        const LinArgs &L = S.o[u];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            if (t >= L.n) break;
+            double2 x;
+            if (u > 0 && L.x[t] == S.o[0].out) x = v[0];
+            else if (u > 1 && L.x[t] == S.o[1].out) x = v[1];
+            else if (u > 2 && L.x[t] == S.o[2].out) x = v[2];
+            else x = L.x[t][e];
+            v[u].x += L.c[t] * x.x;
+            v[u].y += L.c[t] * x.y;
+        }
+        if (i == j) v[u].x += L.ident;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+        if (u < S.nout) S.o[u].out[e] = v[u];
+}
+
 // Gershgorin terms of Q (cond_est, matcomp.f90:498-528) in k_cond's layout: per angle, per
 // 16-row block b and column j, the sum of |Q(i,j)| over the block's rows i != j; |Q(j,j)|
 __global__ void k_gersh(Geo g, const double2 *__restrict__ Q, double *__restrict__ colsum,
@@ -1832,6 +1871,25 @@ static void lin(Batch &b, double2 *out, std::initializer_list<std::pair<double, 
     hipLaunchKernelGGL(k_lin, dim3((unsigned)((g.mat * g.A + 255) / 256)), dim3(256), 0, b.stream, L);
 }
 
+// several lin() calls in order, in one pass (k_lin_set)
+using LinTerms = std::initializer_list<std::pair<double, const double2 *>>;
+static void lins(Batch &b, std::initializer_list<std::pair<double2 *, LinTerms>> outs)
+{
+    const Geo &g = b.run->g;
+    LinSet S{};
+    for (const auto &o : outs) {
+        LinArgs &L = S.o[S.nout++];
+        L.g = g;
+        for (const auto &t : o.second) {
+            L.c[L.n] = t.first;
+            L.x[L.n] = t.second;
+            ++L.n;
+        }
+        L.out = o.first;
+    }
+    hipLaunchKernelGGL(k_lin_set, dim3((unsigned)((g.mat * g.A + 255) / 256)), dim3(256), 0, b.stream, g, S);
+}
+
 static void build_m(Batch &b, const VVi *col, double2 *M)
 {
     const Runner &R = *b.run;
@@ -1970,25 +2028,16 @@ static void gs_slice(Batch &b, const Table &t, int l)
     build_mf(b, c0 + t.nv, m1);
     build_mf(b, c0 + 2 * t.nv, m2);
     bmm(b, m0, Q, sm, 1.0, 0.0);   // s1 = m0 q, r1 = p
-    lin(b, SQ, {{1.0, P}});
-    lin(b, SP, {{1.0, sm}});
-    lin(b, q2, {{1.0, Q}, {h, P}});
-    lin(b, ps, {{1.0, P}, {h, sm}});   // r2
+    lins(b, {{SQ, {{1.0, P}}}, {SP, {{1.0, sm}}}, {q2, {{1.0, Q}, {h, P}}}, {ps, {{1.0, P}, {h, sm}}}});   // r2
     bmm(b, m1, q2, sm, 1.0, 0.0);      // s2
-    lin(b, SQ, {{1.0, SQ}, {2.0, ps}});
-    lin(b, SP, {{1.0, SP}, {2.0, sm}});
-    lin(b, q2, {{1.0, Q}, {h, ps}});
-    lin(b, ps, {{1.0, P}, {h, sm}});   // r3
+    lins(b, {{SQ, {{1.0, SQ}, {2.0, ps}}}, {SP, {{1.0, SP}, {2.0, sm}}}, {q2, {{1.0, Q}, {h, ps}}},
+             {ps, {{1.0, P}, {h, sm}}}});   // r3
     bmm(b, m1, q2, sm, 1.0, 0.0);      // s3
-    lin(b, SQ, {{1.0, SQ}, {2.0, ps}});
-    lin(b, SP, {{1.0, SP}, {2.0, sm}});
-    lin(b, q2, {{1.0, Q}, {dz, ps}});
-    lin(b, ps, {{1.0, P}, {dz, sm}});  // r4
+    lins(b, {{SQ, {{1.0, SQ}, {2.0, ps}}}, {SP, {{1.0, SP}, {2.0, sm}}}, {q2, {{1.0, Q}, {dz, ps}}},
+             {ps, {{1.0, P}, {dz, sm}}}});   // r4
     bmm(b, m2, q2, sm, 1.0, 0.0);      // s4
-    lin(b, SQ, {{1.0, SQ}, {1.0, ps}});
-    lin(b, SP, {{1.0, SP}, {1.0, sm}});
-    lin(b, Q, {{1.0, Q}, {dz / 6, SQ}});
-    lin(b, P, {{1.0, P}, {dz / 6, SP}});
+    lins(b, {{SQ, {{1.0, SQ}, {1.0, ps}}}, {SP, {{1.0, SP}, {1.0, sm}}}, {Q, {{1.0, Q}, {dz / 6, SQ}}},
+             {P, {{1.0, P}, {dz / 6, SP}}}});
 }
 
 int runner_begin(trhepd_ctx *ctx, Batch &b, const Table &t)
