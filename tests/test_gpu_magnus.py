@@ -59,15 +59,41 @@ def test_magnus_periodic_matches_rule(engine, method):
     assert e < 1e-10
 
 
-@pytest.mark.parametrize("method", [1, 2])
-def test_magnus_n521_vs_oracle(engine, method):
-    surf = O.load_case(os.path.join(REF, "test521"), 0.68, method)
+def _one_ulp_response(surf):
+    """How far the oracle's own curve moves when the potential table is scaled by
+    1 + 2^-52 (per-beam relative, normalised L2): the rounding amplification of the method
+    at this deck and dz."""
+    v, vi = O.table(surf)
+    r1, r2, ga, az = surf.angles()[0]
+    g = O.gamma(surf, ga, az)
+    f = surf.scatter_f(r1, r2)
+    a = O.intensity(surf, O.integrate(surf, f.copy(), g, v, vi)[0], g, ga)
+    b = O.intensity(surf, O.integrate(surf, f.copy(), g, v * (1 + 2.0 ** -52), vi)[0], g, ga)
+    return O.rel_err(b, a), np.linalg.norm(a - b) / np.linalg.norm(a)
+
+
+@pytest.mark.parametrize("method,dz", [(1, "0.68"), (2, "0.68"), (3, "0.68"), (2, "0.1")])
+def test_magnus_n521_vs_oracle(engine, method, dz):
+    """m2/m4 at the N = 521 FP64 bar (1e-8 per beam, 1e-10 normalised L2).  m6 at dz 0.68
+    amplifies rounding about 75x more than m4 (a one-ulp change of the table moves the
+    oracle's own curve by ~1.5e-8 per beam, 2e-11 L2, against 1.9e-10 / 7e-14 for m4), so
+    its bar is 10x that one-ulp response, as for RK4."""
+    print("case test521", dz, "method", method, flush=True)
+    surf = O.load_case(os.path.join(REF, "test521"), float(dz), method)
     f2, st = _device(engine, surf, method)
-    rows = O.surface_curve(surf)
+    stats = []
+    rows = O.surface_curve(surf, stats=stats)
+    ren = int(sum(s[0] for s in stats))
     e = O.rel_err(f2[0], rows[0][1])
     l2 = np.linalg.norm(f2[0] - rows[0][1]) / np.linalg.norm(rows[0][1])
-    print("N=521 method %d: rel %.3e L2 %.3e renorms %d" % (method, e, l2, st["n_renorm"]))
-    assert e < 1e-8 and l2 < 1e-10
+    print("N=521 method %d dz %s: rel %.3e L2 %.3e renorms device %d oracle %d" % (method, dz, e, l2, st["n_renorm"], ren))
+    bar_e, bar_l2 = 1e-8, 1e-10
+    if method == 3:
+        s_e, s_l2 = _one_ulp_response(surf)
+        print("one-ulp response rel %.3e L2 %.3e" % (s_e, s_l2))
+        bar_e, bar_l2 = max(bar_e, 10 * s_e), max(bar_l2, 10 * s_l2)
+    assert e < bar_e and l2 < bar_l2
+    assert abs(st["n_renorm"] - ren) <= max(1, ren // 100)
 
 
 def test_magnus_in_mixed_context_is_fp64(engine, mixed_engine):
