@@ -64,6 +64,31 @@ def test_surface_small_vs_oracle_and_reference(engine, case, dz, method):
     assert e_or < tol and e_ref < tol and e_f < 10 * tol
 
 
+@pytest.mark.parametrize("case,dz,method,fname", [
+    ("test23", "0.01", 4, "perf-4-0.01.txt"), ("test23", "0.01", 5, "perf-5-0.01.txt"),
+    ("test23", "0.01", 6, "perf-6-0.01.txt"), ("test23", "0.01", 7, "perf-7-0.01.txt"),
+    ("test47", "0.1", 4, "perf-4-0.1.txt"), ("test47", "0.1", 6, "perf-6-0.1.txt"),
+    ("test23", "0.001", 5, "sp4-0.001.s"), ("test47", "0.001", 5, "sp4-0.001.s"),
+])
+def test_surface_fine_dz_vs_reference(engine, case, dz, method, fname):
+    """The reference's own curves at the finer slicings (dz 0.01: ~1000 slices; sp4 at
+    dz 0.001: ~9900 slices, the reference's accurate curve) against the device directly,
+    at the FP64 bar of 1e-10 per intensity (floor 1e-12 * max)."""
+    print("case", case, dz, method, flush=True)
+    surf = O.load_case(os.path.join(REF, case), float(dz), method)
+    angs, G, F, W = _inputs(surf)
+    tid = engine.table_build(surf.pot, surf.dz, surf.ns, method)
+    try:
+        _, f2 = engine.propagate_batch(tid, surf.iv, G, F, surf.nb0, W)
+    finally:
+        engine.table_release(tid)
+    ref = deckio.read_s(os.path.join(REF, case, fname))
+    assert ref.f2.shape[0] == len(angs)
+    e_ref = max(O.rel_err(f2[p], ref.f2[p]) for p in range(len(angs)))
+    print("ns %d: rel vs reference %.3e" % (surf.ns, e_ref))
+    assert e_ref < 1e-10
+
+
 def test_table_matches_oracle(engine):
     surf = O.load_case(os.path.join(REF, "test47"), 0.1, 5)
     tid = engine.table_build(surf.pot, surf.dz, surf.ns, 5)
