@@ -15,7 +15,7 @@ from test_gpu_surface import _inputs, _oracle_rows
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("dz,method", [("0.68", 5), ("0.68", 6), ("0.47", 5)])
+@pytest.mark.parametrize("dz,method", [("0.68", 5), ("0.68", 6), ("0.47", 5), ("0.47", 6)])
 def test_surface_n521_vs_oracle_and_reference(engine, dz, method):
     print("case test521", dz, method, flush=True)
     surf = O.load_case(os.path.join(REF, "test521"), float(dz), method)
