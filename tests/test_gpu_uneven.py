@@ -63,6 +63,21 @@ def test_batch_held_by_graph_is_not_released(reqs):
         res.release()
 
 
+def test_table_held_by_batch_or_graph_is_not_released(reqs):
+    """A table stays while a resident batch (or a graph that captured one) reads it."""
+    with lib.Engine(0) as eng:
+        eng.set_renorm_period(-1)
+        res = uneven.Resident(eng, reqs[:2])
+        res.capture()
+        for tid in res.tids:
+            with pytest.raises(lib.TrhepdError) as e:
+                eng.table_release(tid)
+            assert e.value.code == lib.E_ARG
+        res.launch()           # the graph still replays on intact tables
+        assert all(np.all(np.isfinite(f2)) for f2 in res.results())
+        res.release()
+
+
 def test_autotuned_contexts_agree(reqs):
     """Two contexts tune independently (timing may pick different tiles) and must agree."""
     out = []

@@ -52,26 +52,6 @@ void dev_free(trhepd_ctx *ctx, void *p, size_t bytes)
     ctx->held -= bytes;
 }
 
-int scratch(trhepd_ctx *ctx, int slot, size_t bytes, void **p)
-{
-    if ((int)ctx->bufs.size() <= slot) ctx->bufs.resize(slot + 1);
-    DevBuf &b = ctx->bufs[slot];
-    if (b.bytes < bytes) {
-        if (b.p) {
-            TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-            dev_free(ctx, b.p, b.bytes);
-            b.p = nullptr;
-            b.bytes = 0;
-        }
-        const size_t want = (bytes + 4095) & ~size_t(4095);
-        int rc = dev_alloc(ctx, &b.p, want);
-        if (rc) return rc;
-        b.bytes = want;
-    }
-    *p = b.p;
-    return TRHEPD_OK;
-}
-
 int prof_mark(trhepd_ctx *ctx, hipStream_t stream)
 {
     if (ctx->capturing) return TRHEPD_OK;
@@ -245,7 +225,6 @@ int trhepd_ctx_destroy(trhepd_ctx *ctx)
         (void)hipStreamDestroy(b.stream);
     }
     for (hipEvent_t e : ctx->prof.ev) (void)hipEventDestroy(e);
-    for (DevBuf &b : ctx->bufs) dev_free(ctx, b.p, b.bytes);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
     (void)hipStreamDestroy(ctx->stream);
@@ -402,6 +381,10 @@ int trhepd_table_fetch(trhepd_ctx *ctx, int table_id, double *v, double *vi)
 int trhepd_table_release(trhepd_ctx *ctx, int table_id)
 {
     if (!ctx) return TRHEPD_E_ARG;
+    // a resident batch (and so any graph that captured it) reads the table on every run
+    for (const Batch &b : ctx->batches)
+        if (b.table_id == table_id)
+            return fail(ctx, TRHEPD_E_ARG, "table %d is used by batch %d: release the batch first", table_id, b.id);
     for (size_t i = 0; i < ctx->tables.size(); ++i)
         if (ctx->tables[i].id == table_id) {
             TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -700,6 +683,11 @@ int trhepd_graph_launch(trhepd_ctx *ctx, int graph_id)
     for (GraphRec &g : ctx->graphs)
         if (g.id == graph_id) gr = &g;
     if (!gr) return fail(ctx, TRHEPD_E_NOTFOUND, "no graph %d", graph_id);
+    for (int id : gr->batch_ids) {
+        const Batch *b = find_batch(ctx, id);
+        if (!b || !find_table(ctx, b->table_id))
+            return fail(ctx, TRHEPD_E_NOTFOUND, "graph %d lost batch %d or its table", graph_id, id);
+    }
     TRY_HIP(ctx, hipSetDevice(ctx->device));
     ctx->stats = Stats();
     TRY_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
@@ -712,9 +700,10 @@ int trhepd_graph_launch(trhepd_ctx *ctx, int graph_id)
     int singular = 0;
     for (int id : gr->batch_ids) {
         Batch *b = find_batch(ctx, id);
+        const Table *t = b ? find_table(ctx, b->table_id) : nullptr;
+        if (!b || !t) return fail(ctx, TRHEPD_E_NOTFOUND, "graph %d lost batch %d or its table", graph_id, id);
         int cnt[16];
         TRY_HIP(ctx, hipMemcpy(cnt, b->cnt, 64, hipMemcpyDeviceToHost));
-        const Table *t = find_table(ctx, b->table_id);
         ctx->stats.n_renorm += cnt[0];
         ctx->stats.n_products += (long long)b->nang * t->ns * rkn_coef(t->method).s;
         singular |= cnt[1];

@@ -40,11 +40,6 @@ struct Table {
     size_t bytes = 0;
 };
 
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-};
-
 struct Stats {
     long long n_renorm = 0;
     long long n_products = 0;
@@ -105,8 +100,6 @@ struct trhepd_ctx {
     std::string err;
     int next_table = 1;
     std::vector<trhepd::Table> tables;
-    // scratch owned by the context, grown on demand (kept across calls)
-    std::vector<trhepd::DevBuf> bufs;
     trhepd::Stats stats;
     int next_batch = 1;
     int renorm_period = 0;   // 0: reference rule (cond_est > 1e3); K > 0: every K slices
@@ -134,8 +127,6 @@ int hip_check(trhepd_ctx *ctx, hipError_t e, const char *what);
 // device allocation accounted against the context budget
 int dev_alloc(trhepd_ctx *ctx, void **p, size_t bytes);
 void dev_free(trhepd_ctx *ctx, void *p, size_t bytes);
-// scratch slot `slot` of at least `bytes` (reused across calls)
-int scratch(trhepd_ctx *ctx, int slot, size_t bytes, void **p);
 
 Table *find_table(trhepd_ctx *ctx, int id);
 Batch *find_batch(trhepd_ctx *ctx, int id);

@@ -72,6 +72,7 @@ struct Request {
     hipEvent_t ev[RUN_AHEAD] = {};       // end of the last RUN_AHEAD slices
     Clock::time_point t_submit, t_done;
     int rc = TRHEPD_OK;
+    long long n_renorm = 0;              // renormalisations summed over the request's angles
     std::string err;
 };
 
@@ -139,6 +140,14 @@ static int complete(trhepd_ctx *ctx, Request &r)
     r.f.assign((size_t)2 * r.nb * r.nb * r.nang, 0.0);
     r.f2.assign((size_t)r.nb * r.nang, 0.0);
     int rc = trhepd_batch_fetch(ctx, r.batch_id, r.f.data(), r.f2.data());
+    // counters of the run: [0] renormalisations, [1] exact zero pivots seen by the LU
+    int cnt[16] = {};
+    if (!rc) {
+        const Batch *b = find_batch(ctx, r.batch_id);
+        rc = hip_check(ctx, hipMemcpy(cnt, b->cnt, sizeof cnt, hipMemcpyDeviceToHost), "request counters");
+    }
+    if (!rc) r.n_renorm = cnt[0];
+    if (!rc && cnt[1]) rc = fail(ctx, TRHEPD_E_SINGULAR, "request %d: exact zero pivot in a renormalisation LU", r.id);
     r.t_done = Clock::now();
     drop_device_state(ctx, r);
     return rc;
