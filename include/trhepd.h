@@ -154,6 +154,20 @@ int trhepd_propagate_batch(trhepd_ctx *ctx, int table_id, int nb, int nang, cons
                            const double *gamma, double *f, double *f2, int nb0,
                            const double *wnsga2);
 
+/* trhepd_propagate_batch on device-resident arrays, allocating nothing.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  iv, gamma, f, f2 and wnsga2 are
+ * DEVICE pointers with the shapes of trhepd_propagate_batch; f is updated in place.  The
+ * library's own work (signed gather map, counters, flags) lives in a fixed device array of
+ * 9472 bytes per device, so N <= 48 (the small-N path) and int_method 4-7 only; larger N or
+ * other methods return TRHEPD_E_ARG.  All contexts on one device share that array: calls that
+ * overlap on one device race.  The call synchronises the context's stream before it returns;
+ * statuses are those of trhepd_propagate_batch (an index of iv outside 1..nv is refused before
+ * f is touched). */
+int trhepd_propagate_batch_noalloc(trhepd_ctx *ctx, int table_id, int nb, int nang, const int *iv,
+                                   const double *gamma, double *f, double *f2, int nb0,
+                                   const double *wnsga2);
+
 /* The same slice loop with device-resident inputs, for callers that run a batch more
  * than once or keep several batches on the device (serving).  batch_create uploads the
  * inputs (same arguments as trhepd_propagate_batch); batch_run integrates all angles,

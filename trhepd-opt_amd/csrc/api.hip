@@ -201,6 +201,8 @@ int trhepd_ctx_create(const trhepd_opts *opts, trhepd_ctx **out)
         delete c;
         return TRHEPD_E_HIP;
     }
+    // device address of the fixed work area of trhepd_propagate_batch_noalloc on this device
+    noalloc_attach(c);
     *out = c;
     return TRHEPD_OK;
 }

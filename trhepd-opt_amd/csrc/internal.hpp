@@ -100,6 +100,10 @@ struct trhepd_ctx {
     std::string err;
     int next_table = 1;
     std::vector<trhepd::Table> tables;
+    // device address of the file-scope work area of trhepd_propagate_batch_noalloc on this
+    // context's device (null if it could not be obtained; noalloc_err then holds the status)
+    void *noalloc = nullptr;
+    hipError_t noalloc_err = hipSuccess;
     trhepd::Stats stats;
     int next_batch = 1;
     int renorm_period = 0;   // 0: reference rule (cond_est > 1e3); K > 0: every K slices
@@ -127,6 +131,8 @@ int hip_check(trhepd_ctx *ctx, hipError_t e, const char *what);
 // device allocation accounted against the context budget
 int dev_alloc(trhepd_ctx *ctx, void **p, size_t bytes);
 void dev_free(trhepd_ctx *ctx, void *p, size_t bytes);
+// looks up this device's copy of the no-allocation entry point's work area (device current)
+void noalloc_attach(trhepd_ctx *ctx);
 
 Table *find_table(trhepd_ctx *ctx, int id);
 Batch *find_batch(trhepd_ctx *ctx, int id);

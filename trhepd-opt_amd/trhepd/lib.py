@@ -28,7 +28,8 @@ SYMBOLS = ("trhepd_ctx_create", "trhepd_ctx_destroy", "trhepd_last_error", "trhe
            "trhepd_propagate_batch", "trhepd_batch_create", "trhepd_batch_run", "trhepd_batch_run_many", "trhepd_batch_fetch",
            "trhepd_batch_release", "trhepd_last_stats", "trhepd_set_renorm_period", "trhepd_set_mixed_digits", "trhepd_profile",
            "trhepd_kernel_stats", "trhepd_serve_submit", "trhepd_serve_step", "trhepd_serve_result",
-           "trhepd_graph_create", "trhepd_graph_launch", "trhepd_graph_release", "trhepd_bulk_reflect")
+           "trhepd_graph_create", "trhepd_graph_launch", "trhepd_graph_release", "trhepd_bulk_reflect",
+           "trhepd_propagate_batch_noalloc")
 
 
 class TrhepdError(RuntimeError):
@@ -78,6 +79,7 @@ def load() -> C.CDLL:
     L.trhepd_table_fetch.argtypes = [vp, i, vp, vp]
     L.trhepd_table_release.argtypes = [vp, i]
     L.trhepd_propagate_batch.argtypes = [vp, i, i, i, vp, vp, vp, vp, i, vp]
+    L.trhepd_propagate_batch_noalloc.argtypes = [vp, i, i, i, vp, vp, vp, vp, i, vp]
     L.trhepd_last_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), dp]
     L.trhepd_batch_create.argtypes = [vp, i, i, i, vp, vp, vp, i, vp, ip]
     L.trhepd_batch_run.argtypes = [vp, i]
@@ -333,6 +335,16 @@ class Engine:
                                                  _p(w)))
         return ff.transpose(0, 2, 1), f2
 
+    def propagate_batch_noalloc(self, tid: int, nb: int, nang: int, d_iv: int, d_gamma: int, d_f: int,
+                                nb0: int, d_f2: Optional[int] = None, d_wnsga2: Optional[int] = None):
+        """trhepd_propagate_batch_noalloc on caller-owned device memory (addresses as ints, the
+        library's layouts: iv (nb, nb) int32 column-major, gamma (nb, nang), f (nb, nb, nang)
+        complex column-major, f2 (nb, nang), wnsga2 (nang)).  N <= 48; allocates nothing."""
+        self._chk(self._L.trhepd_propagate_batch_noalloc(self._h, tid, nb, nang, C.c_void_p(d_iv),
+                                                         C.c_void_p(d_gamma), C.c_void_p(d_f),
+                                                         None if d_f2 is None else C.c_void_p(d_f2), nb0,
+                                                         None if d_wnsga2 is None else C.c_void_p(d_wnsga2)))
+
     # -- device-resident batches ----------------------------------------------
     def batch_create(self, tid: int, iv, gamma, f, nb0: int, wnsga2=None) -> int:
         """Upload one batch (same conventions as propagate_batch); returns its id."""
@@ -386,3 +398,44 @@ class Engine:
         r, p, ms = C.c_longlong(), C.c_longlong(), C.c_double()
         self._chk(self._L.trhepd_last_stats(self._h, C.byref(r), C.byref(p), C.byref(ms)))
         return {"n_renorm": r.value, "n_products": p.value, "device_ms": ms.value}
+
+
+class DeviceArray:
+    """A device copy of a numpy array, allocated through the HIP runtime the library links
+    (for callers that hand device memory to the library, e.g. trhepd_propagate_batch_noalloc)."""
+
+    _hip = None
+
+    @classmethod
+    def _rt(cls):
+        if cls._hip is None:
+            load()                               # resolves libamdhip64 the way the library does
+            H = C.CDLL("libamdhip64.so.7")
+            H.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            H.hipFree.argtypes = [C.c_void_p]
+            H.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            H.hipSetDevice.argtypes = [C.c_int]
+            cls._hip = H
+        return cls._hip
+
+    def __init__(self, a: np.ndarray, device: int = 0):
+        H = self._rt()
+        self.a = np.ascontiguousarray(a)
+        self.nbytes = self.a.nbytes
+        p = C.c_void_p()
+        if H.hipSetDevice(device) or H.hipMalloc(C.byref(p), max(self.nbytes, 1)):
+            raise TrhepdError(E_NOMEM, "hipMalloc of %d bytes failed" % self.nbytes)
+        self.ptr = p.value
+        if H.hipMemcpy(p, self.a.ctypes.data_as(C.c_void_p), self.nbytes, 1):   # host to device
+            raise TrhepdError(E_HIP, "hipMemcpy to the device failed")
+
+    def get(self) -> np.ndarray:
+        out = np.empty_like(self.a)
+        if self._rt().hipMemcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), self.nbytes, 2):
+            raise TrhepdError(E_HIP, "hipMemcpy from the device failed")
+        return out
+
+    def free(self):
+        if self.ptr:
+            self._rt().hipFree(C.c_void_p(self.ptr))
+            self.ptr = 0
