@@ -60,6 +60,45 @@ def cpu_baseline(st, blocks, method, dz2, threads, sample_slices):
                                                                    sample_slices)}
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_reference(st, blocks, method, dz2, threads, trials):
+    """The reference's own OpenMP + MKL program (sim-trhepd-rheed surf.exe, built from its
+    sources by oracle/ref.mk) on the first angle of the same deck: the same N, slab and dz,
+    so the same per-point work.  At N >= 100 the reference integrates angles one after
+    another with MKL's threads inside each product (surflf.f90:73-78), so one angle's time is
+    its time per point.  Time = the omp_get_wtime delta the program prints
+    (surfm.f90:95-96), median over `trials` runs (test/test23/perf-test.sh:21-35 uses 5)."""
+    import copy
+    import statistics
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import refrun
+    bp = copy.deepcopy(st.bp)
+    bp.ng = 1
+    bp.domains[0].blocks = [[blocks[0][0]]]
+    surf_txt, bulkp = st.surf.format().encode(), bp.to_bytes()
+    times = []
+    for k in range(trials):
+        _, t, wall = refrun.surf(surf_txt, bulkp, dz2, method, threads=threads, timeout=600)
+        times.append(t if t else wall)
+        log("reference CPU trial %d: %.2f s" % (k, times[-1]))
+    med = statistics.median(times)
+    return {"value": 1.0 / med, "unit": "points/s", "cores": threads, "kind": "reference",
+            "sample": "sim-trhepd-rheed surf.exe built from the reference's sources (amdflang -O3 -fopenmp, "
+                      "MKL LP64 + GNU threading), sp4 dz=%.2f, 1 of the 40 angles of the same N=%d deck (%d "
+                      "trials, median of the program's own omp_get_wtime, OMP_NUM_THREADS=%d): %s s"
+                      % (dz2, bp.nb[0], trials, threads, ", ".join("%.2f" % t for t in times)),
+            "cpu": _cpu_model()}
+
+
 def run_path(eng, st, blocks, args, method, dz2):
     """Times `args.steps` full curves of one precision path; returns the figures and the
     intensities of the last step."""
@@ -126,7 +165,8 @@ def main():
     ap.add_argument("--no-one-batch", action="store_true",
                     help="skip the untimed single-batch curve that measures the kernels without stream sharing")
     ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--cpu-slices", type=int, default=256)
+    ap.add_argument("--cpu-slices", type=int, default=256, help="slices the C oracle fallback times")
+    ap.add_argument("--cpu-trials", type=int, default=3, help="runs of the reference CPU program (median)")
     ap.add_argument("--renorm-period", type=int, default=-1,
                     help="0: the reference's Gershgorin rule; K: every K slices; -1: K from the evanescent growth rate")
     ap.add_argument("--split", type=int, default=0,
@@ -304,10 +344,15 @@ def main():
         out["serving"] = sv
         out["p99_ms"] = sv.get("mixed", sv.get("fp64"))["p99_ms"]
     if not args.no_cpu:
+        # the reference's own build when oracle/ref.mk has built it, else the C oracle
         try:
-            out["cpu_baseline"] = cpu_baseline(st, blocks, method, dz2, args.cpu_threads, args.cpu_slices)
+            out["cpu_baseline"] = cpu_reference(st, blocks, method, dz2, args.cpu_threads, args.cpu_trials)
         except Exception as e:   # the CPU leg is reported, never fatal to the GPU figure
-            out["cpu_baseline"] = {"value": None, "error": repr(e)}
+            log("reference CPU build unavailable (%r): timing the C oracle instead" % (e,))
+            try:
+                out["cpu_baseline"] = cpu_baseline(st, blocks, method, dz2, args.cpu_threads, args.cpu_slices)
+            except Exception as e2:
+                out["cpu_baseline"] = {"value": None, "error": repr(e2)}
     eng64.close()
     print(json.dumps(out))
 
