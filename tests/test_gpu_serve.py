@@ -97,3 +97,30 @@ def test_request_larger_than_the_budget_fails(reqs):
         with pytest.raises(lib.TrhepdError) as e:
             eng.serve_result(rid)
     assert e.value.code == lib.E_BUDGET
+
+
+def test_mix_with_an_n521_request(engine, reqs):
+    """An N = 521 request (the largest of the serving configuration) admitted while smaller
+    ones run, and a small one arriving while it runs: every result is bit-identical to the
+    request served alone and to propagate_batch."""
+    st = _structure_with(521)
+    r = serve.structure_request(engine, st, dz2=0.1, key="n521")
+    k = slice(0, 4)
+    big = serve.Request(r.key, r.pot, r.dz, r.ns, r.method, r.iv, r.G[k], r.F[k], r.W[k], r.nb0)
+    mix = [reqs[1], big, reqs[0], reqs[2]]
+    print("N", [q.nb for q in mix], "slices", [q.ns for q in mix], flush=True)
+    with lib.Engine(0) as eng:
+        eng.set_renorm_period(-1)
+        alone = _alone(eng, mix)
+        run = serve.serve(eng, mix, arrivals=[0.0, 0.02, 0.3, 0.35])
+        tid = eng.table_build(big.pot, big.dz, big.ns, big.method)
+        try:
+            _, ref = eng.propagate_batch(tid, big.iv, big.G, big.F, big.nb0, big.W)
+        finally:
+            eng.table_release(tid)
+    print("peak admitted", run["peak_active"], "latencies ms", [round(q[2], 1) for q in run["results"]])
+    for a, b in zip(alone, [q[1] for q in run["results"]]):
+        assert np.array_equal(a, b)
+        assert np.all(np.isfinite(a)) and np.all(a >= 0)
+    assert np.array_equal(alone[1], ref)
+    assert run["peak_active"] >= 2

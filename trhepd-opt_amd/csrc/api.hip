@@ -125,25 +125,74 @@ RknCoef rkn_coef(int method)
     return c;
 }
 
-int batch_start(trhepd_ctx *ctx, Batch &b, const Table &t)
+// counters cleared, F reset to the uploaded input, start event
+static int batch_prelude(trhepd_ctx *ctx, Batch &b)
 {
     const size_t nn = (size_t)b.nb * b.nb;
     b.stats = Stats();
     launch_fill_i32(b.stream, b.cnt, 16, 0);
     TRY_HIP(ctx, hipEventRecord(b.ev0, b.stream));
     TRY_HIP(ctx, hipMemcpyAsync(b.fout, b.fin, nn * b.nang * 16, hipMemcpyDeviceToDevice, b.stream));
+    return TRHEPD_OK;
+}
+
+int batch_start(trhepd_ctx *ctx, Batch &b, const Table &t)
+{
+    int rc = batch_prelude(ctx, b);
+    if (rc) return rc;
     if (b.run) return runner_begin(ctx, b, t);
     return run_small(ctx, b.stream, batch_period(ctx, b, t), t, b.nb, b.nang, b.gmap, b.gamma, b.fout,
                      b.has_f2 ? b.f2 : nullptr, b.nb0, b.w, b.cnt);
+}
+
+// Small-N batches that share N, integrator, dz and nb0 (structures with the same beams)
+// run as one launch on the first one's stream; the others' streams wait for their prelude
+// before it and resume after it, so every batch's own events still bracket its work.
+static int start_small_group(trhepd_ctx *ctx, const std::vector<Batch *> &g, const std::vector<const Table *> &gt)
+{
+    if (g.size() == 1) return batch_start(ctx, *g[0], *gt[0]);
+    std::vector<SmallBatch> sb;
+    for (size_t i = 0; i < g.size(); ++i) {
+        Batch &b = *g[i];
+        int rc = batch_prelude(ctx, b);
+        if (rc) return rc;
+        sb.push_back(SmallBatch{gt[i], b.nb, b.nang, b.nb0, batch_period(ctx, b, *gt[i]), b.gmap, b.gamma, b.fout,
+                                b.has_f2 ? b.f2 : nullptr, b.w, b.cnt});
+        if (i) {
+            TRY_HIP(ctx, hipEventRecord(b.evs, b.stream));
+            TRY_HIP(ctx, hipStreamWaitEvent(g[0]->stream, b.evs, 0));
+        }
+    }
+    int rc = run_small_multi(ctx, g[0]->stream, sb);
+    if (rc) return rc;
+    TRY_HIP(ctx, hipEventRecord(g[0]->evs, g[0]->stream));
+    for (size_t i = 1; i < g.size(); ++i) TRY_HIP(ctx, hipStreamWaitEvent(g[i]->stream, g[0]->evs, 0));
+    return TRHEPD_OK;
 }
 
 int enqueue_batches(trhepd_ctx *ctx, const std::vector<Batch *> &bs, const std::vector<const Table *> &ts)
 {
     const int n = (int)bs.size();
     int rc = TRHEPD_OK, L = 0;
+    std::vector<bool> done(n, false);
     for (int i = 0; i < n && !rc; ++i) {
-        rc = batch_start(ctx, *bs[i], *ts[i]);
-        if (bs[i]->run) L = std::max(L, ts[i]->ns);
+        if (done[i]) continue;
+        if (bs[i]->run || !small_fusable(bs[i]->nb, ts[i]->method)) {
+            rc = batch_start(ctx, *bs[i], *ts[i]);
+            if (bs[i]->run) L = std::max(L, ts[i]->ns);
+            done[i] = true;
+            continue;
+        }
+        std::vector<Batch *> g;
+        std::vector<const Table *> gt;
+        for (int j = i; j < n; ++j)
+            if (!done[j] && !bs[j]->run && bs[j]->nb == bs[i]->nb && ts[j]->method == ts[i]->method &&
+                ts[j]->dz == ts[i]->dz && bs[j]->nb0 == bs[i]->nb0 && small_fusable(bs[j]->nb, ts[j]->method)) {
+                g.push_back(bs[j]);
+                gt.push_back(ts[j]);
+                done[j] = true;
+            }
+        rc = start_small_group(ctx, g, gt);
     }
     // slices of all large-N batches interleaved: while the host waits for one batch's
     // renormalisation flags the others' kernels are already queued on their streams
@@ -224,6 +273,7 @@ int trhepd_ctx_destroy(trhepd_ctx *ctx)
         dev_free(ctx, b.d, b.bytes);
         (void)hipEventDestroy(b.ev0);
         (void)hipEventDestroy(b.ev1);
+        (void)hipEventDestroy(b.evs);
         (void)hipStreamDestroy(b.stream);
     }
     for (hipEvent_t e : ctx->prof.ev) (void)hipEventDestroy(e);
@@ -447,6 +497,7 @@ int trhepd_batch_create(trhepd_ctx *ctx, int table_id, int nb, int nang, const i
     hipError_t e = hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking);
     if (!e) e = hipEventCreate(&b.ev0);
     if (!e) e = hipEventCreate(&b.ev1);
+    if (!e) e = hipEventCreateWithFlags(&b.evs, hipEventDisableTiming);
     if (!e) e = hipMemcpyAsync(b.gmap, gm.data(), nn * 4, hipMemcpyHostToDevice, b.stream);
     if (!e) e = hipMemcpyAsync(b.gamma, gamma, (size_t)nb * nang * 16, hipMemcpyHostToDevice, b.stream);
     if (!e) e = hipMemcpyAsync(b.fin, f, nn * nang * 16, hipMemcpyHostToDevice, b.stream);
@@ -459,6 +510,7 @@ int trhepd_batch_create(trhepd_ctx *ctx, int table_id, int nb, int nang, const i
         dev_free(ctx, b.d, b.bytes);
         if (b.ev0) (void)hipEventDestroy(b.ev0);
         if (b.ev1) (void)hipEventDestroy(b.ev1);
+        if (b.evs) (void)hipEventDestroy(b.evs);
         if (b.stream) (void)hipStreamDestroy(b.stream);
         return rc2;
     }
@@ -558,6 +610,7 @@ int trhepd_batch_release(trhepd_ctx *ctx, int batch_id)
             dev_free(ctx, b.d, b.bytes);
             (void)hipEventDestroy(b.ev0);
             (void)hipEventDestroy(b.ev1);
+            (void)hipEventDestroy(b.evs);
             (void)hipStreamDestroy(b.stream);
             ctx->batches.erase(ctx->batches.begin() + i);
             return TRHEPD_OK;

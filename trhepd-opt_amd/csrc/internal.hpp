@@ -62,6 +62,7 @@ struct Batch {
     int *cnt = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t evs = nullptr;   // orders this batch's stream against a launch shared with others
     Runner *run = nullptr;
     Stats stats;
     double gim_max = 0.0;   // max |Im gamma| over beams and angles (evanescent growth rate)
@@ -218,6 +219,20 @@ void launch_stage_i8(hipStream_t st, const MixedStage &S);
 int run_small(trhepd_ctx *ctx, hipStream_t stream, int period, const Table &t, int nb, int nang, const int *d_gmap,
               const double2 *d_gamma, double2 *d_f, double *d_f2, int nb0, const double *d_wnsga2,
               int *d_counters);
+// several small-N batches (same N, integrator, dz and nb0; any tables and slice counts) in
+// one launch of the small-N kernel, one workgroup per (batch, angle)
+struct SmallBatch {
+    const Table *t;
+    int nb, nang, nb0, period;
+    const int *gmap;
+    const double2 *gamma;
+    double2 *f;
+    double *f2;
+    const double *w;
+    int *cnt;
+};
+bool small_fusable(int nb, int method);
+int run_small_multi(trhepd_ctx *ctx, hipStream_t stream, const std::vector<SmallBatch> &bs);
 // batched large-N path, split into phases so that several batches interleave:
 // runner_create allocates the batch's working memory; begin enqueues the boundary
 // transform; slice_enqueue(l) the stages of slice l up to the end kick (the renormalisation

@@ -331,13 +331,12 @@ __device__ void lu_solve(SLds<NP> &S, int N)
     __syncthreads();
 }
 
+// The whole integration of angle `pt` of one batch by one workgroup.
 template <int NP>
-__global__ __launch_bounds__(256) void k_small(SmallArgs A)
+__device__ __forceinline__ void small_body(const SmallArgs &A, const int pt, SLds<NP> &S)
 {
-    __shared__ SLds<NP> S;
     constexpr int LD = SLds<NP>::LD;
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int pt = blockIdx.x;
     const int N = A.N;
     const double dz = A.dz;
     for (int e = tid; e < NP * LD; e += nt) {
@@ -538,6 +537,59 @@ __global__ __launch_bounds__(256) void k_small(SmallArgs A)
     }
 }
 
+// one workgroup per angle of one batch
+template <int NP>
+__global__ __launch_bounds__(256) void k_small(SmallArgs A)
+{
+    __shared__ SLds<NP> S;
+    small_body<NP>(A, blockIdx.x, S);
+}
+
+// Batches that differ only in their table, arrays and slice count (several structures with
+// the same beams, integrator and step) in one launch: one workgroup per (batch, angle), the
+// batch found from the first global workgroup of each.  The per-batch part travels in the
+// kernel arguments, so a launch needs no device memory of its own.
+constexpr int SMALL_MULTI = 24;
+struct SmallPart {
+    const VVi *table;
+    const int *gmap;
+    const double2 *gamma;
+    double2 *f;
+    double *f2;
+    const double *wnsga2;
+    int *counters;
+    int nv, ns, period, first;
+};
+struct SmallMultiArgs {
+    SmallArgs base;   // the fields every batch shares (N, method, dz, coefficients, nb0)
+    int nbatch;
+    SmallPart part[SMALL_MULTI];
+};
+static_assert(sizeof(SmallMultiArgs) <= 4096, "kernel arguments of the fused launch exceed 4 KiB");
+
+template <int NP>
+__global__ __launch_bounds__(256) void k_small_multi(SmallMultiArgs M)
+{
+    __shared__ SLds<NP> S;
+    const int blk = blockIdx.x;
+    int k = 0;
+    for (int u = 1; u < M.nbatch; ++u)
+        if (blk >= M.part[u].first) k = u;
+    const SmallPart &P = M.part[k];
+    SmallArgs A = M.base;
+    A.table = P.table;
+    A.gmap = P.gmap;
+    A.gamma = P.gamma;
+    A.f = P.f;
+    A.f2 = P.f2;
+    A.wnsga2 = P.wnsga2;
+    A.counters = P.counters;
+    A.nv = P.nv;
+    A.ns = P.ns;
+    A.period = P.period;
+    small_body<NP>(A, blk - P.first, S);
+}
+
 // hp_int_spa1d (surf_prkn.f90:485-524): the scalar N = 1 form used by int_method 6.
 __global__ void k_spa1d(SmallArgs A, int npts)
 {
@@ -615,6 +667,54 @@ int run_small(trhepd_ctx *ctx, hipStream_t stream, int period, const Table &t, i
         return fail(ctx, TRHEPD_E_ARG, "small path supports N <= 48 (got %d)", nb);
     }
     TRY_HIP(ctx, hipGetLastError());
+    return TRHEPD_OK;
+}
+
+}  // namespace trhepd
+
+namespace trhepd {
+
+bool small_fusable(int nb, int method) { return nb >= 1 && nb <= 48 && !(nb == 1 && method == TRHEPD_SP6); }
+
+int run_small_multi(trhepd_ctx *ctx, hipStream_t stream, const std::vector<SmallBatch> &bs)
+{
+    if (bs.empty()) return TRHEPD_OK;
+    const int nb = bs[0].nb;
+    for (size_t i0 = 0; i0 < bs.size(); i0 += SMALL_MULTI) {
+        const size_t i1 = std::min(bs.size(), i0 + SMALL_MULTI);
+        SmallMultiArgs M{};
+        const Table &t0 = *bs[i0].t;
+        M.base.N = nb;
+        M.base.method = t0.method;
+        M.base.dz = t0.dz;
+        M.base.co = rkn_coef(t0.method);
+        M.base.nb0 = bs[i0].nb0;
+        M.nbatch = (int)(i1 - i0);
+        int blocks = 0;
+        for (size_t i = i0; i < i1; ++i) {
+            const SmallBatch &b = bs[i];
+            if (b.nb != nb || b.t->method != t0.method || b.t->dz != t0.dz || b.nb0 != bs[i0].nb0 ||
+                !small_fusable(b.nb, b.t->method))
+                return fail(ctx, TRHEPD_E_ARG, "fused small-N launch: batches differ in N, method, dz or nb0");
+            SmallPart &P = M.part[i - i0];
+            P.table = b.t->d;
+            P.gmap = b.gmap;
+            P.gamma = b.gamma;
+            P.f = b.f;
+            P.f2 = b.f2;
+            P.wnsga2 = b.w;
+            P.counters = b.cnt;
+            P.nv = b.t->nv;
+            P.ns = b.t->ns;
+            P.period = b.period;
+            P.first = blocks;
+            blocks += b.nang;
+        }
+        if (nb <= 16) hipLaunchKernelGGL(k_small_multi<16>, dim3(blocks), dim3(256), 0, stream, M);
+        else if (nb <= 32) hipLaunchKernelGGL(k_small_multi<32>, dim3(blocks), dim3(256), 0, stream, M);
+        else hipLaunchKernelGGL(k_small_multi<48>, dim3(blocks), dim3(256), 0, stream, M);
+        TRY_HIP(ctx, hipGetLastError());
+    }
     return TRHEPD_OK;
 }
 
