@@ -46,7 +46,7 @@ enum {
     TRHEPD_PREC_MIXED = 1    /* large-N stage products as exact int8 digit-sliced MFMA
                                 (base-254 digits per operand, int32 accumulation, FP64
                                 recombination); FP64 state, table, LU and boundary
-                                transforms.  N <= 48, the Magnus methods and rk4gs run
+                                transforms.  N <= 48, the Magnus methods, RK4 and rk4gs run
                                 the FP64 path. */
 };
 
@@ -148,8 +148,7 @@ int trhepd_bulk_reflect(trhepd_ctx *ctx, int table_id, int ngr, const int *nbg, 
  *   f2      (nb, nang) out intensities (surflf.f90:143-151), may be NULL
  *   nb0     1-based index of the (0,0) beam
  *   wnsga2  (nang) K sin^2(theta) per angle
- * The integrator is the table's int_method: 1-6 and 8 at every N, 7 for N <= 48
- * (TRHEPD_E_ARG above). */
+ * The integrator is the table's int_method (1-8), at every N up to 600. */
 int trhepd_propagate_batch(trhepd_ctx *ctx, int table_id, int nb, int nang, const int *iv,
                            const double *gamma, double *f, double *f2, int nb0,
                            const double *wnsga2);
@@ -229,7 +228,7 @@ int trhepd_set_mixed_digits(trhepd_ctx *ctx, int digits);
  * replaces one angle loop of srfref (surflf.f90:79-153) for one structure.
  *
  * trhepd_serve_submit queues a request: the potential descriptor of its structure (as for
- * trhepd_table_build), dz, ns and int_method (4, 5 or 6; 7 for N <= 48), and the angle batch
+ * trhepd_table_build), dz, ns and int_method (4, 5, 6 or 7), and the angle batch
  * (as for trhepd_batch_create; wnsga2 is required).  Every input is copied, so the caller's
  * arrays may be freed on return.  Nothing touches the device until admission.
  *

@@ -17,14 +17,15 @@ DST = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref")
 PICK = {
     "test23": ["0.68", "0.1", "0.01"],
     "test47": ["0.68", "0.1"],
-    "test521": ["0.68", "0.47"],
+    # N = 521: the coarse pair plus the paper's headline steps (sp4 0.068, sp6 0.15, rk4 0.015,
+    # test/plots-plus/table.tex:17-19) and the fine steps up to the bench's slice count and beyond
+    "test521": ["0.68", "0.47", "0.15", "0.1", "0.068", "0.047", "0.022", "0.015", "0.01"],
 }
 for case, dzs in PICK.items():
     os.makedirs(os.path.join(DST, case), exist_ok=True)
     for f in ("bulk.txt", "surf.txt", "bulkP.b"):
         shutil.copy(os.path.join(SRC, case, f), os.path.join(DST, case, f))
-    if case != "test521":
-        shutil.copy(os.path.join(SRC, case, "surf-bulkP-str-accurate.s"), os.path.join(DST, case, "sp4-0.001.s"))
+    shutil.copy(os.path.join(SRC, case, "surf-bulkP-str-accurate.s"), os.path.join(DST, case, "sp4-0.001.s"))
     for m in (4, 5, 6, 7):
         for dz in dzs:
             shutil.copy(os.path.join(SRC, case, "test-results", "perf-%d-%s.txt" % (m, dz)),

@@ -1,7 +1,8 @@
 """GPU parity of the batched large-N path (N = 521, the reference's test521 deck).
 
-Tolerance: normalised L2 <= 1e-10 per angle and <= 1e-8 per beam (the reference's own
-FP64 rounding noise at N = 521, see test_oracle_golden.py)."""
+Tolerance: per beam and normalised L2, max(1e-10, 10 x) the larger of the reference's own
+one-ulp response and the spread between two builds of the reference, both measured per
+(method, dz) by tests/golden/make_sensitivity.py (test_gpu_fine._bars)."""
 import os
 
 import numpy as np
@@ -10,6 +11,7 @@ import pytest
 import oracle as O
 from conftest import REF
 from trhepd import deckio
+from test_gpu_fine import _bars
 from test_gpu_surface import _inputs, _oracle_rows
 
 pytestmark = pytest.mark.gpu
@@ -34,10 +36,10 @@ def test_surface_n521_vs_oracle_and_reference(engine, dz, method):
     l2_or = np.linalg.norm(f2[0] - of2[0]) / np.linalg.norm(of2[0])
     print("stats", st, "rel vs oracle %.3e (L2 %.3e)  vs reference %.3e (L2 %.3e)" % (e_or, l2_or, e_ref, l2_ref))
     assert st["n_renorm"] >= 1
-    # per-beam: the reference's own FP64 noise at N = 521 (oracle vs reference 1.3e-9 on
-    # sp6 at dz 0.68); the normalised L2 error is the strict gate
-    assert e_or < 1e-8 and l2_or < 1e-10
-    assert e_ref < 1e-8 and l2_ref < 1e-10
+    bar_b, bar_l2 = _bars("%d-%s" % (method, dz))
+    print("bars %.1e / %.1e" % (bar_b, bar_l2))
+    assert e_or < bar_b and l2_or < bar_l2
+    assert e_ref < bar_b and l2_ref < bar_l2
 
 
 def test_batched_many_angles_same_as_single(engine):
@@ -98,7 +100,8 @@ def test_auto_periodic_renormalisation_n521(engine):
         engine.table_release(tid)
     ref = deckio.read_s(os.path.join(REF, "test521", "perf-5-0.47.txt"))
     l2 = np.linalg.norm(f2[0] - ref.f2[0]) / np.linalg.norm(ref.f2[0])
-    assert O.rel_err(f2[0], ref.f2[0]) < 1e-8 and l2 < 1e-10
+    bar_b, bar_l2 = _bars("5-0.47")
+    assert O.rel_err(f2[0], ref.f2[0]) < bar_b and l2 < bar_l2
 
 
 def _device_curve(eng, surf, method, period=0):
@@ -118,7 +121,7 @@ def _device_curve(eng, surf, method, period=0):
 @pytest.mark.parametrize("period", [0, -1])
 def test_leapfrog_n521(engine, dz, period):
     """Stormer-Verlet (int_method 4) on the batched path against the reference's perf-4
-    curves and the oracle, at the N = 521 FP64 bar (1e-8 per beam, 1e-10 normalised L2)."""
+    curves and the oracle, at the measured N = 521 FP64 bars."""
     print("case test521 leapfrog", dz, period, flush=True)
     surf = O.load_case(os.path.join(REF, "test521"), float(dz), 4)
     f2, st = _device_curve(engine, surf, 4, period)
@@ -128,8 +131,9 @@ def test_leapfrog_n521(engine, dz, period):
     for other in (ref.f2[0], of2[0]):
         e = O.rel_err(f2[0], other)
         l2 = np.linalg.norm(f2[0] - other) / np.linalg.norm(other)
-        print("renorms", st["n_renorm"], "rel %.3e L2 %.3e" % (e, l2))
-        assert e < 1e-8 and l2 < 1e-10
+        bar_b, bar_l2 = _bars("4-%s" % dz)
+        print("renorms", st["n_renorm"], "rel %.3e L2 %.3e (bars %.1e / %.1e)" % (e, l2, bar_b, bar_l2))
+        assert e < bar_b and l2 < bar_l2
 
 
 def _rk4_sensitivity(surf):
@@ -145,15 +149,13 @@ def _rk4_sensitivity(surf):
     return a, O.rel_err(b, a), np.linalg.norm(a - b) / np.linalg.norm(a)
 
 
-RK4_OFF = "RK4 above 48 beams is not offered: its batched stage kernel faulted the device without a cause shown"
-
-
-@pytest.mark.skip(reason=RK4_OFF)
 @pytest.mark.parametrize("dz", ["0.68", "0.47"])
 def test_rk4_n521(engine, dz):
-    """Classical RK4 (int_method 7) on the batched path.  Bar: within 10x of the oracle's
-    own one-ulp sensitivity (measured here: ~1e-3 per beam, ~2e-4 L2 at dz 0.68) against
-    both the oracle and the reference's perf-7 curve."""
+    """Classical RK4 (int_method 7) on the batched path at steps outside its stability region
+    for this deck's strongly evanescent beams.  Bar: within 10x of the oracle's own one-ulp
+    sensitivity (measured here: ~1e-3 per beam, ~2e-4 L2 at dz 0.68) against both the oracle
+    and the reference's perf-7 curve.  (The stable steps 0.047 and 0.015 are in
+    test_gpu_fine.py, at the reference's own noise level.)"""
     print("case test521 rk4", dz, flush=True)
     surf = O.load_case(os.path.join(REF, "test521"), float(dz), 7)
     f2, st = _device_curve(engine, surf, 7)
@@ -167,7 +169,7 @@ def test_rk4_n521(engine, dz):
     assert st["n_renorm"] == surf.ns
 
 
-@pytest.mark.parametrize("method", [4, pytest.param(7, marks=pytest.mark.skip(reason=RK4_OFF))])
+@pytest.mark.parametrize("method", [4, 7])
 def test_leapfrog_rk4_many_angles_same_as_single(engine, method):
     surf = O.load_case(os.path.join(REF, "test521"), 0.68, method)
     a, G, F, W = _inputs(surf)
@@ -194,17 +196,3 @@ def test_leapfrog_mixed_n521(engine, mixed_engine):
     e = O.rel_err(fmx[0], f64[0])
     print("leapfrog mixed vs FP64 rel %.3e" % e)
     assert e < 1e-6
-
-
-def test_rk4_large_n_is_refused(engine):
-    """RK4 above 48 beams fails cleanly at batch creation."""
-    from trhepd import lib
-    surf = O.load_case(os.path.join(REF, "test521"), 0.68, 7)
-    angs, G, F, W = _inputs(surf)
-    tid = engine.table_build(surf.pot, surf.dz, surf.ns, 7)
-    try:
-        with pytest.raises(lib.TrhepdError) as e:
-            engine.propagate_batch(tid, surf.iv, G, F, surf.nb0, W)
-        assert e.value.code == lib.E_ARG
-    finally:
-        engine.table_release(tid)

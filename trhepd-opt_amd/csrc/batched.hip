@@ -1695,6 +1695,8 @@ static int magnus_mats(int method)
 }
 // rk4_intgs: slice matrices, stage q, product, stage p, the two derivative sums, Cholesky factor
 constexpr int GS_MATS = 9;
+// RK4 on the stage GEMM: the product X_r, the stage operands Q_r / P_r, the derivative sums
+constexpr int RK4_MATS = 5;
 // methods whose products are all per-angle (FP64 in either precision, no autotuned tiles)
 static bool dense_method(int method) { return magnus_mats(method) || method == TRHEPD_RK4GS; }
 
@@ -1702,10 +1704,6 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
 {
     if (t.method < TRHEPD_M2 || t.method > TRHEPD_RK4GS)
         return fail(ctx, TRHEPD_E_ARG, "unknown int_method %d", t.method);
-    // RK4 above 48 beams is not offered: a batched RK4 stage kernel faulted the device on its
-    // first run without a cause shown, and was removed
-    if (t.method == TRHEPD_RK4)
-        return fail(ctx, TRHEPD_E_ARG, "int_method 7 runs on the device for N <= 48 only (N = %d)", b.nb);
     if (b.nb > 600) return fail(ctx, TRHEPD_E_ARG, "N = %d exceeds the device path limit of 600 beams", b.nb);
     Runner *R = new Runner();
     Geo &g = R->g;
@@ -1723,14 +1721,15 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
                  oco = take((size_t)g.A * 8), oxi = take((size_t)g.A * 4), ore = take((size_t)g.A * 4),
                  opv = take((size_t)g.A * g.NP * 4), oli = take((size_t)g.A * 4), oal = take((size_t)g.A * 4),
                  opm = take((size_t)2 * g.A * LU_PERM * 4);
-    const int nw = t.method == TRHEPD_RK4GS ? GS_MATS : magnus_mats(t.method);
+    const int nw = t.method == TRHEPD_RK4GS ? GS_MATS : t.method == TRHEPD_RK4 ? RK4_MATS : magnus_mats(t.method);
     const size_t omg = nw ? take(nw * mb) : 0;
     const bool gs = t.method == TRHEPD_RK4GS;
     const size_t ox = gs ? take((size_t)g.A * g.NP * sizeof(double2)) : 0,
                  oxp = gs ? take((size_t)g.A * (g.NP / 16) * g.NP * sizeof(double2)) : 0,
                  ops = gs ? take((size_t)g.A * sizeof(PwState)) : 0;
-    // the Magnus and rk4gs products are all per-angle: they run in FP64 in either precision
-    R->mixed = ctx->precision == TRHEPD_PREC_MIXED && !dense_method(t.method);
+    // the Magnus and rk4gs products are all per-angle, and RK4's stage operands are formed
+    // outside the stage kernel: they run in FP64 in either precision
+    R->mixed = ctx->precision == TRHEPD_PREC_MIXED && !dense_method(t.method) && t.method != TRHEPD_RK4;
     size_t oqd = 0, omd = 0, oeq = 0, oem = 0, orm = 0;
     if (R->mixed) {
         // digit buffers sized for the widest padding any candidate tile needs
@@ -2066,8 +2065,9 @@ int runner_begin(trhepd_ctx *ctx, Batch &b, const Table &t)
     if (rc) return rc;
     hipLaunchKernelGGL(k_inv_scale, eg, eb, 0, b.stream, g, (const double2 *)R.gc, R.P);
     hipLaunchKernelGGL(k_fill_i32, dim3((g.A + 255) / 256), dim3(256), 0, b.stream, R.xid, g.A, 1);
-    // leapfrog starts from Q = I + dz/2 P (surf_magnus.f90:63-67), the Magnus methods from Q = I
-    if (t.method == TRHEPD_LF || magnus_mats(t.method))
+    // leapfrog starts from Q = I + dz/2 P (surf_magnus.f90:63-67), the Magnus methods and RK4
+    // from Q = I (surf_rk.f90:46-47)
+    if (t.method == TRHEPD_LF || t.method == TRHEPD_RK4 || magnus_mats(t.method))
         qreset(b, R.all, g.A, t.method == TRHEPD_LF ? 0.5 * t.dz : 0.0);
     // the Magnus methods carry rp from slice to slice, starting at zero
     if (magnus_mats(t.method))
@@ -2150,6 +2150,55 @@ static void stage_launch(trhepd_ctx *ctx, Batch &b, const Table &t, int mode, co
     S.diagabs = R.diagabs;
     gemm_launch(ctx, b, S);
     if (mode != 2) std::swap(R.Q, R.Qn);
+}
+
+// rk4_int (surf_rk.f90:48-78) on the batched path.  Each stage product X_r = Q_r M(z_r) is the
+// stage GEMM's accumulate mode (dst += 1 (Q M0 + Q diag(d_a)), the launch the RKN end kick
+// uses) into a zeroed buffer; after a renormalisation Q = I and X_1 = M exactly, as the
+// reference's x_is_id branch sets it.  The combinations follow the reference's order:
+// t2 = t1 + c k_r with c = dz/2, dz/2, dz, then t1 += (dz/6) (((k1 + 2 k2) + 2 k3) + k4).
+static void rk4_product(trhepd_ctx *ctx, Batch &b, const VVi *col, const double2 *Qop, double2 *X)
+{
+    Runner &R = *b.run;
+    const Geo &g = R.g;
+    hipLaunchKernelGGL(k_zero2, dim3((unsigned)((g.mat * g.A + 255) / 256)), dim3(256), 0, b.stream, X,
+                       g.mat * g.A);
+    hipLaunchKernelGGL(k_gather, dim3((unsigned)((g.mat + 255) / 256)), dim3(256), 0, b.stream, g, b.gmap, col, R.M);
+    StageArgs S;
+    S.g = g;
+    S.Q = Qop;
+    S.Qn = X;   // not written in mode 2
+    S.P = X;
+    S.M0 = R.M;
+    S.g2c = R.g2c;
+    S.col = col;
+    S.alpha = 1.0;
+    S.beta = 0.0;
+    S.coef = nullptr;
+    S.mode = 2;
+    S.colsum = R.colsum;
+    S.diagabs = R.diagabs;
+    gemm_launch(ctx, b, S);
+}
+
+static void rk4_slice(trhepd_ctx *ctx, Batch &b, const Table &t, int l)
+{
+    Runner &R = *b.run;
+    double2 *X = R.W[0], *Qs = R.W[1], *Ps = R.W[2], *SQ = R.W[3], *SP = R.W[4];
+    const double h = 0.5 * t.dz, dz = t.dz;
+    // matrices at z = dz (l - 1), z + dz/2 and z + dz (rk4_int_init, surf_rk.f90:17-24)
+    const VVi *c0 = t.d + (size_t)(2 * (l - 1)) * t.nv, *c1 = c0 + t.nv, *c2 = c1 + t.nv;
+    rk4_product(ctx, b, c0, R.Q, X);   // k1 = [P, Q m0]
+    lins(b, {{SQ, {{1.0, R.P}}}, {SP, {{1.0, X}}}, {Qs, {{1.0, R.Q}, {h, R.P}}}, {Ps, {{1.0, R.P}, {h, X}}}});
+    rk4_product(ctx, b, c1, Qs, X);    // k2 = [P2, Q2 m1]
+    lins(b, {{SQ, {{1.0, SQ}, {2.0, Ps}}}, {SP, {{1.0, SP}, {2.0, X}}}, {Qs, {{1.0, R.Q}, {h, Ps}}},
+             {Ps, {{1.0, R.P}, {h, X}}}});
+    rk4_product(ctx, b, c1, Qs, X);    // k3 = [P3, Q3 m1]
+    lins(b, {{SQ, {{1.0, SQ}, {2.0, Ps}}}, {SP, {{1.0, SP}, {2.0, X}}}, {Qs, {{1.0, R.Q}, {dz, Ps}}},
+             {Ps, {{1.0, R.P}, {dz, X}}}});
+    rk4_product(ctx, b, c2, Qs, X);    // k4 = [P4, Q4 m2]
+    lins(b, {{SQ, {{1.0, SQ}, {1.0, Ps}}}, {SP, {{1.0, SP}, {1.0, X}}}, {R.Q, {{1.0, R.Q}, {dz / 6, SQ}}},
+             {R.P, {{1.0, R.P}, {dz / 6, SP}}}});
 }
 
 // Stage-product tile variants: FP64 tile widths 32 / 48 / 64 columns; mixed (column
@@ -2243,6 +2292,11 @@ int runner_slice_enqueue(trhepd_ctx *ctx, Batch &b, const Table &t, int l)
         }
         TRY_HIP(ctx, hipGetLastError());
         return TRHEPD_OK;
+    } else if (t.method == TRHEPD_RK4) {
+        rk4_slice(ctx, b, t, l);
+        const int nblk = g.NP / 16;
+        hipLaunchKernelGGL(k_gersh, dim3((unsigned)(((size_t)g.A * nblk * g.NP + 255) / 256)), dim3(256), 0,
+                           b.stream, g, (const double2 *)R.Q, R.colsum, R.diagabs);
     } else if (magnus_mats(t.method)) {
         magnus_slice(b, t, l);
         const int nblk = g.NP / 16;
@@ -2282,7 +2336,7 @@ int runner_slice_finish(trhepd_ctx *ctx, Batch &b, const Table &t)
     const Geo &g = R.g;
     // leapfrog and the Magnus methods restart from Q = I after a renormalisation (the RKN
     // path rebuilds Q in its first stage)
-    const bool reset = t.method == TRHEPD_LF || magnus_mats(t.method);
+    const bool reset = t.method == TRHEPD_LF || t.method == TRHEPD_RK4 || magnus_mats(t.method);
     const bool gs = t.method == TRHEPD_RK4GS;
     if (gs && R.forced >= 0) {
         if (R.forced) cqr(b, R.all, g.A, true);

@@ -167,7 +167,7 @@ int trhepd_serve_submit(trhepd_ctx *ctx, const trhepd_potential *pot, double dz,
     if (!pot || !iv || !gamma || !f || !wnsga2 || !request_id || nb < 1 || nang < 1 || ns < 1 || nb0 < 1 ||
         nb0 > nb || !(dz > 0.0))
         return fail(ctx, TRHEPD_E_ARG, "serve_submit: bad arguments");
-    if (method != TRHEPD_SP4 && method != TRHEPD_SP6 && method != TRHEPD_LF && !(method == TRHEPD_RK4 && nb <= 48))
+    if (method != TRHEPD_SP4 && method != TRHEPD_SP6 && method != TRHEPD_LF && method != TRHEPD_RK4)
         return fail(ctx, TRHEPD_E_ARG, "int_method %d is not available on the device path at N = %d", method, nb);
     Server &S = server(ctx);
     Request r;
