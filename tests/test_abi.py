@@ -80,3 +80,64 @@ def test_fortran_bindings_on_device():
     print(r.stdout)
     assert r.returncode == 0
     assert all(line.endswith("status 0") for line in r.stdout.splitlines() if "status" in line)
+
+
+PROP = os.path.join(REPO, "trhepd-opt_amd", "build", "fortran", "propagate_check")
+
+
+def _write_fortran_input(path, surf, method, G, F, W):
+    """the input file of propagate_check: header, descriptor arrays, the angle batch (the
+    byte layouts the C ABI takes, which are the Fortran driver's own arrays)"""
+    p = surf.pot
+    nang, nb = G.shape
+    hdr = np.array([p["nv"], p["nvb"], p["natms"], p["natmb"], p["nelm"], p["nabr"], p["nabi"], surf.ns, method,
+                    nb, nang, surf.nb0], dtype=np.int32)
+    with open(path, "wb") as fh:
+        fh.write(hdr.tobytes())
+        fh.write(np.array([p["cc"], p["smesh"], surf.dz], dtype=np.float64).tobytes())
+        fh.write(np.ascontiguousarray(p["ielm"], dtype=np.int32).tobytes())
+        for k in ("z", "dchg", "ghk", "ocr", "b", "asf"):
+            fh.write(np.ascontiguousarray(p[k], dtype=np.float64).tobytes())
+        if p["natms"]:
+            fh.write(np.ascontiguousarray(p["st0"], dtype=np.complex128).tobytes())
+        if p["natmb"]:
+            fh.write(np.ascontiguousarray(p["st1"], dtype=np.complex128).tobytes())
+            fh.write(np.ascontiguousarray(p["st2"], dtype=np.complex128).tobytes())
+        fh.write(np.ascontiguousarray(np.asarray(surf.iv, dtype=np.int32).T).tobytes())
+        fh.write(np.ascontiguousarray(G, dtype=np.complex128).tobytes())
+        fh.write(np.ascontiguousarray(np.asarray(F, dtype=np.complex128).transpose(0, 2, 1)).tobytes())
+        fh.write(np.ascontiguousarray(W, dtype=np.float64).tobytes())
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(PROP), reason="amdflang not available: Fortran bindings not built")
+def test_fortran_drives_table_build_and_propagate_batch(tmp_path, engine):
+    """The test23 deck (69 angles, sp4, dz 0.68) through trhepd_table_build and
+    trhepd_propagate_batch called from Fortran: the same bits as the same calls from C
+    (through the Python binding), and the reference's recorded curve to 1e-10."""
+    import subprocess
+    surf = O.load_case(os.path.join(REF, "test23"), 0.68, lib.SP4)
+    angs = surf.angles()
+    G = np.array([O.gamma(surf, ga, az) for (_, _, ga, az) in angs])
+    F = np.array([surf.scatter_f(r1, r2) for (r1, r2, _, _) in angs])
+    W = np.array([surf.wn * np.sin(ga) ** 2 for (_, _, ga, _) in angs])
+    fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    _write_fortran_input(fin, surf, lib.SP4, G, F, W)
+    r = subprocess.run([PROP, fin, fout], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    nang, nb = G.shape
+    raw = np.fromfile(fout, dtype=np.uint8)
+    nf = nb * nb * nang * 16
+    fo = np.frombuffer(raw[:nf].tobytes(), dtype=np.complex128).reshape(nang, nb, nb).transpose(0, 2, 1)
+    f2 = np.frombuffer(raw[nf:].tobytes(), dtype=np.float64).reshape(nang, nb)
+    tid = engine.table_build(surf.pot, surf.dz, surf.ns, lib.SP4)
+    try:
+        fc, f2c = engine.propagate_batch(tid, surf.iv, G, F, surf.nb0, W)
+    finally:
+        engine.table_release(tid)
+    assert np.array_equal(f2, f2c) and np.array_equal(fo, fc)
+    ref = deckio.read_s(os.path.join(REF, "test23", "perf-5-0.68.txt"))
+    e = max(O.rel_err(f2[p], ref.f2[p]) for p in range(nang))
+    print("Fortran-driven test23 sp4 dz 0.68 vs recorded %.3e" % e)
+    assert e < 1e-10
