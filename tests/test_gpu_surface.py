@@ -146,3 +146,34 @@ def test_periodic_renormalisation_matches_reference(engine, case, dz, period):
     print("periodic", period, "renorms", st["n_renorm"], "rel vs reference %.3e" % e_ref)
     assert st["n_renorm"] == len(angs) * (surf.ns // period + (1 if surf.ns % period else 0))
     assert e_ref < 1e-10
+
+
+# general positions, the special positions of the 17 plane groups (axes, diagonals, the
+# hexagonal mirror lines, 2-, 3-, 4- and 6-fold points, glide-related points), and points just
+# inside and just outside strfac's 2e-4 special-position tolerance
+_POSITIONS = [(0.123, 0.371), (0.0, 0.0), (0.5, 0.0), (0.0, 0.5), (0.5, 0.5), (0.3, 0.0), (0.0, 0.3),
+              (0.3, 0.5), (0.5, 0.3), (0.27, 0.27), (0.27, -0.27), (0.27, 0.54), (0.54, 0.27), (1 / 3, 2 / 3),
+              (2 / 3, 1 / 3), (0.25, 0.25), (0.25, 0.75), (0.75, 0.25), (0.25, 0.0), (0.0, 0.25),
+              (1e-5, 0.31), (3e-4, 0.31), (0.31, 0.31 + 1e-5), (0.31, 0.31 + 3e-4), (0.2, 0.4 + 1e-5)]
+
+
+@pytest.mark.parametrize("nsg", list(range(1, 18)))
+@pytest.mark.parametrize("cell", [(1, 0, 0, 1), (2, 1, 0, 1)])
+def test_plane_group_orbits_match_oracle(engine, nsg, cell):
+    """The device setup's orbit expansion (host) and phase sums (device) against the
+    oracle's strfac for every plane group, at general and special positions: a missing or
+    extra image, or a wrong tolerance branch, changes the structure factor at O(1)."""
+    ma, mb, na, nb = cell
+    ih, ik = np.meshgrid(np.arange(-3, 4), np.arange(-3, 4))
+    gh = (-2 * np.pi) * ih.reshape(-1).astype(float)
+    gk = (-2 * np.pi) * ik.reshape(-1).astype(float)
+    x = np.array([p[0] for p in _POSITIONS])
+    y = np.array([p[1] for p in _POSITIONS])
+    dx, dy = 0.11, -0.07
+    st = engine.scpot_stf(nsg, ma, mb, na, nb, gh, gk, x, y, dx, dy).reshape(len(x), len(gh))
+    L = O.lib()
+    for j in range(len(x)):
+        ref = np.zeros(len(gh), dtype=np.complex128)
+        L.orc_strfac(nsg, len(gh), ma, mb, na, nb, gh, gk, x[j], y[j], dx, dy, ref)
+        err = np.abs(st[j] - ref).max()
+        assert err < 1e-12, "group %d cell %s position %s: %.3e" % (nsg, cell, _POSITIONS[j], err)
