@@ -125,7 +125,7 @@ int trhepd_table_release(trhepd_ctx *ctx, int table_id);
  * Reflection matrices of the semi-infinite bulk for nang angles and ngr groups of coupled
  * beams.  table_id: the bulk potential at the ns slice midpoints of one unit layer (built
  * with int_method 1 or 4, whose schedule is the midpoints, from the three-unit descriptor of
- * bulkio.f90:189-196).  nbg: group sizes (each <= 64); iv: blkghk's in-group map
+ * bulkio.f90:189-196).  nbg: group sizes (each <= 600); iv: blkghk's in-group map
  * iv(nbgm, nb) (bulksub.f90:297-336); gamma: complex (nb, nang) for the beams in group
  * order; phase: complex (nb) column phases of the layer shift, or NULL when there is none.
  * Every group's unit-layer transfer matrix is the product of the slice matrices (computed
@@ -137,6 +137,11 @@ int trhepd_table_release(trhepd_ctx *ctx, int table_id);
 int trhepd_bulk_reflect(trhepd_ctx *ctx, int table_id, int ngr, const int *nbg, int nbgm, const int *iv,
                         int nang, const double *gamma, const double *phase, int ml, double epsb,
                         double *blocks, int *layers);
+/* Groups of up to n beams (default 64, at most 64) run one workgroup per (angle, group);
+ * larger groups (up to 600 beams) run batched over the angles on the slice loop's per-angle
+ * product, elementwise and LU kernels.  n = 0 sends every group to the batched path.  The two
+ * agree to rounding.  No counterpart in the reference (blkref has one code path). */
+int trhepd_set_bulk_batched_above(trhepd_ctx *ctx, int n);
 
 /* ---- slice loop (replaces the angle-loop body of srfref, surflf.f90:79-153) ---- */
 

@@ -32,13 +32,28 @@ def _job(engine, case):
     return bp, driver.BulkJob(engine, bp)
 
 
+# the batched path (groups above 64 beams) forced onto every group of the reference decks,
+# whose groups are small enough to be well conditioned: the same checks as the default path
+PATHS = [64, 0]
+
+
+def _engine_for(path):
+    from trhepd import lib
+    e = lib.Engine(0)
+    e.set_bulk_batched_above(path)
+    return e
+
+
+@pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("case,tol", [("ge001", 1e-10), ("test23", 1e-10), ("test47", 1e-8)])
-def test_bulk_matches_reference_file(engine, case, tol):
-    bp, job = _job(engine, case)
-    try:
-        blocks, layers = job.run()
-    finally:
-        job.release()
+def test_bulk_matches_reference_file(case, tol, path):
+    print("bulk", case, "one-workgroup path up to", path, "beams", flush=True)
+    with _engine_for(path) as engine:
+        bp, job = _job(engine, case)
+        try:
+            blocks, layers = job.run()
+        finally:
+            job.release()
     n1, n2 = bp.nrep
     ref = bp.domains[0].blocks
     worst = 0.0
@@ -69,12 +84,14 @@ def test_bulk_matches_oracle(engine, case):
     assert worst <= (1e-10 if case == "test23" else 1e-8)
 
 
-def test_bulk_521_well_conditioned_groups(engine):
-    bp, job = _job(engine, "test521")
-    try:
-        blocks, layers = job.run()
-    finally:
-        job.release()
+@pytest.mark.parametrize("path", PATHS)
+def test_bulk_521_well_conditioned_groups(path):
+    with _engine_for(path) as engine:
+        bp, job = _job(engine, "test521")
+        try:
+            blocks, layers = job.run()
+        finally:
+            job.release()
     orc = OB.Bulk(bp).blocks()
     ref = bp.domains[0].blocks[0][0]
     good = [g for g in range(len(ref)) if _err(orc[0][0][g], ref[g]) <= 2e-9]
@@ -156,3 +173,26 @@ def test_whole_program_on_device(engine):
     err = max(O.rel_err(f2[p], ref.f2[p]) for p in range(len(f2)))
     print("whole program test23: rel vs reference %.3e" % err)
     assert err < 1e-10
+
+
+def test_bulk_group_above_64_beams():
+    """A single group of 97 beams (a 1x1 synthetic cell: every beam is a bulk beam) on the
+    batched path: finite blocks, angles independent of each other, the deck's six layers.
+    (Its values cannot be pinned: with evanescent beams of Im gamma ~ 20-30 per A the unit
+    layer's transfer matrix reaches e^(Im gamma cc) ~ 1e25 and beyond, and blkref's recursion
+    loses every digit in FP64 -- two FP64 builds of it disagree at O(1) already at N = 23.)"""
+    from trhepd import lib, synth
+    bd = synth.bulk_deck(97, 0.5, 1.2 + 1e-6, 0.1)
+    bp = driver.bulk_header(bd)
+    with lib.Engine(0) as engine:
+        job = driver.BulkJob(engine, bp)
+        try:
+            full, layers = job.run()
+            sub, _ = job.run([1, 5])
+        finally:
+            job.release()
+    assert len(full) == 8 and len(full[0]) == 1 and full[0][0].shape == (97, 97)
+    assert all(np.isfinite(b[0]).all() for b in full)
+    for q, p in enumerate([1, 5]):
+        assert np.array_equal(sub[q][0], full[p][0])
+    assert layers.min() == layers.max() == bp.ml == 6

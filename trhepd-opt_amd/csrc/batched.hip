@@ -1650,6 +1650,175 @@ __global__ void k_gs_strans(Geo g, const double2 *__restrict__ gamma, const doub
     Pt[et] = make_double2(gx.x + y.y, gx.y - y.x);
 }
 
+// ---------------------------------------------------------------------------
+// Bulk reflection of a group of more than 64 beams (blkref, bulk.f90:51-124), batched over
+// angles on the kernels above: the slice matrices E = [[c(H), i s(H)], [i H s(H), c(H)]] as
+// in the one-workgroup bulk kernel (Horner series after scaling, squarings), their product,
+// the unit-layer matrix T = Cm P Cm^-1 with the column phases, and the layer recursion with a
+// right division solved by the batched LU on the transposed system.  2n x 2n matrices use a
+// second geometry of 2n rows (compact blocks: rows 0..n-1 on top, n..2n-1 below).
+
+// per slice: the largest angle-independent row sum of |H| (off-diagonal |v + vi| and
+// |conj(v - vi)|, plus |v + vi| of the (0,0) component on the diagonal); the host adds
+// max |gamma^2|.  One workgroup per slice.
+__global__ void k_bulkl_norm(int n, int boff, const int *__restrict__ ivg, int nbgm, const VVi *__restrict__ table,
+                             int nv, double *__restrict__ out)
+{
+    __shared__ double red[256];
+    const VVi *col = table + (size_t)blockIdx.x * nv;
+    const double d0 = hypot(col[0].v.x + col[0].vi.x, col[0].v.y + col[0].vi.y);
+    double mx = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double s = d0;
+        for (int j = 0; j < n; ++j) {
+            if (j == i) continue;
+            const int m = i > j ? ivg[(size_t)(boff + j) * nbgm + i] : ivg[(size_t)(boff + i) * nbgm + j];
+            const VVi t = col[m - 1];
+            s += i > j ? hypot(t.v.x + t.vi.x, t.v.y + t.vi.y) : hypot(t.v.x - t.vi.x, t.v.y - t.vi.y);
+        }
+        mx = fmax(mx, s);
+    }
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// H of one slice for every angle (trmatg.f90:20-29): v + vi + gamma^2 on the diagonal,
+// v + vi at iv(i, j) below it, conj(v - vi) at iv(j, i) above it; padding zero
+__global__ void k_bulkl_h(Geo g, int boff, const int *__restrict__ ivg, int nbgm, const VVi *__restrict__ col,
+                          const double2 *__restrict__ gamma, int nb, double2 *__restrict__ H)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.mat * g.A) return;
+    const int a = (int)(e / g.mat);
+    const int r = (int)(e % g.mat), i = r % g.NP, j = r / g.NP;
+    double2 h = make_double2(0, 0);
+    if (i < g.N && j < g.N) {
+        if (i == j) {
+            const double2 gi = gamma[(size_t)a * nb + boff + i];
+            const double2 gg = cmul(gi, gi);
+            h = make_double2(col[0].v.x + col[0].vi.x + gg.x, col[0].v.y + col[0].vi.y + gg.y);
+        } else if (i > j) {
+            const VVi t = col[ivg[(size_t)(boff + j) * nbgm + i] - 1];
+            h = make_double2(t.v.x + t.vi.x, t.v.y + t.vi.y);
+        } else {
+            const VVi t = col[ivg[(size_t)(boff + i) * nbgm + j] - 1];
+            h = make_double2(t.v.x - t.vi.x, -(t.v.y - t.vi.y));
+        }
+    }
+    H[e] = h;
+}
+
+// E = [[c, i s], [i HS, c]] (2n-row geometry g2) from c, s, HS (n-row geometry g)
+__global__ void k_bulkl_e(Geo g, Geo g2, const double2 *__restrict__ Cc, const double2 *__restrict__ Ss,
+                          const double2 *__restrict__ HS, double2 *__restrict__ E)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g2.mat * g2.A) return;
+    const int a = (int)(e / g2.mat);
+    const int r = (int)(e % g2.mat), i = r % g2.NP, j = r / g2.NP;
+    const int n = g.N;
+    double2 v = make_double2(0, 0);
+    if (i < 2 * n && j < 2 * n) {
+        const int bi = i >= n, bj = j >= n, ii = i - bi * n, jj = j - bj * n;
+        const size_t q = (size_t)a * g.mat + (size_t)jj * g.NP + ii;
+        if (bi == bj) v = Cc[q];
+        else if (!bi) v = make_double2(-Ss[q].y, Ss[q].x);
+        else v = make_double2(-HS[q].y, HS[q].x);
+    }
+    E[e] = v;
+}
+
+// unit-layer matrix T = Cm P Cm^-1, Cm = [[Gamma, I], [Gamma, -I]], then the column phases of
+// the layer shift (bulk.f90:83-93); written as its four n x n blocks (geometry g)
+__global__ void k_bulkl_t(Geo g, Geo g2, const double2 *__restrict__ P, const double2 *__restrict__ gamma, int nb,
+                          int boff, const double2 *__restrict__ phase, double2 *__restrict__ T11,
+                          double2 *__restrict__ T12, double2 *__restrict__ T21, double2 *__restrict__ T22)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.mat * g.A) return;
+    const int a = (int)(e / g.mat);
+    const int r = (int)(e % g.mat), ii = r % g.NP, jj = r / g.NP;
+    const int n = g.N;
+    double2 v[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+    if (ii < n && jj < n) {
+        const double2 *Pa = P + (size_t)a * g2.mat;
+        const double2 *ga = gamma + (size_t)a * nb + boff;
+        const double2 ph = phase ? phase[boff + jj] : make_double2(1, 0);
+        for (int bi = 0; bi < 2; ++bi)
+            for (int bj = 0; bj < 2; ++bj) {
+                // rows of Cm P: top = Gamma P1. + P2., bottom = Gamma P1. - P2.
+                auto cp = [&](int col) {
+                    const double2 p1 = Pa[(size_t)col * g2.NP + ii], p2 = Pa[(size_t)col * g2.NP + n + ii];
+                    const double2 gp = cmul(ga[ii], p1);
+                    return bi ? make_double2(gp.x - p2.x, gp.y - p2.y) : make_double2(gp.x + p2.x, gp.y + p2.y);
+                };
+                // Cm^-1 = 1/2 [[Gamma^-1, Gamma^-1], [I, -I]]
+                const double2 x = cdiv(cp(jj), ga[jj]), y = cp(n + jj);
+                double2 t = bj ? make_double2(0.5 * (x.x - y.x), 0.5 * (x.y - y.y))
+                               : make_double2(0.5 * (x.x + y.x), 0.5 * (x.y + y.y));
+                if (phase) t = cmul(t, ph);
+                v[bi * 2 + bj] = t;
+            }
+    }
+    T11[e] = v[0];
+    T12[e] = v[1];
+    T21[e] = v[2];
+    T22[e] = v[3];
+}
+
+// dst(i, j) = src(j, i) per angle (padding zero)
+__global__ void k_transpose(Geo g, const double2 *__restrict__ src, double2 *__restrict__ dst)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.mat * g.A) return;
+    const int a = (int)(e / g.mat);
+    const int r = (int)(e % g.mat), i = r % g.NP, j = r / g.NP;
+    dst[e] = (i < g.N && j < g.N) ? src[(size_t)a * g.mat + (size_t)i * g.NP + j] : make_double2(0, 0);
+}
+
+// blkref's judgement (bulk.f90:104-117) after layer l, one workgroup per angle: |R_ii|^2
+// is kept from layer 20 on; from layer 21 an angle whose largest change is below epsb is
+// finished: its R goes to the output (n x n, column-major) and its layer count is set.
+// At the last layer (final != 0) every unfinished angle is written out with layer l.
+__global__ void k_bulkl_judge(Geo g, const double2 *__restrict__ R, double *__restrict__ r20, int l, double epsb,
+                              int final, int *__restrict__ done, int *__restrict__ layers, double2 *__restrict__ out,
+                              size_t out_stride)
+{
+    __shared__ double red[256];
+    __shared__ int fin;
+    const int a = blockIdx.x, n = g.N;
+    if (done[a]) return;
+    const double2 *Ra = R + (size_t)a * g.mat;
+    double mx = 0.0;
+    if (l >= 20)
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            const double2 d = Ra[(size_t)i * g.NP + i];
+            const double r2 = d.x * d.x + d.y * d.y;
+            if (l > 20) mx = fmax(mx, fabs(r2 - r20[(size_t)a * g.NP + i]));
+            r20[(size_t)a * g.NP + i] = r2;
+        }
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) fin = (l > 20 && red[0] < epsb) || final;
+    __syncthreads();
+    if (!fin) return;
+    double2 *o = out + (size_t)a * out_stride;
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) o[e] = Ra[(size_t)(e / n) * g.NP + (e % n)];
+    if (threadIdx.x == 0) {
+        done[a] = 1;
+        layers[a] = l;
+    }
+}
+
 }  // namespace
 
 struct Runner {
@@ -2403,6 +2572,158 @@ int runner_end(trhepd_ctx *ctx, Batch &b, const Table &t)
     hipLaunchKernelGGL(k_out, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, b.stream, g, (const double2 *)R.P,
                        (const double2 *)R.gc, b.fout, b.has_f2 ? b.f2 : nullptr, b.nb0, (const double *)b.w, 1);
     TRY_HIP(ctx, hipGetLastError());
+    return TRHEPD_OK;
+}
+
+// Bulk reflection of one group of n > 64 beams for all nang angles (blkref, bulk.f90:51-124),
+// batched over the angles.  Device inputs as trhepd_bulk_reflect holds them: ivg (nbgm, nb),
+// gamma [nang][nb], phase [nb] or null; gmax2 bounds |gamma^2| over the group's beams and the
+// angles.  Angle a's n x n block goes to out + a * out_stride; layers[a] the layers it used.
+int bulk_large_group(trhepd_ctx *ctx, hipStream_t st, const Table &t, int n, int boff, int nang, const int *ivg,
+                     int nbgm, const double2 *gamma, int nb, const double2 *phase, double gmax2, int ml, double epsb,
+                     double2 *out, size_t out_stride, int *layers)
+{
+    if (n > 600) return fail(ctx, TRHEPD_E_ARG, "bulk group of %d beams exceeds the device limit of 600", n);
+    Geo g{nang, n, (n + 15) / 16 * 16, 0};
+    g.mat = (size_t)g.NP * g.NP;
+    Geo g2{nang, 2 * n, (2 * n + 15) / 16 * 16, 0};
+    g2.mat = (size_t)g2.NP * g2.NP;
+    const size_t mb = g.mat * g.A * sizeof(double2), mb2 = g2.mat * g2.A * sizeof(double2);
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~size_t(255); return o; };
+    size_t om[11];
+    for (size_t &o : om) o = take(mb);
+    const size_t oe = take(mb2), op = take(mb2), opn = take(mb2), opiv = take((size_t)g.A * g.NP * 4),
+                 odi = take((size_t)g.A * (g.NP / 16) * 256 * sizeof(double2)),
+                 opm = take((size_t)2 * g.A * LU_PERM * 4), oflag = take(64), oall = take((size_t)g.A * 4),
+                 or20 = take((size_t)g.A * g.NP * 8), odone = take((size_t)g.A * 4), oly = take((size_t)g.A * 4),
+                 onorm = take((size_t)t.ns * 8);
+    char *d = nullptr;
+    int rc = dev_alloc(ctx, (void **)&d, off);
+    if (rc) return rc;
+    double2 *M[11];
+    for (int k = 0; k < 11; ++k) M[k] = (double2 *)(d + om[k]);
+    double2 *H = M[0], *X = M[1], *Cc = M[2], *Ss = M[3], *Tm = M[4], *HS = M[5];
+    double2 *R = M[6], *t1 = M[7], *t2 = M[8], *Qt = M[9], *Pt = M[10];
+    double2 *E = (double2 *)(d + oe), *P = (double2 *)(d + op), *Pn = (double2 *)(d + opn);
+    int *flag = (int *)(d + oflag), *all = (int *)(d + oall), *done = (int *)(d + odone), *ly = (int *)(d + oly);
+    double *r20 = (double *)(d + or20), *norm = (double *)(d + onorm);
+    auto grid = [](size_t n_) { return dim3((unsigned)((n_ + 255) / 256)); };
+    auto lin = [&](double2 *o, std::initializer_list<std::pair<double, const double2 *>> terms, double ident) {
+        LinArgs L{};
+        L.g = g;
+        for (const auto &q : terms) {
+            L.c[L.n] = q.first;
+            L.x[L.n] = q.second;
+            ++L.n;
+        }
+        L.out = o;
+        L.ident = ident;
+        hipLaunchKernelGGL(k_lin, grid(g.mat * g.A), dim3(256), 0, st, L);
+    };
+    auto prod = [&](const Geo &gg, const double2 *A, const double2 *B, double2 *C, double beta) {
+        const int nbl = gg.NP / 16;
+        const BmmArgs S{gg, A, B, C, 1.0, beta, 0.0, nullptr, 0};
+        hipLaunchKernelGGL(k_bmm, dim3((unsigned)((nbl * nbl + 3) / 4), gg.A), dim3(256), 0, st, S);
+    };
+    std::vector<int> seq(g.A);
+    for (int a = 0; a < g.A; ++a) seq[a] = a;
+    std::vector<double> hn(t.ns);
+    hipError_t e = hipMemsetAsync(d, 0, off, st);
+    if (!e) e = hipMemcpyAsync(all, seq.data(), (size_t)g.A * 4, hipMemcpyHostToDevice, st);
+    if (!e) {
+        hipLaunchKernelGGL(k_bulkl_norm, dim3(t.ns), dim3(256), 0, st, n, boff, ivg, nbgm, (const VVi *)t.d, t.nv, norm);
+        e = hipMemcpyAsync(hn.data(), norm, (size_t)t.ns * 8, hipMemcpyDeviceToHost, st);
+    }
+    if (!e) e = hipStreamSynchronize(st);
+    if (e) {
+        dev_free(ctx, d, off);
+        return hip_check(ctx, e, "bulk (large group) setup");
+    }
+    // P = I (2n x 2n)
+    {
+        LinArgs L{};
+        L.g = g2;
+        L.out = P;
+        L.ident = 1.0;
+        hipLaunchKernelGGL(k_lin, grid(g2.mat * g2.A), dim3(256), 0, st, L);
+    }
+    const double dz = t.dz;
+    for (int l = 0; l < t.ns; ++l) {
+        // scaling h = dz / 2^sq with |h^2 H| <= 1 and the series degree, as the one-workgroup kernel
+        const double xn = (hn[l] + gmax2) * dz * dz;
+        int sq = 0;
+        while (xn / (double)(1ull << (2 * sq)) > 1.0 && sq < 30) ++sq;
+        const double h = dz / (double)(1ull << sq), h2 = h * h;
+        const double x = xn / (double)(1ull << (2 * sq));
+        int K = 1;
+        for (double term = x / 2.0; term > 1e-18 && K < 20;) {
+            ++K;
+            term *= x / ((2.0 * K - 1) * (2.0 * K));
+        }
+        hipLaunchKernelGGL(k_bulkl_h, grid(g.mat * g.A), dim3(256), 0, st, g, boff, ivg, nbgm,
+                           (const VVi *)(t.d + (size_t)l * t.nv), gamma, nb, H);
+        lin(X, {{h2, H}}, 0.0);
+        lin(Cc, {}, 1.0);
+        lin(Ss, {}, 1.0);
+        for (int k = K; k >= 1; --k) {   // Horner: c = I - X/((2k-1)2k) (...), s = I - X/(2k(2k+1)) (...)
+            prod(g, X, Cc, Tm, 0.0);
+            lin(Cc, {{-1.0 / ((2.0 * k - 1) * (2.0 * k)), Tm}}, 1.0);
+            prod(g, X, Ss, Tm, 0.0);
+            lin(Ss, {{-1.0 / ((2.0 * k) * (2.0 * k + 1)), Tm}}, 1.0);
+        }
+        lin(Ss, {{h, Ss}}, 0.0);
+        prod(g, H, Ss, HS, 0.0);
+        hipLaunchKernelGGL(k_bulkl_e, grid(g2.mat * g2.A), dim3(256), 0, st, g, g2, (const double2 *)Cc,
+                           (const double2 *)Ss, (const double2 *)HS, E);
+        for (int r = 0; r < sq; ++r) {   // E <- E^2
+            prod(g2, E, E, Pn, 0.0);
+            std::swap(E, Pn);
+        }
+        prod(g2, E, P, Pn, 0.0);   // the running product T_l ... T_1
+        std::swap(P, Pn);
+    }
+    double2 *T11 = H, *T12 = X, *T21 = Cc, *T22 = Ss;
+    hipLaunchKernelGGL(k_bulkl_t, grid(g.mat * g.A), dim3(256), 0, st, g, g2, (const double2 *)P, gamma, nb, boff,
+                       phase, T11, T12, T21, T22);
+    LuArgs L{g, Qt, Pt, (int *)(d + opiv), all, flag, (double2 *)(d + odi), (int *)(d + opm)};
+    for (int l = 1; l <= ml && !rc; ++l) {
+        // R = (T11 R + T12)(T21 R + T22)^-1, the right division as (t2^T)^-1 t1^T = R^T
+        lin(t1, {{1.0, T12}}, 0.0);
+        lin(t2, {{1.0, T22}}, 0.0);
+        if (l > 1) {
+            prod(g, T11, R, t1, 1.0);
+            prod(g, T21, R, t2, 1.0);
+        }
+        hipLaunchKernelGGL(k_transpose, grid(g.mat * g.A), dim3(256), 0, st, g, (const double2 *)t2, Qt);
+        hipLaunchKernelGGL(k_transpose, grid(g.mat * g.A), dim3(256), 0, st, g, (const double2 *)t1, Pt);
+        rc = lu_on(ctx, st, L, g.A);
+        if (rc) break;
+        hipLaunchKernelGGL(k_transpose, grid(g.mat * g.A), dim3(256), 0, st, g, (const double2 *)Pt, R);
+        hipLaunchKernelGGL(k_bulkl_judge, dim3(g.A), dim3(256), 0, st, g, (const double2 *)R, r20, l, epsb,
+                           l == ml ? 1 : 0, done, ly, out, out_stride);
+        if (l > 20 && l < ml) {   // stop once every angle has settled
+            std::vector<int> hd(g.A);
+            e = hipMemcpyAsync(hd.data(), done, (size_t)g.A * 4, hipMemcpyDeviceToHost, st);
+            if (!e) e = hipStreamSynchronize(st);
+            if (e) {
+                rc = hip_check(ctx, e, "bulk (large group) judgement");
+                break;
+            }
+            bool any = false;
+            for (int v : hd) any |= v == 0;
+            if (!any) break;
+        }
+    }
+    int fl[16] = {};
+    if (!rc) e = hipGetLastError();
+    if (!rc && !e) e = hipMemcpyAsync(layers, ly, (size_t)g.A * 4, hipMemcpyDeviceToHost, st);
+    if (!rc && !e) e = hipMemcpyAsync(fl, flag, 64, hipMemcpyDeviceToHost, st);
+    const hipError_t e2 = hipStreamSynchronize(st);
+    dev_free(ctx, d, off);
+    if (rc) return rc;
+    if (e || e2) return hip_check(ctx, e ? e : e2, "bulk (large group)");
+    if (fl[0]) return fail(ctx, TRHEPD_E_SINGULAR, "exact zero pivot in a bulk layer solve");
     return TRHEPD_OK;
 }
 

@@ -388,6 +388,14 @@ __global__ __launch_bounds__(256) void k_bulk(BulkArgs A)
 
 using namespace trhepd;
 
+extern "C" int trhepd_set_bulk_batched_above(trhepd_ctx *ctx, int n)
+{
+    if (!ctx) return TRHEPD_E_ARG;
+    if (n < 0 || n > 64) return fail(ctx, TRHEPD_E_ARG, "bulk group limit of the one-workgroup kernel is 0..64 (got %d)", n);
+    ctx->bulk_one_wg_max = n;
+    return TRHEPD_OK;
+}
+
 extern "C" int trhepd_bulk_reflect(trhepd_ctx *ctx, int table_id, int ngr, const int *nbg, int nbgm, const int *iv,
                                    int nang, const double *gamma, const double *phase, int ml, double epsb,
                                    double *blocks, int *layers)
@@ -401,8 +409,8 @@ extern "C" int trhepd_bulk_reflect(trhepd_ctx *ctx, int table_id, int ngr, const
         return fail(ctx, TRHEPD_E_ARG, "bulk_reflect: bad arguments");
     int nb = 0;
     for (int g = 0; g < ngr; ++g) {
-        if (nbg[g] < 1 || nbg[g] > nbgm || nbg[g] > 64)
-            return fail(ctx, TRHEPD_E_ARG, "group %d has %d beams (1..%d, at most 64)", g + 1, nbg[g], nbgm);
+        if (nbg[g] < 1 || nbg[g] > nbgm || nbg[g] > 600)
+            return fail(ctx, TRHEPD_E_ARG, "group %d has %d beams (1..%d, at most 600)", g + 1, nbg[g], nbgm);
         nb += nbg[g];
     }
     // iv entries used: strict lower triangle of each group, 1..nv
@@ -418,16 +426,27 @@ extern "C" int trhepd_bulk_reflect(trhepd_ctx *ctx, int table_id, int ngr, const
         }
     }
     TRY_HIP(ctx, hipSetDevice(ctx->device));
+    // groups of up to 64 beams: one workgroup per (angle, group) item; larger groups: the
+    // batched path over angles (bulk_large_group), one group after another
+    const int ONE_WG_MAX = ctx->bulk_one_wg_max;
     std::vector<BulkItem> items;
-    size_t ws = 0, out = 0;
+    std::vector<int> big;           // groups of the batched path
+    std::vector<size_t> goff(ngr);  // offset of a group's block within one angle's output
+    size_t per_angle = 0;
+    for (int g = 0; g < ngr; ++g) {
+        goff[g] = per_angle;
+        per_angle += (size_t)nbg[g] * nbg[g];
+        if (nbg[g] > ONE_WG_MAX) big.push_back(g);
+    }
+    size_t ws = 0, out = per_angle * nang;
     for (int a = 0; a < nang; ++a) {
         int b0 = 0;
         for (int g = 0; g < ngr; ++g) {
             const int n = nbg[g], NP = (n + 15) & ~15;
-            BulkItem it{a, g, n, b0, ws, out};
-            ws += (size_t)6 * NP * NP + (size_t)3 * 4 * NP * NP;
-            out += (size_t)n * n;
-            items.push_back(it);
+            if (n <= ONE_WG_MAX) {
+                items.push_back(BulkItem{a, g, n, b0, ws, (size_t)a * per_angle + goff[g]});
+                ws += (size_t)6 * NP * NP + (size_t)3 * 4 * NP * NP;
+            }
             b0 += n;
         }
     }
@@ -464,21 +483,39 @@ extern "C" int trhepd_bulk_reflect(trhepd_ctx *ctx, int table_id, int ngr, const
     if (!e) e = hipMemcpyAsync(d + o_iv, iv, (size_t)nbgm * nb * 4, hipMemcpyHostToDevice, ctx->stream);
     if (!e) e = hipMemcpyAsync(d + o_g, gamma, (size_t)nang * nb * 16, hipMemcpyHostToDevice, ctx->stream);
     if (!e && phase) e = hipMemcpyAsync(d + o_ph, phase, (size_t)nb * 16, hipMemcpyHostToDevice, ctx->stream);
-    if (!e) {
+    if (!e && n_it) {
         hipLaunchKernelGGL(k_bulk, dim3((unsigned)n_it), dim3(256), 0, ctx->stream, A);
         e = hipGetLastError();
     }
-    if (!e) e = hipMemcpyAsync(blocks, d + o_out, out * 16, hipMemcpyDeviceToHost, ctx->stream);
-    if (!e) e = hipMemcpyAsync(ly.data(), d + o_ly, n_it * 4, hipMemcpyDeviceToHost, ctx->stream);
+    std::vector<int> lyb((size_t)nang * big.size());
+    for (size_t k = 0; k < big.size() && !e && !rc; ++k) {
+        const int g = big[k];
+        int b0 = 0;
+        double gmax2 = 0.0;
+        for (int q = 0; q < g; ++q) b0 += nbg[q];
+        for (int a = 0; a < nang; ++a)
+            for (int i = 0; i < nbg[g]; ++i) {
+                const double gr = gamma[2 * ((size_t)a * nb + b0 + i)], gi = gamma[2 * ((size_t)a * nb + b0 + i) + 1];
+                gmax2 = std::max(gmax2, gr * gr + gi * gi);
+            }
+        rc = bulk_large_group(ctx, ctx->stream, *t, nbg[g], b0, nang, A.ivg, nbgm, A.gamma, nb, A.phase, gmax2, ml,
+                              epsb, A.out + goff[g], per_angle, lyb.data() + k * nang);
+    }
+    if (!e && !rc) e = hipMemcpyAsync(blocks, d + o_out, out * 16, hipMemcpyDeviceToHost, ctx->stream);
+    if (!e && !rc && n_it) e = hipMemcpyAsync(ly.data(), d + o_ly, n_it * 4, hipMemcpyDeviceToHost, ctx->stream);
     if (!e) e = hipStreamSynchronize(ctx->stream);
     dev_free(ctx, d, off);
+    if (rc) return rc;
     if (e) return hip_check(ctx, e, "bulk_reflect");
-    if (layers)
+    if (layers) {
         for (size_t i = 0; i < n_it; ++i) {
             // items were reordered: report in (angle, group) order
             const BulkItem &it = order[i];
             layers[(size_t)it.ang * ngr + it.grp] = ly[i];
         }
+        for (size_t k = 0; k < big.size(); ++k)
+            for (int a = 0; a < nang; ++a) layers[(size_t)a * ngr + big[k]] = lyb[k * nang + a];
+    }
     for (size_t q = 0; q < 2 * out; ++q)
         if (!std::isfinite(blocks[q])) return fail(ctx, TRHEPD_E_NONFINITE, "non-finite bulk reflection");
     return TRHEPD_OK;

@@ -109,6 +109,7 @@ struct trhepd_ctx {
     int next_batch = 1;
     int renorm_period = 0;   // 0: reference rule (cond_est > 1e3); K > 0: every K slices
     int mixed_digits = 7;    // digits per operand of the sliced-integer stage products
+    int bulk_one_wg_max = 64;   // bulk groups of more beams run on the batched path
     std::vector<trhepd::Batch> batches;
     trhepd::Prof prof;
     // autotuned stage-product tiles: (precision, N, angles, digits) -> variant index
@@ -233,6 +234,11 @@ struct SmallBatch {
 };
 bool small_fusable(int nb, int method);
 int run_small_multi(trhepd_ctx *ctx, hipStream_t stream, const std::vector<SmallBatch> &bs);
+// blkref for one group of more than 64 beams, batched over angles (device pointers; see
+// trhepd_bulk_reflect); out + a * out_stride receives angle a's n x n block
+int bulk_large_group(trhepd_ctx *ctx, hipStream_t st, const Table &t, int n, int boff, int nang, const int *ivg,
+                     int nbgm, const double2 *gamma, int nb, const double2 *phase, double gmax2, int ml, double epsb,
+                     double2 *out, size_t out_stride, int *layers);
 // batched large-N path, split into phases so that several batches interleave:
 // runner_create allocates the batch's working memory; begin enqueues the boundary
 // transform; slice_enqueue(l) the stages of slice l up to the end kick (the renormalisation

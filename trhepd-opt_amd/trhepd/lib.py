@@ -29,7 +29,7 @@ SYMBOLS = ("trhepd_ctx_create", "trhepd_ctx_destroy", "trhepd_last_error", "trhe
            "trhepd_batch_release", "trhepd_last_stats", "trhepd_set_renorm_period", "trhepd_set_mixed_digits", "trhepd_profile",
            "trhepd_kernel_stats", "trhepd_serve_submit", "trhepd_serve_step", "trhepd_serve_result",
            "trhepd_graph_create", "trhepd_graph_launch", "trhepd_graph_release", "trhepd_bulk_reflect",
-           "trhepd_propagate_batch_noalloc")
+           "trhepd_propagate_batch_noalloc", "trhepd_set_bulk_batched_above")
 
 
 class TrhepdError(RuntimeError):
@@ -89,6 +89,7 @@ def load() -> C.CDLL:
     L.trhepd_profile.argtypes = [vp, i]
     L.trhepd_set_renorm_period.argtypes = [vp, i]
     L.trhepd_set_mixed_digits.argtypes = [vp, i]
+    L.trhepd_set_bulk_batched_above.argtypes = [vp, i]
     L.trhepd_kernel_stats.argtypes = [vp, C.POINTER(C.c_longlong), dp]
     L.trhepd_serve_submit.argtypes = [vp, C.POINTER(Potential), d, i, i, i, i, vp, vp, vp, i, vp, ip]
     L.trhepd_serve_step.argtypes = [vp, ip, ip, ip]
@@ -385,6 +386,10 @@ class Engine:
     def set_mixed_digits(self, digits: int):
         """Digits per operand of the mixed-precision stage products (5, 6 or 7)."""
         self._chk(self._L.trhepd_set_mixed_digits(self._h, int(digits)))
+
+    def set_bulk_batched_above(self, n: int):
+        """Bulk groups of more than n beams run batched over angles (default 64; 0: all)."""
+        self._chk(self._L.trhepd_set_bulk_batched_above(self._h, int(n)))
 
     def profile(self, enable: bool):
         self._chk(self._L.trhepd_profile(self._h, 1 if enable else 0))

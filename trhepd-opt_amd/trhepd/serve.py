@@ -38,9 +38,10 @@ class Request:
 
 
 def structure_request(eng: "lib.Engine", st: "synth.Structure", dz2: float = 0.02, method: int = lib.SP4,
-                      bulk_layers: int = 2, key: str = "") -> Request:
+                      bulk_layers: Optional[int] = None, key: str = "") -> Request:
     """The request of a synthetic structure: potential descriptor (set up on the device),
-    bulk reflection of `bulk_layers` bulk layers, gamma and intensity weights."""
+    bulk reflection of `bulk_layers` bulk layers (the deck's ML by default), gamma and
+    intensity weights."""
     blocks = synth.bulk_reflection(eng, st, layers=bulk_layers, dz2=dz2, method=method)
     job = driver.SurfaceJob(eng, st.bp, st.surf, dz2, method, build_table=False)
     G, F, W, _ = job.inputs(0, blocks)

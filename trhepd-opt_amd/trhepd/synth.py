@@ -10,9 +10,15 @@ in the cell and heights uniform in 1.0-2.5 A above the top bulk atom; the surfac
 
 The decks are written in the reference's own formats (bulk.txt / surf.txt) and the
 bulkP.b header comes from ``driver.bulk_header``.  The bulk reflection matrices the
-surface loop starts from are computed on the device by integrating a slab of bulk unit
-layers (``bulk_slab_deck``) from F = 0; this replaces the reference's transfer-matrix
-bulk stage for the synthetic decks only.
+surface loop starts from are those of the deck's ML unit layers (6) on a non-reflecting
+substrate -- what blkref's recursion computes from R = 0 -- obtained on the device by
+integrating that slab of bulk unit layers (``bulk_slab_deck``) from F = 0 with the
+renormalised ratio-form propagator.  blkref's own route (the transfer matrix of a unit
+layer, trmat / bulk.f90:69-93) is ill conditioned on these 1x1 decks: every beam is a bulk
+beam, the strongly evanescent ones (|Im gamma| up to ~19 per A at N = 23, more at larger N)
+make the unit layer's transfer matrix grow like exp(|Im gamma| cc) ~ 1e25, and two FP64
+builds of blkref (the reference's bulk program and the numpy restatement) return reflection
+matrices that differ at O(1) and rocking curves that differ by O(10) on the N = 23 deck.
 """
 from __future__ import annotations
 
@@ -118,11 +124,12 @@ def config(cfg: int, seed: int = 1234, index: int = 0) -> Structure:
     return Structure(bd, sd, driver.bulk_header(bd))
 
 
-def bulk_reflection(eng: "lib.Engine", st: Structure, layers: int = 2, dz2: float = 0.02,
+def bulk_reflection(eng: "lib.Engine", st: Structure, layers: Optional[int] = None, dz2: float = 0.02,
                     method: int = lib.SP4):
     """Bulk reflection blocks [irep1][irep2][group] for a synthetic structure: the reflection
-    of `layers` bulk unit layers integrated from F = 0 on the device."""
-    slab = bulk_slab_deck(st.bulk, layers)
+    of `layers` bulk unit layers (the deck's ML by default) integrated from F = 0 on the
+    device."""
+    slab = bulk_slab_deck(st.bulk, st.bulk.ml if layers is None else layers)
     job = driver.SurfaceJob(eng, st.bp, slab, dz2, method)
     try:
         blocks = []
