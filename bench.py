@@ -10,10 +10,12 @@ timed the same way.
 Also reported: the dominant kernel's roofline (the int8 digit-sliced stage product,
 3 ND (ND + 1) N^3 int8 ops per angle against the 5.0 POP/s dense int8 peak; the FP64 stage
 product, 8 N^3 flops per angle against 78.6 TFLOP/s, for the FP64 path) from HIP events
-around every launch on the batch streams during the timed steps; the same kernels measured
-in one untimed curve run as a single batch (no concurrent stream sharing the device); the
-uneven-size batch in one captured graph; the serving sample; and the CPU oracle timed on a
-bounded sample of the same workload on this host.
+around every launch of that kernel on its stream, in one untimed curve run as a single batch
+(no concurrent stream shares the device, so the bracketed time is the kernel's own), with
+the concurrent-batch figures of the timed steps beside it; the small-beam configurations C1
+and C2 (C2's 896 solves in one fused launch); the uneven-size batch in one captured graph;
+the serving sample; and the reference's own CPU program (or, where it is not built, the C
+oracle) timed on a bounded sample of the same workload on this host.
 """
 import argparse
 import json
@@ -137,6 +139,56 @@ def run_path(eng, st, blocks, args, method, dz2):
     return out
 
 
+def run_small_configs(eng, args):
+    """The small-beam configurations: C1 (one N=23 structure, 56 angles) and C2 (16 N=47
+    structures x 56 angles, run as one fused launch of the small-N kernel), sp4 at dz 0.02,
+    FP64, inputs resident.  A step is one batch_run_many over all of a configuration's
+    batches; points/s counts its (structure, angle) solves."""
+    import numpy as np
+    from trhepd import driver, lib, synth
+    out = {}
+    for name, cfg, count in (("c1", 1, 1), ("c2", 2, 16)):
+        sts = [synth.config(cfg, seed=args.seed, index=i) for i in range(count)]
+        blocks = synth.bulk_reflection(eng, sts[0])
+        jobs, bids = [], []
+        try:
+            for st in sts:
+                job = driver.SurfaceJob(eng, st.bp, st.surf, 0.02, lib.SP4)
+                jobs.append(job)
+                G, F, W, A = job.inputs(0, blocks)
+                bids.append(eng.batch_create(job.table_id, job.iv, G, F, job.nb0, W))
+            eng.set_renorm_period(0)
+            eng.batch_run_many(bids)   # warm-up
+            t0 = time.perf_counter()
+            ren = 0
+            for _ in range(args.small_steps):
+                eng.batch_run_many(bids)
+                ren += eng.last_stats()["n_renorm"]
+            dt = (time.perf_counter() - t0) / args.small_steps
+            dev_ms = eng.last_stats()["device_ms"]
+            N, ns, nang = jobs[0].nb, jobs[0].ns, len(A)
+        finally:
+            for b in bids:
+                eng.batch_release(b)
+            for job in jobs:
+                job.release()
+        pts = count * nang
+        # 8 N^3 flops per stage product (4M complex count), 6 stages per slice; the
+        # renormalisation solves (about 14.7 N^3 each) on top
+        flops = pts * 8.0 * N ** 3 * 6 * ns + ren / args.small_steps * 14.7 * N ** 3
+        ach = flops / (dev_ms * 1e-3) / 1e12
+        out[name] = {"workload": "%d seeded structure(s) x %d angles, N=%d, sp4 dz=0.02 ns=%d, reference "
+                                 "renormalisation rule, %s" % (count, nang, N, ns, "one fused launch of the small-N "
+                                                              "kernel" if count > 1 else "one launch"),
+                     "points": pts, "points_per_s": pts / dt, "ms_per_step": 1e3 * dt, "device_ms": dev_ms,
+                     "roofline": {"bound": "mfma", "kernel": "k_small / k_small_multi (one workgroup per angle, "
+                                  "state in LDS, v_mfma_f64_16x16x4f64)", "achieved": ach, "peak": PEAK_FP64,
+                                  "unit": "TFLOP/s", "frac": ach / PEAK_FP64, "traffic": None}}
+        log("small-N %s: %d points in %.1f ms (device %.1f ms): %.0f points/s, %.2f TFLOP/s"
+            % (name, pts, 1e3 * dt, dev_ms, pts / dt, ach))
+    return out
+
+
 def run_serving(args, precision, pool, budget):
     """The serving workload: a bounded sample of the uneven-size Poisson request stream,
     served by the library's continuous-batching loop under an HBM budget."""
@@ -180,6 +232,11 @@ def main():
                     help="also run a plain device copy of this many MiB (calibrates HBM byte counters)")
     ap.add_argument("--uneven-structures", type=int, default=16,
                     help="structures of the uneven-size graph-captured measurement (0: skip)")
+    ap.add_argument("--small-steps", type=int, default=3,
+                    help="timed runs of the small-beam configurations C1 and C2 (0: skip)")
+    ap.add_argument("--small-only", action="store_true",
+                    help="time only the small-beam configurations C1 and C2 (for a kernel trace of the "
+                         "small-N kernel)")
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests of the serving sample (0: skip the serving measurement)")
     ap.add_argument("--serve-pool", type=int, default=8, help="distinct seeded structures the requests cycle over")
@@ -196,11 +253,16 @@ def main():
     from trhepd import lib, synth
 
     method, dz2 = lib.SP4, 0.02
-    st = synth.config(3, seed=args.seed)
     eng64 = lib.Engine(0, precision=lib.PREC_FP64)
+    if args.small_only:
+        small = run_small_configs(eng64, argparse.Namespace(**dict(vars(args), small_steps=max(1, args.small_steps))))
+        eng64.close()
+        print(json.dumps({"small_n": small}))
+        return
+    st = synth.config(3, seed=args.seed)
     t0 = time.perf_counter()
-    blocks = synth.bulk_reflection(eng64, st, layers=2, dz2=dz2, method=method)
-    log("bulk reflection (2 bulk layers, device, FP64): %.1f s" % (time.perf_counter() - t0))
+    blocks = synth.bulk_reflection(eng64, st, dz2=dz2, method=method)
+    log("bulk reflection (%d bulk layers, device, FP64): %.1f s" % (st.bulk.ml, time.perf_counter() - t0))
     paths = [p.strip() for p in args.paths.split(",") if p.strip()]
     res = {}
     if args.copy_calibration_mb > 0:
@@ -277,14 +339,21 @@ def main():
         "scaling": "weak", "vs_baseline": None,
         "dtype": "i8 digit-sliced products + f64" if "mixed" in res else "f64", "data": "synthetic",
         "config": cfg,
-        "roofline": roofline(head, "mixed" in res),
     }
-    if "mixed" in res and "fp64" in res:
-        out["roofline_fp64"] = roofline(res["fp64"], False)
+    # the headline roofline: the dominant kernel's own launch time, measured in a curve run
+    # as one batch (no concurrent stream shares the device, so launches do not overlap);
+    # the concurrent-batch figures, whose event brackets overlap, are reported beside it
     if "one_batch" in head:
-        out["roofline_one_batch"] = roofline(head["one_batch"], "mixed" in res, shared=False)
-        if "fp64" in res and "mixed" in res and "one_batch" in res["fp64"]:
-            out["roofline_fp64_one_batch"] = roofline(res["fp64"]["one_batch"], False, shared=False)
+        out["roofline"] = roofline(head["one_batch"], "mixed" in res, shared=False)
+        out["roofline_concurrent"] = roofline(head, "mixed" in res)
+    else:
+        out["roofline"] = roofline(head, "mixed" in res, shared=head["nbatch"] > 1)
+    if "mixed" in res and "fp64" in res:
+        f64 = res["fp64"]
+        out["roofline_fp64"] = roofline(f64["one_batch"], False, shared=False) if "one_batch" in f64 else \
+            roofline(f64, False)
+    if args.small_steps > 0:
+        out["small_n"] = run_small_configs(eng64, args)
     if args.uneven_structures > 0:
         from trhepd import uneven
         t0 = time.perf_counter()

@@ -332,8 +332,10 @@ __device__ void lu_solve(SLds<NP> &S, int N)
 }
 
 // The whole integration of angle `pt` of one batch by one workgroup.
+// co: the integrator's coefficients, read where the kernel arguments hold them (a local copy
+// of their dynamically indexed array would live in scratch memory)
 template <int NP>
-__device__ __forceinline__ void small_body(const SmallArgs &A, const int pt, SLds<NP> &S)
+__device__ __forceinline__ void small_body(const SmallArgs &A, const RknCoef &co, const int pt, SLds<NP> &S)
 {
     constexpr int LD = SLds<NP>::LD;
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -384,8 +386,8 @@ __device__ __forceinline__ void small_body(const SmallArgs &A, const int pt, SLd
         __syncthreads();
         xid = 0;
     }
-    const int s = A.co.s;
-    const double cb1 = A.co.bab[0];
+    const int s = co.s;
+    const double cb1 = co.bab[0];
     const bool rkn = A.method == TRHEPD_SP4 || A.method == TRHEPD_SP6;
     if (A.method == TRHEPD_RK4) {   // X^H = I (surf_rk.f90:46-47)
         for (int e = tid; e < N * N; e += nt) {
@@ -427,7 +429,7 @@ __device__ __forceinline__ void small_body(const SmallArgs &A, const int pt, SLd
             __syncthreads();
         } else {
             for (int k = 1; k <= s; ++k) {
-                const double cb = A.co.bab[2 * k - 2], ca = A.co.bab[2 * k - 1];
+                const double cb = co.bab[2 * k - 2], ca = co.bab[2 * k - 1];
                 const VVi *col = A.table + (size_t)(s * (l - 1) + k - 1) * A.nv;
                 if (k == 1) {
                     if (xid) {
@@ -542,7 +544,7 @@ template <int NP>
 __global__ __launch_bounds__(256) void k_small(SmallArgs A)
 {
     __shared__ SLds<NP> S;
-    small_body<NP>(A, blockIdx.x, S);
+    small_body<NP>(A, A.co, blockIdx.x, S);
 }
 
 // Batches that differ only in their table, arrays and slice count (several structures with
@@ -587,7 +589,7 @@ __global__ __launch_bounds__(256) void k_small_multi(SmallMultiArgs M)
     A.nv = P.nv;
     A.ns = P.ns;
     A.period = P.period;
-    small_body<NP>(A, blk - P.first, S);
+    small_body<NP>(A, M.base.co, blk - P.first, S);
 }
 
 // hp_int_spa1d (surf_prkn.f90:485-524): the scalar N = 1 form used by int_method 6.
