@@ -232,12 +232,13 @@ def main():
                     help="also run a plain device copy of this many MiB (calibrates HBM byte counters)")
     ap.add_argument("--uneven-structures", type=int, default=16,
                     help="structures of the uneven-size graph-captured measurement (0: skip)")
+    ap.add_argument("--uneven-steps", type=int, default=2, help="timed launches of the uneven-batch graph")
     ap.add_argument("--small-steps", type=int, default=3,
                     help="timed runs of the small-beam configurations C1 and C2 (0: skip)")
     ap.add_argument("--small-only", action="store_true",
                     help="time only the small-beam configurations C1 and C2 (for a kernel trace of the "
                          "small-N kernel)")
-    ap.add_argument("--serve-requests", type=int, default=24,
+    ap.add_argument("--serve-requests", type=int, default=64,
                     help="requests of the serving sample (0: skip the serving measurement)")
     ap.add_argument("--serve-pool", type=int, default=8, help="distinct seeded structures the requests cycle over")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="mean arrival rate (requests/s)")
@@ -357,7 +358,7 @@ def main():
     if args.uneven_structures > 0:
         from trhepd import uneven
         t0 = time.perf_counter()
-        ureqs = uneven.requests(eng64, args.uneven_structures, seed=args.seed)
+        ureqs = uneven.requests(eng64, args.uneven_structures, seed=args.seed, log=log)
         log("uneven set: %d structures, N %s (%.1f s)" % (len(ureqs), [r.nb for r in ureqs], time.perf_counter() - t0))
         uv = {"workload": "first %d of the seeded uneven-size structures: N from {23,47,97,199,307}, 3-12 atoms, "
                           "8-20 A slabs, 28-56 angles, sp4 dz=0.02, periodic renormalisation; all batches in one "
@@ -370,7 +371,7 @@ def main():
             if p != "fp64":
                 eng.set_mixed_digits(args.mixed_digits)
             try:
-                r = uneven.measure(eng, ureqs, steps=2, warmup=1)
+                r = uneven.measure(eng, ureqs, steps=args.uneven_steps, warmup=1, log=log)
             finally:
                 eng.close()
             uf2[p] = r.pop("f2")
@@ -401,8 +402,9 @@ def main():
             r = run_serving(args, lib.PREC_FP64 if p == "fp64" else lib.PREC_MIXED, pool, budget)
             f2s[p] = r.pop("f2")
             sv[p] = r
-            log("serving %s: %.2f points/s p50 %.0f ms p99 %.0f ms (peak %d admitted)"
-                % (p, r["points_per_s"], r["p50_ms"], r["p99_ms"], r["peak_admitted"]))
+            log("serving %s: %.2f points/s p50 %.0f ms p99 %.0f ms (queued p50 %.0f ms, service p50 %.0f ms; "
+                "peak %d admitted)" % (p, r["points_per_s"], r["p50_ms"], r["p99_ms"], r["queued_ms"]["p50"],
+                                       r["service_ms"]["p50"], r["peak_admitted"]))
         if "fp64" in f2s and "mixed" in f2s:
             errs = []
             for a, b in zip(f2s["mixed"], f2s["fp64"]):

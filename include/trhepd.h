@@ -253,6 +253,10 @@ int trhepd_serve_submit(trhepd_ctx *ctx, const trhepd_potential *pot, double dz,
                         int nb0, const double *wnsga2, int *request_id);
 int trhepd_serve_step(trhepd_ctx *ctx, int *n_active, int *n_queued, int *n_done);
 int trhepd_serve_result(trhepd_ctx *ctx, int request_id, double *f, double *f2, double *latency_ms);
+/* The split of the last request trhepd_serve_result returned (finished or failed): time queued
+ * (submit to admission) and time in service (admission to observed completion), ms.  Their sum
+ * is its latency. */
+int trhepd_serve_last_timing(trhepd_ctx *ctx, double *queued_ms, double *service_ms);
 
 /* Kernel timing: when enabled, every stage-product kernel launch is bracketed by HIP
  * events on the context's stream; trhepd_kernel_stats returns the number of such

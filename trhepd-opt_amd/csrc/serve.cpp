@@ -58,7 +58,8 @@ struct PotCopy {
 };
 
 enum { QUEUED = 0, ACTIVE = 1, FINISHING = 2, DONE = 3, FAILED = 4 };
-constexpr int RUN_AHEAD = 2;   // slices a request may be enqueued ahead of the device
+constexpr int RUN_AHEAD = 4;        // slices a request may be enqueued ahead of the device
+constexpr int SLICES_PER_STEP = 16;  // slices one step may enqueue for a periodic-schedule request
 
 struct Request {
     int id = 0, state = QUEUED;
@@ -70,7 +71,7 @@ struct Request {
     int table_id = 0, batch_id = 0;
     int slice = 0;                       // next slice to enqueue (1-based)
     hipEvent_t ev[RUN_AHEAD] = {};       // end of the last RUN_AHEAD slices
-    Clock::time_point t_submit, t_done;
+    Clock::time_point t_submit, t_admit, t_done;
     int rc = TRHEPD_OK;
     long long n_renorm = 0;              // renormalisations summed over the request's angles
     std::string err;
@@ -79,6 +80,7 @@ struct Request {
 }  // namespace
 
 struct Server {
+    double last_queued_ms = 0.0, last_service_ms = 0.0;   // of the last collected request
     std::deque<int> queue;
     std::map<int, Request> reqs;
     int next_id = 1;
@@ -114,6 +116,7 @@ void server_destroy(trhepd_ctx *ctx)
 // admits one request: builds its table and batch; E_BUDGET / E_NOMEM leave it queued
 static int admit(trhepd_ctx *ctx, Request &r)
 {
+    r.t_admit = Clock::now();
     int rc = trhepd_table_build(ctx, &r.pot.p, r.dz, r.ns, r.method, &r.table_id);
     if (!rc)
         rc = trhepd_batch_create(ctx, r.table_id, r.nb, r.nang, r.iv.data(), r.gamma.data(), r.f.data(), r.nb0,
@@ -227,23 +230,59 @@ int trhepd_serve_step(trhepd_ctx *ctx, int *n_active, int *n_queued, int *n_done
             r.rc = rc;
             r.err = ctx->err;
             r.state = FAILED;
-            r.t_done = Clock::now();
+            r.t_done = r.t_admit = Clock::now();
         }
         S.queue.pop_front();
     }
-    // 3. one slice of every admitted request, interleaved across their streams
-    std::vector<Request *> act;
+    // 3. advance the admitted requests.  Requests under the reference's renormalisation rule
+    //    need the host after every slice (the flags of the Gershgorin test): one slice each,
+    //    interleaved across their streams.  Requests under a periodic schedule need no host
+    //    decision: each is given as many slices as its run-ahead events allow (up to
+    //    SLICES_PER_STEP), without blocking, so that cheap requests are not held to the pace of
+    //    the most expensive one.  The host blocks only when no request could advance.
+    std::vector<Request *> act, gated;
     for (auto &kv : S.reqs)
         if (kv.second.state == ACTIVE) act.push_back(&kv.second);
+    std::vector<Request *> host_rule;
+    bool moved = false;
     for (Request *r : act) {
         Batch *b = find_batch(ctx, r->batch_id);
         const Table *t = find_table(ctx, r->table_id);
-        // bound the run-ahead: slice l may be enqueued once slice l - RUN_AHEAD has finished
+        if (!b->run || batch_period(ctx, *b, *t) <= 0) {
+            host_rule.push_back(r);
+            continue;
+        }
+        for (int k = 0; k < SLICES_PER_STEP && r->state == ACTIVE; ++k) {
+            // slice l may be enqueued once slice l - RUN_AHEAD has finished
+            if (r->slice > RUN_AHEAD) {
+                const hipError_t q = hipEventQuery(r->ev[r->slice % RUN_AHEAD]);
+                if (q == hipErrorNotReady) {
+                    if (k == 0) gated.push_back(r);
+                    break;
+                }
+                if (q != hipSuccess) return hip_check(ctx, q, "request run-ahead");
+            }
+            int rc = runner_slice_enqueue(ctx, *b, *t, r->slice);
+            if (!rc) rc = runner_slice_finish(ctx, *b, *t);
+            if (rc) return rc;
+            TRY_HIP(ctx, hipEventRecord(r->ev[r->slice % RUN_AHEAD], b->stream));
+            if (r->slice == t->ns) {
+                if ((rc = runner_end(ctx, *b, *t))) return rc;
+                TRY_HIP(ctx, hipEventRecord(b->ev1, b->stream));
+                r->state = FINISHING;
+            }
+            ++r->slice;
+            moved = true;
+        }
+    }
+    for (Request *r : host_rule) {
+        Batch *b = find_batch(ctx, r->batch_id);
+        const Table *t = find_table(ctx, r->table_id);
         if (r->slice > RUN_AHEAD) TRY_HIP(ctx, hipEventSynchronize(r->ev[r->slice % RUN_AHEAD]));
         int rc = runner_slice_enqueue(ctx, *b, *t, r->slice);
         if (rc) return rc;
     }
-    for (Request *r : act) {
+    for (Request *r : host_rule) {
         Batch *b = find_batch(ctx, r->batch_id);
         const Table *t = find_table(ctx, r->table_id);
         int rc = runner_slice_finish(ctx, *b, *t);
@@ -255,7 +294,11 @@ int trhepd_serve_step(trhepd_ctx *ctx, int *n_active, int *n_queued, int *n_done
             r->state = FINISHING;
         }
         ++r->slice;
+        moved = true;
     }
+    // nothing could advance: wait for the oldest gating slice of one request instead of spinning
+    if (!moved && !gated.empty())
+        TRY_HIP(ctx, hipEventSynchronize(gated[0]->ev[gated[0]->slice % RUN_AHEAD]));
     int na = 0, nd = 0;
     for (auto &kv : S.reqs) {
         na += kv.second.state == ACTIVE || kv.second.state == FINISHING;
@@ -276,6 +319,8 @@ int trhepd_serve_result(trhepd_ctx *ctx, int request_id, double *f, double *f2, 
     Request &r = it->second;
     if (r.state != DONE && r.state != FAILED) return TRHEPD_E_PENDING;
     if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r.t_done - r.t_submit).count();
+    S.last_queued_ms = std::chrono::duration<double, std::milli>(r.t_admit - r.t_submit).count();
+    S.last_service_ms = std::chrono::duration<double, std::milli>(r.t_done - r.t_admit).count();
     const int rc = r.rc;
     if (rc) {
         ctx->err = r.err;
@@ -288,3 +333,12 @@ int trhepd_serve_result(trhepd_ctx *ctx, int request_id, double *f, double *f2, 
 }
 
 }  // extern "C"
+
+extern "C" int trhepd_serve_last_timing(trhepd_ctx *ctx, double *queued_ms, double *service_ms)
+{
+    if (!ctx) return TRHEPD_E_ARG;
+    Server &S = server(ctx);
+    if (queued_ms) *queued_ms = S.last_queued_ms;
+    if (service_ms) *service_ms = S.last_service_ms;
+    return TRHEPD_OK;
+}

@@ -29,7 +29,7 @@ SYMBOLS = ("trhepd_ctx_create", "trhepd_ctx_destroy", "trhepd_last_error", "trhe
            "trhepd_batch_release", "trhepd_last_stats", "trhepd_set_renorm_period", "trhepd_set_mixed_digits", "trhepd_profile",
            "trhepd_kernel_stats", "trhepd_serve_submit", "trhepd_serve_step", "trhepd_serve_result",
            "trhepd_graph_create", "trhepd_graph_launch", "trhepd_graph_release", "trhepd_bulk_reflect",
-           "trhepd_propagate_batch_noalloc", "trhepd_set_bulk_batched_above")
+           "trhepd_propagate_batch_noalloc", "trhepd_set_bulk_batched_above", "trhepd_serve_last_timing")
 
 
 class TrhepdError(RuntimeError):
@@ -98,6 +98,7 @@ def load() -> C.CDLL:
     L.trhepd_graph_launch.argtypes = [vp, i]
     L.trhepd_graph_release.argtypes = [vp, i]
     L.trhepd_serve_result.argtypes = [vp, i, vp, vp, dp]
+    L.trhepd_serve_last_timing.argtypes = [vp, dp, dp]
     for name in SYMBOLS:
         getattr(L, name).restype = getattr(L, name).restype or C.c_int
     L.trhepd_last_error.restype = C.c_char_p
@@ -294,7 +295,8 @@ class Engine:
         return a.value, q.value, d.value
 
     def serve_result(self, rid: int, want_f: bool = False):
-        """(f or None, f2, latency_ms) of a finished request, or None while it is pending."""
+        """(f or None, f2, latency_ms, queued_ms, service_ms) of a finished request, or None
+        while it is pending (latency = queued + service)."""
         nang, nb = self._serve_shape[rid]
         f = np.zeros((nang, nb, nb), dtype=np.complex128) if want_f else None
         f2 = np.zeros((nang, nb))
@@ -303,8 +305,10 @@ class Engine:
         if rc == E_PENDING:
             return None
         del self._serve_shape[rid]
+        qd, sv = C.c_double(), C.c_double()
+        self._L.trhepd_serve_last_timing(self._h, C.byref(qd), C.byref(sv))
         self._chk(rc)
-        return (None if f is None else f.transpose(0, 2, 1)), f2, lat.value
+        return (None if f is None else f.transpose(0, 2, 1)), f2, lat.value, qd.value, sv.value
 
     def table_shape(self, tid: int):
         nv, nc = C.c_int(), C.c_int()

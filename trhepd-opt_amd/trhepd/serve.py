@@ -97,11 +97,26 @@ def serve(eng: "lib.Engine", reqs: Sequence[Request], arrivals: Optional[Sequenc
     return {"results": out, "span_s": span, "peak_active": peak}
 
 
-def summary(reqs: Sequence[Request], run: Dict[str, object], first_arrival: float = 0.0) -> Dict[str, float]:
-    """Throughput (points/s over the run) and latency percentiles of a finished run."""
-    lat = np.array([r[2] for r in run["results"]])
+def summary(reqs: Sequence[Request], run: Dict[str, object], first_arrival: float = 0.0) -> Dict[str, object]:
+    """Throughput (points/s over the run), latency percentiles of a finished run, and per beam
+    count the latency, queueing (submit to admission) and service (admission to completion)
+    percentiles."""
+    res = run["results"]
+    lat = np.array([r[2] for r in res])
+    que = np.array([r[3] for r in res])
+    srv = np.array([r[4] for r in res])
     pts = sum(r.nang for r in reqs)
     span = run["span_s"] - first_arrival
+
+    def pct(x):
+        return {"p50": float(np.percentile(x, 50)), "p99": float(np.percentile(x, 99))}
+
+    buckets = {}
+    for nb in sorted({r.nb for r in reqs}):
+        k = [i for i, r in enumerate(reqs) if r.nb == nb]
+        buckets[str(nb)] = {"requests": len(k), "points": int(sum(reqs[i].nang for i in k)),
+                            "latency_ms": pct(lat[k]), "queued_ms": pct(que[k]), "service_ms": pct(srv[k])}
     return {"requests": len(reqs), "points": pts, "span_s": span, "points_per_s": pts / span,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-            "max_ms": float(lat.max()), "peak_admitted": run["peak_active"]}
+            "max_ms": float(lat.max()), "queued_ms": pct(que), "service_ms": pct(srv),
+            "peak_admitted": run["peak_active"], "by_beams": buckets}

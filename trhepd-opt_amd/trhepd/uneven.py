@@ -13,11 +13,13 @@ from . import lib, serve, synth
 
 
 def requests(eng: "lib.Engine", count: int, seed: int = 1234, cfg: int = 4, dz2: float = 0.02,
-             method: int = lib.SP4, max_angles: int = 0) -> List[serve.Request]:
+             method: int = lib.SP4, max_angles: int = 0, log=None) -> List[serve.Request]:
     """`count` seeded structures of the uneven configuration, with their bulk reflections;
     `max_angles` > 0 keeps that many angles of each scan."""
     out = []
     for i in range(count):
+        if log and i and i % 16 == 0:
+            log("uneven set: %d of %d structures set up" % (i, count))
         r = serve.structure_request(eng, synth.config(cfg, seed=seed, index=i), dz2, method, key="cfg%d-%d" % (cfg, i))
         if max_angles:
             k = slice(0, max_angles)
@@ -62,18 +64,25 @@ class Resident:
         self.bids, self.tids = [], []
 
 
-def measure(eng: "lib.Engine", reqs: Sequence[serve.Request], steps: int = 2, warmup: int = 1) -> Dict[str, object]:
+def measure(eng: "lib.Engine", reqs: Sequence[serve.Request], steps: int = 2, warmup: int = 1,
+            log=None) -> Dict[str, object]:
     """Points/s of replaying the captured run of all requests."""
     res = Resident(eng, reqs)
     try:
         t0 = time.perf_counter()
         res.capture()
         cap = time.perf_counter() - t0
-        for _ in range(warmup):
+        if log:
+            log("uneven: %d batches captured in %.1f s" % (len(reqs), cap))
+        for k in range(warmup):
             res.launch()
+            if log:
+                log("uneven: warm-up launch %d done" % k)
         t0 = time.perf_counter()
-        for _ in range(steps):
+        for k in range(steps):
             res.launch()
+            if log:
+                log("uneven: timed launch %d done" % k)
         dt = (time.perf_counter() - t0) / steps
         f2 = res.results()
     finally:
