@@ -110,3 +110,28 @@ def test_mixed_leapfrog_c3_vs_fp64(engine):
     print("C3 leapfrog mixed vs fp64: max rel %.3e" % errs.max())
     assert np.all(np.isfinite(out["mixed"]))
     assert errs.max() < 1e-6
+
+
+def test_every_stage_tile_gives_the_same_bits(monkeypatch):
+    """Each int8 stage-product tile (column blocks x row blocks x Q prefetch depth), pinned one
+    at a time, gives bit-identical
+    intensities: the digit sums are exact integers and every tile combines them in the same
+    order.  Five angles of the N = 521 deck, so that several row tiles run."""
+    surf = O.load_case(os.path.join(REF, "test521"), 0.68, 5)
+    angs, G, F, W = _inputs(surf)
+    scale = np.array([1.0, 0.999, 1.001, 0.998, 1.002])
+    Gb, Fb, Wb = np.array([G[0] * s for s in scale]), np.array([F[0]] * 5), np.array([W[0]] * 5)
+    out = {}
+    for v in range(12):
+        monkeypatch.setenv("TRHEPD_STAGE_VARIANT", str(v))
+        with lib.Engine(0, precision=lib.PREC_MIXED) as eng:
+            eng.set_mixed_digits(6)
+            try:
+                out[v], _ = _run(eng, surf, 5, Gb, Fb, Wb)
+            except lib.TrhepdError as e:   # a tile the registers do not hold
+                assert e.code == lib.E_ARG
+                continue
+        print("variant", v, "done", flush=True)
+    assert len(out) >= 8
+    for v, f2 in out.items():
+        assert np.array_equal(f2, out[0]), v

@@ -22,6 +22,8 @@
 #include <type_traits>
 #include <utility>
 
+#include <cstdlib>
+
 #include "internal.hpp"
 
 namespace trhepd {
@@ -2397,6 +2399,15 @@ int autotune(trhepd_ctx *ctx, Batch &b, const Table &t)
     auto it = ctx->tuned.find(key);
     if (it != ctx->tuned.end()) {
         apply_variant(R, it->second);
+        return TRHEPD_OK;
+    }
+    // TRHEPD_STAGE_VARIANT pins one int8 tile variant of the mixed path instead of timing
+    // them (a test control: every variant must give the same bits)
+    if (const char *pin = R.mixed ? std::getenv("TRHEPD_STAGE_VARIANT") : nullptr) {
+        const int v = std::atoi(pin);
+        if (v < 0 || v >= n_variants(R) || !apply_variant(R, v))
+            return fail(ctx, TRHEPD_E_ARG, "TRHEPD_STAGE_VARIANT=%s does not name a tile of this batch", pin);
+        ctx->tuned[key] = v;
         return TRHEPD_OK;
     }
     const bool prof = ctx->prof.on;
