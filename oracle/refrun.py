@@ -32,12 +32,13 @@ def _unlimited_stack():
     resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
 
 
-def _run(exe: str, files: Dict[str, bytes], stdin: str, threads: int, timeout: float, outputs):
+def _run(exe: str, files: Dict[str, bytes], stdin: str, threads: int, timeout: float, outputs, extra_env=None):
     if not available(exe):
         raise FileNotFoundError("%s is not built (make -f oracle/ref.mk)" % exe)
     env = dict(os.environ)
     env["OMP_NUM_THREADS"] = str(threads)
     env.setdefault("OMP_STACKSIZE", "512M")
+    env.update(extra_env or {})
     d = tempfile.mkdtemp(prefix="trhepd-ref-")
     try:
         for name, data in files.items():
@@ -75,6 +76,14 @@ def surf(surf_txt: bytes, bulkp: bytes, dz: float, method: int, threads: int = 8
     return out["surf-bulkP.s"].decode(), _omp_seconds(so), wall
 
 
+def surf_domains(surf_txt: bytes, bulkp: bytes, dz: float, method: int, threads: int = 8, timeout: float = 3600):
+    """surf.exe on a deck of several domains (surfm.f90:64-90): returns (the .md text with every
+    domain's curve, the .s text of its domain sum)."""
+    out, _, _ = _run(SURF, {"surf.txt": surf_txt, "bulkP.b": bulkp}, "%r %d\n" % (dz, method), threads, timeout,
+                     ["surf-bulkP.md", "surf-bulkP.s"])
+    return out["surf-bulkP.md"].decode(), out["surf-bulkP.s"].decode()
+
+
 def lapack(surf_txt: bytes, bulkp: bytes, dz: float, threads: int = 8, timeout: float = 3600):
     """str-lapack surf.exe (transfer-matrix method): returns (.s text, printed time, process wall)."""
     out, so, wall = _run(LAPACK, {"surf.txt": surf_txt, "bulkP.b": bulkp}, "%r\n" % dz, threads, timeout,
@@ -82,7 +91,7 @@ def lapack(surf_txt: bytes, bulkp: bytes, dz: float, threads: int = 8, timeout: 
     return out["surf-bulkP.s"].decode(), _omp_seconds(so), wall
 
 
-def bulk(bulk_txt: bytes, threads: int = 8, timeout: float = 3600):
+def bulk(bulk_txt: bytes, threads: int = 8, timeout: float = 3600, extra_env=None):
     """sim-trhepd-rheed bulk.exe: returns (bulkP.b bytes, process wall s)."""
-    out, _, wall = _run(BULK, {"bulk.txt": bulk_txt}, "", threads, timeout, ["bulkP.b"])
+    out, _, wall = _run(BULK, {"bulk.txt": bulk_txt}, "", threads, timeout, ["bulkP.b"], extra_env)
     return out["bulkP.b"], wall

@@ -127,10 +127,12 @@ def test_bulk_rejects_wrong_table(engine):
         job.release()
 
 
-@pytest.mark.parametrize("case", ["ge001", "test23"])
+@pytest.mark.parametrize("case", ["ge001", "test23", "test23dom"])
 def test_bulk_program_from_deck(engine, case, tmp_path):
     """bulk.txt -> bulkP.b as the reference's bulk program writes it: header records
-    byte-identical, reflection blocks within the FP64 bar."""
+    byte-identical, reflection blocks within the FP64 bar.  test23dom has two domains (the
+    second turned by 60 degrees, its incident azimuth AZI + RDOM), each with its own blocks,
+    recorded from one-domain runs of the reference's bulk program."""
     bd = deckio.BulkDeck.parse(open(os.path.join(REF, case, "bulk.txt")).read())
     out = driver.bulk_program(engine, bd)
     path = tmp_path / "bulkP.b"
@@ -139,10 +141,12 @@ def test_bulk_program_from_deck(engine, case, tmp_path):
     ref = deckio.BulkP.read(os.path.join(REF, case, "bulkP.b"))
     hdr = lambda b: deckio.BulkP.to_bytes(_no_blocks(b))
     assert hdr(got) == hdr(ref)
-    for rg, rr in zip(got.domains[0].blocks, ref.domains[0].blocks):
-        for ag, ar in zip(rg, rr):
-            for a, b in zip(ag, ar):
-                assert _err(a, b) <= 1e-10
+    assert len(got.domains) == len(ref.domains) == ref.ndom
+    for dg, dr in zip(got.domains, ref.domains):
+        for rg, rr in zip(dg.blocks, dr.blocks):
+            for ag, ar in zip(rg, rr):
+                for a, b in zip(ag, ar):
+                    assert _err(a, b) <= 1e-10
 
 
 def _no_blocks(bp):

@@ -4,11 +4,13 @@ outputs and what the geometry fixes.
 - Angular convolution: the reference's recorded curves and their convolutions
   (sim-trhepd-rheed/sample/Ge_001_c4x2/output, script/make_convolution.py; str-orig's
   T4 Al / Si(111) sample, tool/make_convolution.py) are reproduced byte for byte.
-- Domain sum (domainsum.f90): no reference file holds a multi-domain run, so it is pinned by
-  constructions whose answer is known: identical domains add with their weights, and a
-  domain rotated by 120 degrees on the hexagonal lattice (its beams obtained from the real
-  lattice's geometry, not from the reference's beam-matching formula) lands beam for beam on
-  the reference domain.
+- Domain sum (domainsum.f90): the reference ships no multi-domain run, so its own programs,
+  built from its sources, ran one (tests/golden/make_domain_fixture.py: test23 plus a second
+  domain turned by 60 degrees, the twin of the p3m1 surface, weights 0.6 / 0.4).  Its .md is
+  re-summed byte for byte, and re-written byte for byte from its numbers.  Constructions whose answer is known add to that:
+  identical domains add with their weights, and a domain rotated by 120 degrees on the
+  hexagonal lattice (its beams obtained from the real lattice's geometry, not from the
+  reference's beam-matching formula) lands beam for beam on the reference domain.
 """
 import math
 import os
@@ -100,3 +102,66 @@ def test_domainsum_refuses_single_domain():
     import pytest
     with pytest.raises(ValueError):
         post.domainsum(open(os.path.join(REF, "ge001", "surf-bulkP.s")).read())
+
+
+DOM = os.path.join(REF, "test23dom")
+
+
+def read_md_rows(md_text):
+    """The numbers of a .md text: header values, and per domain the list of (angle, f2) rows
+    of each azimuth / glancing-angle block."""
+    lines = md_text.splitlines()
+    naz, ng, ndom, nh, nk = (int(v) for v in lines[1].split()[:5])
+    nb = [int(lines[3 + d].split()[0]) for d in range(ndom)]
+    k = 4 + ndom
+    assert lines[3 + ndom] == "#ih,ik"
+    for d in range(ndom):
+        vals = []
+        while len(vals) < 2 * nb[d]:
+            vals += lines[k].split()
+            k += 1
+    nrep1, nrep2 = (naz, ng) if ng > naz else (ng, naz)
+    out = []
+    for d in range(ndom):
+        blocks = []
+        for _ in range(nrep1):
+            rows = []
+            for _ in range(nrep2):
+                v = [deckio._parse_fnum(t) for t in lines[k].split(",") if t.strip()]
+                rows.append((v[0], v[1:1 + nb[d]]))
+                k += 1
+            assert lines[k] == ""
+            k += 1
+            blocks.append(rows)
+        out.append(blocks)
+    return (naz, ng, nh, nk, nb), out
+
+
+def test_domainsum_reproduces_the_reference_two_domain_run():
+    md = open(os.path.join(DOM, "surf-bulkP.md")).read()
+    assert post.domainsum(md) == open(os.path.join(DOM, "surf-bulkP.s")).read()
+
+
+def test_format_md_reproduces_the_reference_md():
+    md = open(os.path.join(DOM, "surf-bulkP.md")).read()
+    (naz, ng, nh, nk, nb), rows = read_md_rows(md)
+    bp = deckio.BulkP.read(os.path.join(DOM, "bulkP.b"))
+    sd = deckio.SurfDeck.parse(open(os.path.join(DOM, "surf.txt")).read(), bp.ndom)
+    assert (bp.ndom, bp.nb, sd.wdom) == (2, [23, 23], [0.6, 0.4])
+    deg = 180.0 / math.pi
+    out = deckio.format_md(naz, ng, bp.nh, bp.nk, bp.gam * deg, bp.nb, [r * deg for r in bp.rdom], sd.wdom,
+                           bp.ih, bp.ik, rows)
+    assert out == md
+    # the two domains' curves differ: the second is not a copy of the first
+    f1 = np.array([r[1] for r in rows[0][0]])
+    f2 = np.array([r[1] for r in rows[1][0]])
+    assert np.max(np.abs(f1 - f2)) > 1e-3 * np.max(f1)
+
+
+def test_domainsum_weights_override_on_the_reference_run():
+    """Weights 1 / 0 give domain 1's curve, at the E12.4 of the domain sum's output."""
+    md = open(os.path.join(DOM, "surf-bulkP.md")).read()
+    _, rows = read_md_rows(md)
+    got = deckio.read_s_text(post.domainsum(md, wdom=[1.0, 0.0]))
+    want = np.array([r[1] for r in rows[0][0]])
+    assert np.array_equal(got.f2, np.vectorize(lambda v: float(deckio.fortran_e(v, 12, 4)))(want))

@@ -87,3 +87,33 @@ def test_oracle_bulk_vs_built_reference_on_reference_deck():
             worst = max(worst, float(np.abs(np.asarray(x) - y).max() / np.abs(y).max()))
     print("oracle bulk vs built reference %.3e" % worst)
     assert worst < 1e-8
+
+
+def test_two_domain_bulk_run_reads_uncleared_memory():
+    """Why the two-domain fixture takes each domain's blocks from a one-domain run: in a
+    two-domain run of the reference's bulk program the potential of the second domain is
+    summed into arrays that are allocated but never cleared (bulkio.f90:184-198,
+    scpot.f90:183-187), so that domain's blocks depend on what the heap held.  Its first
+    domain gets fresh zeroed pages and equals the one-domain run; its second domain does not,
+    and with glibc's MALLOC_PERTURB_ filling new allocations the run writes NaN."""
+    d = os.path.join(REF, "test23dom")
+    two = deckio.BulkP.read(refrun.bulk(open(os.path.join(d, "bulk.txt"), "rb").read(), threads=4)[0])
+    fixture = deckio.BulkP.read(os.path.join(d, "bulkP.b"))    # blocks of one-domain runs
+    err = []
+    for k in range(2):
+        err.append(max(np.abs(a - b).max() / np.abs(b).max()
+                       for ra, rb in zip(two.domains[k].blocks, fixture.domains[k].blocks)
+                       for ga, gb in zip(ra, rb) for a, b in zip(ga, gb)))
+    print("two-domain run vs one-domain runs: domain 1 %.3e, domain 2 %.3e" % tuple(err))
+    assert err[0] == 0.0 and err[1] > 1e-3
+    bad = deckio.BulkP.read(refrun.bulk(open(os.path.join(d, "bulk.txt"), "rb").read(), threads=4,
+                                        extra_env={"MALLOC_PERTURB_": "165"})[0])
+    assert not np.all(np.isfinite(np.asarray(bad.domains[1].blocks[0][0][0])))
+    # the oracle, which clears its potential, gives the one-domain runs' blocks for both
+    pts = [0, 40]
+    n2 = fixture.nrep[1]
+    for k in range(2):
+        ob = OB.Bulk(fixture, k).blocks(points=pts)
+        e = max(np.abs(a - b).max() / np.abs(b).max() for p in pts
+                for a, b in zip(ob[p // n2][p % n2], fixture.domains[k].blocks[p // n2][p % n2]))
+        assert e < 1e-10, (k, e)

@@ -222,6 +222,9 @@ class SurfDeck:
         o.append("%d\t\t,NATM" % len(self.atoms))
         for a in self.atoms:
             o.append("%d,%s,%s,%s,%s" % (a.ielm, _g(a.ocr), _g(a.x), _g(a.y), _g(a.z)))
+        if len(self.wdom) > 1:
+            # read only for several domains (surfio.f90:191-194)
+            o.append(",".join(_g(w) for w in self.wdom) + "\t\t,WDOM")
         return "\n".join(o) + "\n"
 
 

@@ -111,7 +111,7 @@ def domainsum(md_text: str, wdom: Optional[Sequence[float]] = None) -> str:
     for r1 in range(nrep1):
         for r2 in range(nrep2):
             out.append(deckio.format_s_row(angle[r2], s2[r1, r2], 12, 4))
-        out.append(" ")
+        out.append("")   # the empty record of write (3,*), as gfortran writes it
     return "\n".join(out) + "\n"
 
 
