@@ -22,6 +22,7 @@
 #include <type_traits>
 #include <utility>
 
+#include <cstdio>
 #include <cstdlib>
 
 #include "internal.hpp"
@@ -2412,6 +2413,9 @@ int autotune(trhepd_ctx *ctx, Batch &b, const Table &t)
     }
     const bool prof = ctx->prof.on;
     ctx->prof.on = false;
+    // TRHEPD_AUTOTUNE_LOG=1 prints every variant's time per launch to stderr
+    const char *log = std::getenv("TRHEPD_AUTOTUNE_LOG");
+    const bool verbose = log && log[0] == '1';
     int best = -1;
     float best_ms = 0;
     for (int v = 0; v < n_variants(R); ++v) {
@@ -2423,6 +2427,14 @@ int autotune(trhepd_ctx *ctx, Batch &b, const Table &t)
         TRY_HIP(ctx, hipEventSynchronize(b.ev1));
         float ms = 0;
         TRY_HIP(ctx, hipEventElapsedTime(&ms, b.ev0, b.ev1));
+        if (verbose) {
+            if (R.mixed)
+                fprintf(stderr, "autotune N=%d angles=%d digits=%d variant %d (wc %d wr %d pf %d): %.1f us\n", R.g.N,
+                        R.g.A, R.mg.ND, v, R.mg.WC, R.mg.WR, R.mg.PF, ms * 1000.0f / 3);
+            else
+                fprintf(stderr, "autotune N=%d angles=%d fp64 variant %d (bn %d): %.1f us\n", R.g.N, R.g.A, v, R.bn,
+                        ms * 1000.0f / 3);
+        }
         if (best < 0 || ms < best_ms) {
             best = v;
             best_ms = ms;

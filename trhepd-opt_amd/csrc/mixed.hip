@@ -117,7 +117,8 @@ __device__ __forceinline__ void split_block(const MixedGeo &g, Get get, int blk,
         __syncthreads();
         e = scale_exp(smax[c][16]);
     }
-    const size_t plane = (size_t)nblkp * g.KC * 64;   // v4i per digit plane
+    // fragment ((p nblkp + blk) 3 + m) KC + kc of digit p, block blk, real operand m (re, im,
+    // re + im), k chunk kc: the 3 KC chunks a stage product walks are consecutive
     v4i *o = (v4i *)out;
     for (int grp = gq; grp < ng; grp += 16) {
         int w[3][ND][4];
@@ -147,7 +148,7 @@ __device__ __forceinline__ void split_block(const MixedGeo &g, Get get, int blk,
         for (int m = 0; m < 3; ++m)
 #pragma unroll
             for (int p = 0; p < ND; ++p)
-                o[(size_t)(m * ND + p) * plane + ((size_t)blk * g.KC + kc) * 64 + lane] =
+                o[((((size_t)p * nblkp + blk) * 3 + m) * g.KC + kc) * 64 + lane] =
                     v4i{w[m][p][0], w[m][p][1], w[m][p][2], w[m][p][3]};
     }
 }
@@ -195,9 +196,12 @@ struct StageLaunch {
 // Only the shared M0 fragments go through LDS (global -> registers -> LDS, double-buffered);
 // each wave loads its Q fragments straight into registers.  Both are fetched one chunk
 // ahead, so the loads of chunk c+1 overlap the MFMAs of chunk c.  The chunks of the three
-// real products form one pipelined sequence of 3 KC steps.
+// real products form one pipelined sequence of 3 KC steps, whose fragments lie consecutively
+// in each digit plane, so that a chunk's address is a wave-uniform base plus 1 KB per chunk.
+// A four-wave tile is held to 256 VGPRs (two waves per SIMD), so that two workgroups share a
+// CU and one's FP64 epilogue, bound by HBM, overlaps the other's int8 k loop.
 template <int ND, int WC, int WR, int PF>
-__global__ __launch_bounds__(64 * WR, 1) void k_stage_i8l(StageLaunch Lc)
+__global__ __launch_bounds__(64 * WR, (WR == 4 && ND * WC <= 24) ? 2 : 1) void k_stage_i8l(StageLaunch Lc)
 {
     constexpr int NBF = WC * ND, NR = (NBF + WR - 1) / WR;
     extern __shared__ v4i lds[];   // [2][NBF][64]
@@ -207,29 +211,31 @@ __global__ __launch_bounds__(64 * WR, 1) void k_stage_i8l(StageLaunch Lc)
     const int t = (L & 7) * Lc.tiles_per_xcd + (L >> 3);
     if (t >= Lc.ntiles) return;
     const int rt = t / Lc.ncolt, ct = t % Lc.ncolt;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int l16 = lane & 15, kq = lane >> 4;
     const int rb0 = rt * WR, cb0 = ct * WC;
-    const size_t qplane = (size_t)g.RBP * g.KC * 64, mplane = (size_t)g.CBP * g.KC * 64;
-    const v4i *__restrict__ QD = (const v4i *)S.qd + (size_t)(rb0 + wave) * g.KC * 64 + lane;
-    const v4i *__restrict__ MD = (const v4i *)S.md;
     const int nchunk = 3 * g.KC;
+    // wave-uniform fragment bases (in v4i): Q digit p of this wave's row block, and the M0
+    // fragments r of this wave's share of the workgroup's WC ND column fragments
+    const size_t qstep = (size_t)g.RBP * nchunk * 64, mstep = (size_t)g.CBP * nchunk * 64;
+    const v4i *__restrict__ QD = (const v4i *)S.qd + (size_t)(rb0 + wave) * nchunk * 64;
+    const v4i *__restrict__ MD = (const v4i *)S.md;
+    size_t mbase[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        const int f = min(r * WR + wave, NBF - 1), q = f / WC, cb = f % WC;
+        mbase[r] = q * mstep + (size_t)(cb0 + cb) * nchunk * 64;
+    }
     v4i stage[NR];
     auto gload = [&](int c) {
-        const int m = c / g.KC, kc = c - m * g.KC;
 #pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            const int f = r * WR + wave;
-            if (f < NBF) {
-                const int q = f / WC, cb = f % WC;
-                stage[r] = MD[(size_t)(m * ND + q) * mplane + ((size_t)(cb0 + cb) * g.KC + kc) * 64 + lane];
-            }
-        }
+        for (int r = 0; r < NR; ++r)
+            if (r * WR + wave < NBF) stage[r] = MD[mbase[r] + (size_t)c * 64 + lane];
     };
     auto qload = [&](int c, v4i *dst) {
-        const int m = c / g.KC, kc = c - m * g.KC;
 #pragma unroll
-        for (int p = 0; p < ND; ++p) dst[p] = QD[(size_t)(m * ND + p) * qplane + (size_t)kc * 64];
+        for (int p = 0; p < ND; ++p) dst[p] = QD[p * qstep + (size_t)c * 64 + lane];
     };
     auto sstore = [&](int buf) {
 #pragma unroll
@@ -267,8 +273,9 @@ __global__ __launch_bounds__(64 * WR, 1) void k_stage_i8l(StageLaunch Lc)
 #pragma unroll
                 for (int cb = 0; cb < WC; ++cb) acc[p + q][cb] = mfma_i8(b[cb], qcur[p], acc[p + q][cb]);
         }
-        const int m = c / g.KC;
-        if (c - m * g.KC == g.KC - 1) {
+        // the last chunk of real product m: fold its level sums into the FP64 results
+        const int m = c + 1 == g.KC ? 0 : c + 1 == 2 * g.KC ? 1 : c + 1 == nchunk ? 2 : -1;
+        if (m >= 0) {
             const double sr = m == 2 ? 0.0 : (m == 0 ? 1.0 : -1.0);
             const double si = m == 2 ? 1.0 : -1.0;
 #pragma unroll
@@ -417,9 +424,9 @@ MixedGeo mixed_geo(int A, int N, int NP, int ND, int wc, int wr, int pf)
     // are asked for, or the 7-digit 4-block accumulators need the registers of one wave per
     // SIMD.  The LDS holds only the M0 fragments, 2 WC ND KB.
     g.WR = (wr == 4 || (ND == 7 && g.WC == 4)) ? 4 : 8;
-    // a Q prefetch two chunks deep needs a third set of Q registers: eight-wave tiles whose
-    // registers have room for it
-    g.PF = (pf == 2 && g.WR == 8 && mixed_pf2_fits(ND, g.WC)) ? 2 : 1;
+    // a Q prefetch two chunks deep needs a third set of Q registers: tiles whose registers
+    // have room for it
+    g.PF = (pf == 2 && mixed_pf2_fits(ND, g.WC)) ? 2 : 1;
     return g;
 }
 
@@ -465,7 +472,8 @@ template <int ND, int WC>
 static void stage_i8_wr(hipStream_t st, const StageLaunch &Lc)
 {
     if constexpr (mixed_pf2_fits(ND, WC)) {
-        if (Lc.S.g.WR == 8 && Lc.S.g.PF == 2) return stage_i8_on<ND, WC, 8, 2>(st, Lc);
+        if (Lc.S.g.PF == 2)
+            return Lc.S.g.WR == 8 ? stage_i8_on<ND, WC, 8, 2>(st, Lc) : stage_i8_on<ND, WC, 4, 2>(st, Lc);
     }
     if constexpr (!(ND == 7 && WC == 4)) {
         if (Lc.S.g.WR == 8) return stage_i8_on<ND, WC, 8, 1>(st, Lc);
