@@ -223,7 +223,7 @@ def main():
                     help="0: the reference's Gershgorin rule; K: every K slices; -1: K from the evanescent growth rate")
     ap.add_argument("--split", type=int, default=0,
                     help="angle groups run concurrently on separate streams (overlaps the LU chains); "
-                         "0: 2 for the FP64 path, 3 for the mixed path (the fastest layouts measured)")
+                         "0: 2 for both paths (the fastest layout measured: 2-8 groups swept on N=521)")
     ap.add_argument("--paths", default="fp64,mixed", help="precision paths to time (fp64, mixed)")
     ap.add_argument("--mixed-digits", type=int, default=6,
                     help="digits per operand of the mixed path's int8 stage products (5, 6 or 7); the run "
@@ -283,7 +283,7 @@ def main():
             eng.set_mixed_digits(args.mixed_digits)
         log("path %s" % p)
         pargs = argparse.Namespace(**vars(args))
-        pargs.split = args.split or (2 if p == "fp64" else 3)
+        pargs.split = args.split or 2
         res[p] = run_path(eng, st, blocks, pargs, method, dz2)
         if pargs.split > 1 and not args.no_one_batch:
             # the same kernels in one batch of all angles, one untimed curve: per-launch
