@@ -198,6 +198,7 @@ def run_serving(args, precision, pool, budget):
     arr = serve.poisson_arrivals(len(reqs), args.serve_rate, seed=args.seed)
     eng = lib.Engine(0, precision=precision, hbm_budget=budget)
     eng.set_renorm_period(args.renorm_period)
+    eng.serve_set_merge(args.serve_merge_angles, args.serve_merge_window_ms)
     try:
         run = serve.serve(eng, reqs, arrivals=arr)
     finally:
@@ -243,6 +244,10 @@ def main():
     ap.add_argument("--serve-pool", type=int, default=8, help="distinct seeded structures the requests cycle over")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="mean arrival rate (requests/s)")
     ap.add_argument("--serve-budget-gb", type=float, default=24.0, help="HBM budget of the serving context")
+    ap.add_argument("--serve-merge-angles", type=int, default=512,
+                    help="requests of one structure merge into batches of at most this many angles (0: off)")
+    ap.add_argument("--serve-merge-window-ms", type=float, default=200.0,
+                    help="how long the head of the queue waits for requests of its structure to join it")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the slice loop shards by structure and angle with no "
@@ -394,8 +399,10 @@ def main():
         budget = int(args.serve_budget_gb * (1 << 30))
         sv = {"workload": "first %d requests of a Poisson stream (mean %.0f req/s) over %d seeded structures with N "
                           "from {23,47,97,199,307,521}, 3-12 atoms, 8-20 A slabs, 20-56 angles, sp4 dz=0.02; "
-                          "continuous batching under a %.0f GiB HBM budget"
-                          % (args.serve_requests, args.serve_rate, len(pool), args.serve_budget_gb),
+                          "continuous batching under a %.0f GiB HBM budget, requests of one structure merged into "
+                          "batches of up to %d angles (%.0f ms window)"
+                          % (args.serve_requests, args.serve_rate, len(pool), args.serve_budget_gb,
+                             args.serve_merge_angles, args.serve_merge_window_ms),
               "beams": [r.nb for r in pool], "angles": [r.nang for r in pool]}
         f2s = {}
         for p in paths:

@@ -257,6 +257,16 @@ int trhepd_serve_result(trhepd_ctx *ctx, int request_id, double *f, double *f2, 
  * (submit to admission) and time in service (admission to observed completion), ms.  Their sum
  * is its latency. */
 int trhepd_serve_last_timing(trhepd_ctx *ctx, double *queued_ms, double *service_ms);
+/* Merging of requests of one structure (no counterpart in the reference).  With max_angles > 0,
+ * the queued requests that share the head of the queue's potential descriptor, dz, ns,
+ * int_method, beam set (nb, iv) and nb0, and whose renormalisation schedule is the same (the
+ * derived period depends on each request's largest |Im gamma|), are admitted with it as one
+ * batch of at most max_angles angles; a merged batch that does not fit the HBM budget is
+ * retried with half of them.  The head waits until window_ms after its submission before
+ * admission, so that later requests can join it.  Results are bit-identical to those of the
+ * requests served one by one; latency and timing are reported per request.  max_angles 0 (the
+ * default) turns merging off. */
+int trhepd_serve_set_merge(trhepd_ctx *ctx, int max_angles, double window_ms);
 
 /* Kernel timing: when enabled, every stage-product kernel launch is bracketed by HIP
  * events on the context's stream; trhepd_kernel_stats returns the number of such

@@ -124,3 +124,79 @@ def test_mix_with_an_n521_request(engine, reqs):
         assert np.all(np.isfinite(a)) and np.all(a >= 0)
     assert np.array_equal(alone[1], ref)
     assert run["peak_active"] >= 2
+
+
+def _split(r, parts):
+    """the request's angles as `parts` requests of consecutive angles (one structure)"""
+    out = []
+    for k, idx in enumerate(np.array_split(np.arange(r.nang), parts)):
+        out.append(serve.Request("%s/%d" % (r.key, k), r.pot, r.dz, r.ns, r.method, r.iv, r.G[idx], r.F[idx],
+                                 r.W[idx], r.nb0))
+    return out
+
+
+@pytest.mark.parametrize("precision,period", [(lib.PREC_FP64, 6), (lib.PREC_FP64, 0), (lib.PREC_MIXED, 6)])
+def test_merged_requests_same_bits_as_alone(engine, precision, period):
+    """Requests of one structure (an N = 97 scan cut into three) merge into one batch with a
+    request of another structure queued between them; every result is bit-identical to the
+    request served alone, and the merged ones share admission and completion."""
+    r97 = serve.structure_request(engine, _structure_with(97), dz2=0.1, key="m97")
+    r23 = serve.structure_request(engine, _structure_with(23), dz2=0.1, key="m23")
+    parts = _split(serve.Request(r97.key, r97.pot, r97.dz, r97.ns, r97.method, r97.iv, r97.G[:9], r97.F[:9],
+                                 r97.W[:9], r97.nb0), 3)
+    mix = [parts[0], r23, parts[1], parts[2]]
+    print("precision", precision, "period", period, "angles", [q.nang for q in mix], flush=True)
+    with lib.Engine(0, precision=precision) as eng:
+        eng.set_renorm_period(period)
+        alone = _alone(eng, mix)
+        eng.serve_set_merge(1000, 0.0)
+        run = serve.serve(eng, mix)
+    res = run["results"]
+    for a, b in zip(alone, [q[1] for q in res]):
+        assert np.array_equal(a, b)
+    # the three parts were admitted and finished together (equal service times), the N = 23
+    # request on its own
+    svc = [q[4] for q in res]
+    assert svc[0] == svc[2] == svc[3]
+    assert svc[1] != svc[0]
+
+
+def test_merging_waits_for_its_window_and_respects_the_budget(engine):
+    """A later request of the same structure joins the head within the merge window; under a
+    budget that holds one request but not two, a merged group is retried smaller and the
+    requests run one at a time, with the same results."""
+    r97 = serve.structure_request(engine, _structure_with(97), dz2=0.1, key="w97")
+    a, b = _split(serve.Request(r97.key, r97.pot, r97.dz, r97.ns, r97.method, r97.iv, r97.G[:8], r97.F[:8],
+                                r97.W[:8], r97.nb0), 2)
+    with lib.Engine(0) as eng:
+        eng.set_renorm_period(6)
+        alone = _alone(eng, [a, b])
+        eng.serve_set_merge(1000, 150.0)
+        run = serve.serve(eng, [a, b], arrivals=[0.0, 0.05])
+        with pytest.raises(lib.TrhepdError):
+            eng.serve_set_merge(-1, 0.0)
+    res = run["results"]
+    assert np.array_equal(res[0][1], alone[0]) and np.array_equal(res[1][1], alone[1])
+    assert res[0][4] == res[1][4]            # one batch: same service time
+    assert res[0][3] >= 150.0                # the head waited out its window
+    with lib.Engine(0) as probe:
+        probe.set_renorm_period(6)
+        probe.table_release(probe.table_build(a.pot, a.dz, a.ns, a.method))
+        base = probe.bytes_held
+        tid = probe.table_build(a.pot, a.dz, a.ns, a.method)
+        bid = probe.batch_create(tid, a.iv, a.G, a.F, a.nb0, a.W)
+        one = probe.bytes_held - base
+        probe.batch_release(bid)
+        bid = probe.batch_create(tid, a.iv, np.concatenate([a.G, b.G]), np.concatenate([a.F, b.F]), a.nb0,
+                                 np.concatenate([a.W, b.W]))
+        two = probe.bytes_held - base
+        probe.batch_release(bid)
+        probe.table_release(tid)
+    print("bytes: one request %d, the two merged %d" % (one, two))
+    assert two > one
+    with lib.Engine(0, hbm_budget=base + (one + two) // 2) as eng:
+        eng.set_renorm_period(6)
+        eng.serve_set_merge(1000, 0.0)
+        run = serve.serve(eng, [a, b])
+    assert run["peak_active"] == 1
+    assert np.array_equal(run["results"][0][1], alone[0]) and np.array_equal(run["results"][1][1], alone[1])

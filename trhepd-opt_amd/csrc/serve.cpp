@@ -9,7 +9,17 @@
 // reaches its last slice is copied back and its device memory released at once, which lets
 // the next queued request in.  Latency is measured from submit to the step that observed
 // completion (host steady clock).
+//
+// Merging (off unless trhepd_serve_set_merge enables it): queued requests of one structure
+// (identical potential descriptor, dz, ns, integrator, beam set and nb0) that would run
+// under the same renormalisation schedule are admitted together as one batch of all their
+// angles, up to a limit of angles, and split again at completion.  Every angle's
+// arithmetic is the same in a batch of any composition, so a merged request's results are
+// bit-identical to its results alone; merging only saves tables, launches and streams.  A
+// request may wait up to a window after its submission so that others can join it.
+#include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -64,16 +74,20 @@ constexpr int SLICES_PER_STEP = 16;  // slices one step may enqueue for a period
 struct Request {
     int id = 0, state = QUEUED;
     PotCopy pot;
+    double gim_max = 0.0;                // max |Im gamma| over its beams and angles
     double dz = 0;
     int ns = 0, method = 0, nb = 0, nang = 0, nb0 = 1;
     std::vector<int> iv;
     std::vector<double> gamma, f, w, f2;
     int table_id = 0, batch_id = 0;
+    int lead = 0;                        // merged: the request that holds the shared batch
+    int off = 0;                         // merged: first angle of this request in that batch
+    std::vector<int> members;            // the lead: every request of its batch, in angle order
     int slice = 0;                       // next slice to enqueue (1-based)
     hipEvent_t ev[RUN_AHEAD] = {};       // end of the last RUN_AHEAD slices
     Clock::time_point t_submit, t_admit, t_done;
     int rc = TRHEPD_OK;
-    long long n_renorm = 0;              // renormalisations summed over the request's angles
+    long long n_renorm = 0;              // renormalisations summed over the angles of its batch
     std::string err;
 };
 
@@ -85,7 +99,33 @@ struct Server {
     std::map<int, Request> reqs;
     int next_id = 1;
     size_t held_at_refusal = 0;   // device bytes held when the head of the queue last did not fit
+    int merge_angles = 0;         // merging: most angles of one merged batch (0: no merging)
+    double merge_window_ms = 0.0; // merging: how long a request may wait for others to join
 };
+
+// whether two requests run the same structure under the same renormalisation schedule
+static bool same_structure(const trhepd_ctx *ctx, const Request &a, const Request &b)
+{
+    const trhepd_potential &p = a.pot.p, &q = b.pot.p;
+    if (a.dz != b.dz || a.ns != b.ns || a.method != b.method || a.nb != b.nb || a.nb0 != b.nb0 || a.iv != b.iv)
+        return false;
+    if (p.nv != q.nv || p.nvb != q.nvb || p.natms != q.natms || p.natmb != q.natmb || p.nelm != q.nelm ||
+        p.nabr != q.nabr || p.nabi != q.nabi || p.cc != q.cc || p.smesh != q.smesh)
+        return false;
+    const PotCopy &x = a.pot, &y = b.pot;
+    if (x.ielm != y.ielm || x.z != y.z || x.dchg != y.dchg || x.ghk != y.ghk || x.ocr != y.ocr || x.b != y.b ||
+        x.asf != y.asf || x.st0 != y.st0 || x.st1 != y.st1 || x.st2 != y.st2)
+        return false;
+    // the derived periodic schedule depends on the batch's largest |Im gamma|: equal periods only
+    if (ctx->renorm_period < 0) {
+        auto period = [&](const Request &r) {
+            const double rate = r.gim_max * r.dz;
+            return rate > 0.0 ? std::max(1, (int)std::floor(std::log(1e3) / rate)) : r.ns;
+        };
+        if (period(a) != period(b)) return false;
+    }
+    return true;
+}
 
 static Server &server(trhepd_ctx *ctx)
 {
@@ -113,14 +153,33 @@ void server_destroy(trhepd_ctx *ctx)
     ctx->serve = nullptr;
 }
 
-// admits one request: builds its table and batch; E_BUDGET / E_NOMEM leave it queued
-static int admit(trhepd_ctx *ctx, Request &r)
+// admits a request with the requests merged into it (`group`: the lead first, then the others
+// in queue order; each holds only its own angles): builds one table and one batch of all their
+// angles; E_BUDGET / E_NOMEM leave them queued
+static int admit(trhepd_ctx *ctx, Server &S, const std::vector<int> &group)
 {
-    r.t_admit = Clock::now();
+    Request &r = S.reqs.at(group[0]);
+    const Clock::time_point now = Clock::now();
+    int nang = 0;
+    for (int id : group) nang += S.reqs.at(id).nang;
+    std::vector<double> gamma, f, w;
+    const double *pg = r.gamma.data(), *pf = r.f.data(), *pw = r.w.data();
+    if (group.size() > 1) {
+        gamma.reserve((size_t)2 * r.nb * nang);
+        f.reserve((size_t)2 * r.nb * r.nb * nang);
+        w.reserve(nang);
+        for (int id : group) {
+            const Request &m = S.reqs.at(id);
+            gamma.insert(gamma.end(), m.gamma.begin(), m.gamma.end());
+            f.insert(f.end(), m.f.begin(), m.f.end());
+            w.insert(w.end(), m.w.begin(), m.w.end());
+        }
+        pg = gamma.data();
+        pf = f.data();
+        pw = w.data();
+    }
     int rc = trhepd_table_build(ctx, &r.pot.p, r.dz, r.ns, r.method, &r.table_id);
-    if (!rc)
-        rc = trhepd_batch_create(ctx, r.table_id, r.nb, r.nang, r.iv.data(), r.gamma.data(), r.f.data(), r.nb0,
-                                 r.w.data(), &r.batch_id);
+    if (!rc) rc = trhepd_batch_create(ctx, r.table_id, r.nb, nang, r.iv.data(), pg, pf, r.nb0, pw, &r.batch_id);
     for (int k = 0; k < RUN_AHEAD && !rc; ++k) rc = hip_check(ctx, hipEventCreateWithFlags(&r.ev[k], hipEventDisableTiming), "event");
     if (!rc) {
         Batch *b = find_batch(ctx, r.batch_id);
@@ -134,26 +193,54 @@ static int admit(trhepd_ctx *ctx, Request &r)
             r.state = ACTIVE;
         }
     }
-    if (rc) drop_device_state(ctx, r);
-    return rc;
+    if (rc) {
+        drop_device_state(ctx, r);
+        return rc;
+    }
+    r.members = group;
+    int off = 0;
+    for (int id : group) {
+        Request &m = S.reqs.at(id);
+        m.t_admit = now;
+        m.off = off;
+        off += m.nang;
+        if (id != r.id) {
+            m.lead = r.id;
+            m.state = ACTIVE;
+        }
+    }
+    return TRHEPD_OK;
 }
 
-static int complete(trhepd_ctx *ctx, Request &r)
+// copies a finished batch back and hands every request of it its own angles
+static void complete(trhepd_ctx *ctx, Server &S, Request &r)
 {
-    r.f.assign((size_t)2 * r.nb * r.nb * r.nang, 0.0);
-    r.f2.assign((size_t)r.nb * r.nang, 0.0);
-    int rc = trhepd_batch_fetch(ctx, r.batch_id, r.f.data(), r.f2.data());
+    int nang = 0;
+    for (int id : r.members) nang += S.reqs.at(id).nang;
+    std::vector<double> f((size_t)2 * r.nb * r.nb * nang), f2((size_t)r.nb * nang);
+    int rc = trhepd_batch_fetch(ctx, r.batch_id, f.data(), f2.data());
     // counters of the run: [0] renormalisations, [1] exact zero pivots seen by the LU
     int cnt[16] = {};
     if (!rc) {
         const Batch *b = find_batch(ctx, r.batch_id);
         rc = hip_check(ctx, hipMemcpy(cnt, b->cnt, sizeof cnt, hipMemcpyDeviceToHost), "request counters");
     }
-    if (!rc) r.n_renorm = cnt[0];
     if (!rc && cnt[1]) rc = fail(ctx, TRHEPD_E_SINGULAR, "request %d: exact zero pivot in a renormalisation LU", r.id);
-    r.t_done = Clock::now();
+    const std::string err = rc ? ctx->err : std::string();
+    const Clock::time_point now = Clock::now();
     drop_device_state(ctx, r);
-    return rc;
+    for (int id : r.members) {
+        Request &m = S.reqs.at(id);
+        m.t_done = now;
+        m.n_renorm = cnt[0];
+        m.rc = rc;
+        m.err = err;
+        m.state = rc ? FAILED : DONE;
+        if (rc) continue;
+        const size_t fa = (size_t)2 * r.nb * r.nb, f2a = (size_t)r.nb;
+        m.f.assign(f.begin() + fa * m.off, f.begin() + fa * (m.off + m.nang));
+        m.f2.assign(f2.begin() + f2a * m.off, f2.begin() + f2a * (m.off + m.nang));
+    }
 }
 
 }  // namespace trhepd
@@ -184,6 +271,7 @@ int trhepd_serve_submit(trhepd_ctx *ctx, const trhepd_potential *pot, double dz,
     r.nb0 = nb0;
     r.iv.assign(iv, iv + (size_t)nb * nb);
     r.gamma.assign(gamma, gamma + (size_t)2 * nb * nang);
+    for (size_t e = 0; e < (size_t)nb * nang; ++e) r.gim_max = std::max(r.gim_max, std::fabs(gamma[2 * e + 1]));
     r.f.assign(f, f + (size_t)2 * nb * nb * nang);
     r.w.assign(wnsga2, wnsga2 + nang);
     r.t_submit = Clock::now();
@@ -207,16 +295,36 @@ int trhepd_serve_step(trhepd_ctx *ctx, int *n_active, int *n_queued, int *n_done
         const hipError_t q = hipEventQuery(b->ev1);
         if (q == hipErrorNotReady) continue;
         if (q != hipSuccess) return hip_check(ctx, q, "request completion");
-        r.rc = complete(ctx, r);
-        if (r.rc) r.err = ctx->err;
-        r.state = r.rc ? FAILED : DONE;
+        complete(ctx, S, r);
     }
     // 2. admission in arrival order while the HBM budget allows; after a refusal the head of
     //    the queue is tried again only once memory has been released
     while (!S.queue.empty() && !(S.held_at_refusal && ctx->held >= S.held_at_refusal)) {
         Request &r = S.reqs.at(S.queue.front());
+        // merging: the head may wait out its window for requests of its structure to join
+        std::vector<int> group{r.id};
+        if (S.merge_angles > 0) {
+            const double waited = std::chrono::duration<double, std::milli>(Clock::now() - r.t_submit).count();
+            if (waited < S.merge_window_ms) break;
+            int nang = r.nang;
+            for (size_t k = 1; k < S.queue.size(); ++k) {
+                const Request &m = S.reqs.at(S.queue[k]);
+                if (nang + m.nang > S.merge_angles || !same_structure(ctx, r, m)) continue;
+                group.push_back(m.id);
+                nang += m.nang;
+            }
+        }
         const size_t held = ctx->held;
-        const int rc = admit(ctx, r);
+        int rc = admit(ctx, S, group);
+        // a merged batch that does not fit is retried with fewer of the requests
+        while ((rc == TRHEPD_E_BUDGET || rc == TRHEPD_E_NOMEM) && group.size() > 1) {
+            group.resize(group.size() / 2);
+            rc = admit(ctx, S, group);
+        }
+        if (rc && rc != TRHEPD_E_BUDGET && rc != TRHEPD_E_NOMEM && group.size() > 1) {   // the head on its own
+            group.resize(1);
+            rc = admit(ctx, S, group);
+        }
         S.held_at_refusal = 0;
         if (rc == TRHEPD_E_BUDGET || rc == TRHEPD_E_NOMEM) {
             bool any = false;
@@ -231,8 +339,10 @@ int trhepd_serve_step(trhepd_ctx *ctx, int *n_active, int *n_queued, int *n_done
             r.err = ctx->err;
             r.state = FAILED;
             r.t_done = r.t_admit = Clock::now();
+            S.queue.pop_front();
+            continue;
         }
-        S.queue.pop_front();
+        for (int id : group) S.queue.erase(std::find(S.queue.begin(), S.queue.end(), id));
     }
     // 3. advance the admitted requests.  Requests under the reference's renormalisation rule
     //    need the host after every slice (the flags of the Gershgorin test): one slice each,
@@ -242,7 +352,7 @@ int trhepd_serve_step(trhepd_ctx *ctx, int *n_active, int *n_queued, int *n_done
     //    the most expensive one.  The host blocks only when no request could advance.
     std::vector<Request *> act, gated;
     for (auto &kv : S.reqs)
-        if (kv.second.state == ACTIVE) act.push_back(&kv.second);
+        if (kv.second.state == ACTIVE && !kv.second.lead) act.push_back(&kv.second);
     std::vector<Request *> host_rule;
     bool moved = false;
     for (Request *r : act) {
@@ -333,6 +443,17 @@ int trhepd_serve_result(trhepd_ctx *ctx, int request_id, double *f, double *f2, 
 }
 
 }  // extern "C"
+
+extern "C" int trhepd_serve_set_merge(trhepd_ctx *ctx, int max_angles, double window_ms)
+{
+    if (!ctx) return TRHEPD_E_ARG;
+    if (max_angles < 0 || !(window_ms >= 0.0))
+        return fail(ctx, TRHEPD_E_ARG, "serve_set_merge: max_angles %d, window %g ms", max_angles, window_ms);
+    Server &S = server(ctx);
+    S.merge_angles = max_angles;
+    S.merge_window_ms = window_ms;
+    return TRHEPD_OK;
+}
 
 extern "C" int trhepd_serve_last_timing(trhepd_ctx *ctx, double *queued_ms, double *service_ms)
 {

@@ -29,7 +29,8 @@ SYMBOLS = ("trhepd_ctx_create", "trhepd_ctx_destroy", "trhepd_last_error", "trhe
            "trhepd_batch_release", "trhepd_last_stats", "trhepd_set_renorm_period", "trhepd_set_mixed_digits", "trhepd_profile",
            "trhepd_kernel_stats", "trhepd_serve_submit", "trhepd_serve_step", "trhepd_serve_result",
            "trhepd_graph_create", "trhepd_graph_launch", "trhepd_graph_release", "trhepd_bulk_reflect",
-           "trhepd_propagate_batch_noalloc", "trhepd_set_bulk_batched_above", "trhepd_serve_last_timing")
+           "trhepd_propagate_batch_noalloc", "trhepd_set_bulk_batched_above", "trhepd_serve_last_timing",
+           "trhepd_serve_set_merge")
 
 
 class TrhepdError(RuntimeError):
@@ -99,6 +100,7 @@ def load() -> C.CDLL:
     L.trhepd_graph_release.argtypes = [vp, i]
     L.trhepd_serve_result.argtypes = [vp, i, vp, vp, dp]
     L.trhepd_serve_last_timing.argtypes = [vp, dp, dp]
+    L.trhepd_serve_set_merge.argtypes = [vp, i, d]
     for name in SYMBOLS:
         getattr(L, name).restype = getattr(L, name).restype or C.c_int
     L.trhepd_last_error.restype = C.c_char_p
@@ -287,6 +289,12 @@ class Engine:
         self._serve_shape = getattr(self, "_serve_shape", {})
         self._serve_shape[rid.value] = (nang, nb)
         return rid.value
+
+    def serve_set_merge(self, max_angles: int, window_ms: float = 0.0):
+        """Admit queued requests of one structure together as one batch of at most
+        `max_angles` angles, the head of the queue waiting `window_ms` after its submission
+        for others to join (0 angles: no merging)."""
+        self._chk(self._L.trhepd_serve_set_merge(self._h, int(max_angles), float(window_ms)))
 
     def serve_step(self):
         """One step of the request loop; returns (admitted, queued, finished-uncollected)."""
