@@ -86,19 +86,29 @@ def test_bulk_matches_oracle(engine, case):
 
 @pytest.mark.parametrize("path", PATHS)
 def test_bulk_521_well_conditioned_groups(path):
+    """test521's groups that the reference's own algorithm determines in FP64: those whose
+    blocks move by at most 1e-9 when one input moves by one ulp in the reference's program
+    (tests/golden/make_bulk521_sensitivity.py), compared with the recorded bulkP.b at 1e-8.
+    The other groups' blocks move by 1e-8 to O(1) under that perturbation (the recorded file
+    and a build of the same sources differ by as much), so their recorded values are no
+    oracle; they are checked to be finite and their errors printed."""
+    import json
+    resp = json.load(open(os.path.join(REF, "test521", "bulk_sensitivity.json")))["response"]
     with _engine_for(path) as engine:
         bp, job = _job(engine, "test521")
         try:
             blocks, layers = job.run()
         finally:
             job.release()
-    orc = OB.Bulk(bp).blocks()
     ref = bp.domains[0].blocks[0][0]
-    good = [g for g in range(len(ref)) if _err(orc[0][0][g], ref[g]) <= 2e-9]
+    assert len(resp) == len(ref) == len(blocks[0]) == 49
+    good = [g for g in range(49) if resp[g] <= 1e-9]
     assert len(good) >= 7
-    errs = {g: _err(blocks[0][g], ref[g]) for g in good}
-    print("test521 groups compared:", {g: "%.2e" % e for g, e in errs.items()})
-    assert max(errs.values()) <= 1e-8
+    errs = {g: _err(blocks[0][g], ref[g]) for g in range(49)}
+    print("test521 groups compared:", {g: "%.2e" % errs[g] for g in good})
+    print("others (error, one-ulp response):", {g: "%.1e / %.0e" % (errs[g], resp[g]) for g in range(49)
+                                                 if g not in good})
+    assert max(errs[g] for g in good) <= 1e-8
     assert all(np.isfinite(b).all() for b in blocks[0])
 
 
