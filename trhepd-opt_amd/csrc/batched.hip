@@ -188,11 +188,12 @@ __global__ void k_gather(Geo g, const int *__restrict__ gmap, const VVi *__restr
 // (the kick was merged into the previous end-of-slice kick)
 __global__ void k_stage1(Geo g, const int *__restrict__ gmap, const VVi *__restrict__ col,
                          const double2 *__restrict__ g2c, const int *__restrict__ xid, double alpha,
-                         double beta, double2 *__restrict__ Q, double2 *__restrict__ P)
+                         double beta, int only_xid, double2 *__restrict__ Q, double2 *__restrict__ P)
 {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= g.mat * g.A) return;
     const int a = (int)(e / g.mat);
+    if (only_xid && !xid[a]) return;   // its drift ran in the previous slice's end kick
     const int r = (int)(e % g.mat), i = r % g.NP, j = r / g.NP;
     if (i >= g.N || j >= g.N) return;
     if (xid[a]) {
@@ -227,6 +228,7 @@ struct StageArgs {
     const VVi *col;
     double alpha, beta;
     const double *coef;   // per-angle alpha (mode 2) or null
+    const int *xid;       // mode 0: angles whose drift is skipped (beta 0: Qn = Q), or null
     int mode;
     double *colsum;       // [A*NP/16][NP]
     double *diagabs;      // [A*NP]
@@ -334,6 +336,7 @@ __global__ __launch_bounds__(256, BN <= 48 ? 2 : 1) void k_stage_gemm(StageArgs 
         const int Rb = r0 + rb * 16;            // first row of the 16-row block
         const int a = Rb / g.NP, ib = Rb % g.NP;
         const double alpha = S.coef ? S.coef[a] : S.alpha;
+        const double beta = (S.xid && S.xid[a]) ? 0.0 : S.beta;
 #pragma unroll
         for (int cp = 0; cp < NB; cp += EB) {
             double2 qv[EB][4], pv[EB][4], c2v[EB];
@@ -369,7 +372,7 @@ __global__ __launch_bounds__(256, BN <= 48 ? 2 : 1) void k_stage_gemm(StageArgs 
                             p.y += alpha * (ci[rb][cb][r] + qd.y);
                             S.P[idx] = p;
                             if (S.mode != 2) {
-                                const double2 qn = make_double2(q.x + S.beta * p.x, q.y + S.beta * p.y);
+                                const double2 qn = make_double2(q.x + beta * p.x, q.y + beta * p.y);
                                 S.Qn[idx] = qn;
                                 if (S.mode == 1) {
                                     if (i == j) S.diagabs[(size_t)a * g.NP + j] = hypot(qn.x, qn.y);
@@ -1835,6 +1838,10 @@ struct Runner {
     hipEvent_t evf = nullptr;
     double2 *Q, *Qn;            // current / next Q buffer
     int forced = -1;            // this slice's renormalisation: -1 device decides, 0 no, 1 all
+    // RKN: the previous slice's end kick also ran the next slice's first drift (Q += c_a1 dz P)
+    // for the angles that did not renormalise, so the next first stage only serves the others
+    bool fused_prev = false;
+    bool fuse_drift = true;     // TRHEPD_FUSE_DRIFT=0 keeps the drift in the slice's first stage
     // sliced-integer stage products (mixed precision): digits of Q and of one slice's M0s
     bool mixed = false;
     MixedGeo mg;
@@ -1902,6 +1909,9 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
     // the Magnus and rk4gs products are all per-angle, and RK4's stage operands are formed
     // outside the stage kernel: they run in FP64 in either precision
     R->mixed = ctx->precision == TRHEPD_PREC_MIXED && !dense_method(t.method) && t.method != TRHEPD_RK4;
+    // TRHEPD_FUSE_DRIFT=0 runs every slice's first drift in its own pass (a test control: both
+    // forms give the same bits)
+    if (const char *fd = std::getenv("TRHEPD_FUSE_DRIFT")) R->fuse_drift = fd[0] != '0';
     size_t oqd = 0, omd = 0, oeq = 0, oem = 0, orm = 0;
     if (R->mixed) {
         // digit buffers sized for the widest padding any candidate tile needs
@@ -2222,6 +2232,7 @@ int runner_begin(trhepd_ctx *ctx, Batch &b, const Table &t)
     R.Q = R.Q0;
     R.Qn = R.Q1;
     R.rmax_ok = false;
+    R.fused_prev = false;
     if (t.method == TRHEPD_RK4GS) {   // invstrans, bcgs, x of power_est(-100) (surf_rk.f90:109-111)
         hipLaunchKernelGGL(k_gs_setup, eg, eb, 0, b.stream, g, (const double2 *)b.gamma, (const double2 *)b.fout, R.Q,
                            R.P, R.X);
@@ -2267,7 +2278,7 @@ static void gemm_launch(trhepd_ctx *ctx, Batch &b, StageArgs &S)
 
 // stage product number `st` of the slice (its digits, on the mixed path, are md[st])
 static void stage_launch(trhepd_ctx *ctx, Batch &b, const Table &t, int mode, const VVi *col, double alpha,
-                         double beta, const double *coef, int st)
+                         double beta, const double *coef, int st, const int *xid = nullptr)
 {
     Runner &R = *b.run;
     const Geo &g = R.g;
@@ -2290,6 +2301,7 @@ static void stage_launch(trhepd_ctx *ctx, Batch &b, const Table &t, int mode, co
         M.alpha = alpha;
         M.beta = beta;
         M.coef = coef;
+        M.xid = xid;
         M.mode = mode;
         M.colsum = R.colsum;
         M.diagabs = R.diagabs;
@@ -2317,6 +2329,7 @@ static void stage_launch(trhepd_ctx *ctx, Batch &b, const Table &t, int mode, co
     S.alpha = alpha;
     S.beta = beta;
     S.coef = coef;
+    S.xid = xid;
     S.mode = mode;
     S.colsum = R.colsum;
     S.diagabs = R.diagabs;
@@ -2347,6 +2360,7 @@ static void rk4_product(trhepd_ctx *ctx, Batch &b, const VVi *col, const double2
     S.alpha = 1.0;
     S.beta = 0.0;
     S.coef = nullptr;
+    S.xid = nullptr;
     S.mode = 2;
     S.colsum = R.colsum;
     S.diagabs = R.diagabs;
@@ -2451,8 +2465,12 @@ int runner_slice_enqueue(trhepd_ctx *ctx, Batch &b, const Table &t, int l)
 {
     Runner &R = *b.run;
     const Geo &g = R.g;
-    R.rmax_ok = false;   // the slice starts by changing Q outside the stage kernels
     const RknCoef co = rkn_coef(t.method);
+    // after an end kick that carried the drift, Q changes outside the stage kernels only for
+    // the renormalised angles; with none of them, the maxima the end kick gathered still hold
+    const bool rkn = t.method == TRHEPD_SP4 || t.method == TRHEPD_SP6;
+    const bool skip_first = rkn && R.fused_prev && R.forced == 0;
+    if (!skip_first) R.rmax_ok = false;
     const int s = co.s;
     const size_t ntot = g.mat * g.A;
     const dim3 eb(256), eg((unsigned)((ntot + 255) / 256));
@@ -2498,8 +2516,10 @@ int runner_slice_enqueue(trhepd_ctx *ctx, Batch &b, const Table &t, int l)
         const VVi *col1 = t.d + (size_t)(s * (l - 1)) * t.nv;
         // the s stage products of slice l use table columns s(l-1)+1 .. s l
         if (R.mixed) launch_msplit(b.stream, R.mg, b.gmap, col1 + t.nv, t.nv, s, R.md, R.em);
-        hipLaunchKernelGGL(k_stage1, eg, eb, 0, b.stream, g, b.gmap, col1, (const double2 *)R.g2c,
-                           (const int *)R.xid, co.bab[0] * t.dz, co.bab[1] * t.dz, R.Q, R.P);
+        if (!skip_first)
+            hipLaunchKernelGGL(k_stage1, eg, eb, 0, b.stream, g, b.gmap, col1, (const double2 *)R.g2c,
+                               (const int *)R.xid, co.bab[0] * t.dz, co.bab[1] * t.dz, R.fused_prev ? 1 : 0, R.Q,
+                               R.P);
         for (int k = 2; k <= s; ++k)
             stage_launch(ctx, b, t, k == s ? 1 : 0, t.d + (size_t)(s * (l - 1) + k - 1) * t.nv,
                          co.bab[2 * k - 2] * t.dz, co.bab[2 * k - 1] * t.dz, nullptr, k - 2);
@@ -2515,9 +2535,16 @@ int runner_slice_enqueue(trhepd_ctx *ctx, Batch &b, const Table &t, int l)
         TRY_HIP(ctx, hipEventRecord(R.evf, b.stream));
     }
     // RKN: the end-of-slice kick (merged with the next slice's first kick unless the angle
-    // renormalises)
-    if (t.method == TRHEPD_SP4 || t.method == TRHEPD_SP6)
-        stage_launch(ctx, b, t, 2, t.d + (size_t)(s * l) * t.nv, 0.0, 0.0, R.coef, s - 1);
+    // renormalises).  Unless every angle renormalises here, it also runs the next slice's
+    // first drift Q += c_a1 dz P for the angles that do not (per-angle beta 0 for the others,
+    // whose Q the renormalisation then reads unchanged).
+    if (rkn) {
+        const VVi *col = t.d + (size_t)(s * l) * t.nv;
+        const bool fuse = R.fuse_drift && R.forced != 1 && l != t.ns;
+        if (fuse) stage_launch(ctx, b, t, 0, col, 0.0, co.bab[1] * t.dz, R.coef, s - 1, R.xid);
+        else stage_launch(ctx, b, t, 2, col, 0.0, 0.0, R.coef, s - 1);
+        R.fused_prev = fuse;
+    }
     TRY_HIP(ctx, hipGetLastError());
     return TRHEPD_OK;
 }

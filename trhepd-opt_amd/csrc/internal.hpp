@@ -200,6 +200,7 @@ struct MixedStage {
     const VVi *col;
     double alpha, beta;
     const double *coef;
+    const int *xid;      // mode 0: angles whose drift is skipped (beta 0: Qn = Q), or null
     int mode;
     double *colsum, *diagabs;
     unsigned long long *rmax;   // mode 0/1: per-row maxima of Qn for the next split (bit patterns), or null

@@ -325,6 +325,7 @@ __global__ __launch_bounds__(64 * WR, (WR == 4 && ND * WC <= 24) ? 2 : 1) void k
     const int R = (rb0 + wave) * 16 + l16;
     const int a = R / g.NP, i = R % g.NP;
     const double alpha = S.coef ? S.coef[a] : S.alpha;
+    const double beta = (S.xid && S.xid[a]) ? 0.0 : S.beta;
     const int eqr = S.eq[R];
     double qmax = 0.0;   // this lane's maximum of cmax3(Qn) (mode 0/1)
     // The Q, P, conj(gamma^2) and column-scale loads of two column blocks are issued together
@@ -367,7 +368,7 @@ __global__ __launch_bounds__(64 * WR, (WR == 4 && ND * WC <= 24) ? 2 : 1) void k
                     p.y += alpha * (im[cb][r] * sc + qd.y);
                     S.P[idx] = p;
                     if (S.mode != 2) {
-                        const double2 qn = make_double2(q.x + S.beta * p.x, q.y + S.beta * p.y);
+                        const double2 qn = make_double2(q.x + beta * p.x, q.y + beta * p.y);
                         S.Qn[idx] = qn;
                         qmax = fmax(qmax, cmax3(qn));
                         if (S.mode == 1) {
