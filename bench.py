@@ -36,6 +36,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+# HBM bytes per launch of the int8 stage product at N = 521, 40 angles, 6 digits, from the
+# committed counter summary (rocprofv3 FETCH_SIZE and WRITE_SIZE over the kernel alone, each
+# scaled by its reading for a plain copy of known size)
+TRAFFIC_PROFILE = "profiles/k_stage_i8l_n521_40angles_counters.md"
+
+
+def stage_i8_traffic():
+    try:
+        with open(os.path.join(REPO, TRAFFIC_PROFILE)) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(st, blocks, method, dz2, threads, sample_slices):
     """The oracle on a bounded sample: `sample_slices` slices of one angle of the same
     deck (same N, same per-slice cost), scaled to a full point."""
@@ -335,9 +352,20 @@ def main():
         ach = ops / (avg_ms * 1e-3) / 1e12
         note = ("; concurrent batches overlap, so the per-launch duration is an upper bound on the kernel's own "
                 "time" if shared and r["nbatch"] > 1 else "; one batch, no concurrent stream")
-        return {"bound": "mfma", "kernel": kern + note,
-                "achieved": ach, "peak": peak, "unit": unit, "frac": ach / peak, "traffic": None,
-                "launches": r["launches"], "avg_launch_ms": avg_ms}
+        out = {"bound": "mfma", "kernel": kern + note,
+               "achieved": ach, "peak": peak, "unit": unit, "frac": ach / peak, "traffic": None,
+               "launches": r["launches"], "avg_launch_ms": avg_ms}
+        # the counted traffic is for the shape of the one-batch launch (40 angles of N = 521)
+        if mixed and args.mixed_digits == 6 and N == 521 and r["per_launch"] == 40:
+            out["traffic"] = stage_i8_traffic()
+            if out["traffic"] is not None:
+                alg = (3 * 6 * (r["per_launch"] * 528 // 16) * 9 * 1024   # Q digits
+                       + 3 * 6 * 33 * 9 * 1024                          # one stage's M0 digits
+                       + 64.0 * 528 * 528 * r["per_launch"])             # FP64 state read + written
+                out["traffic_source"] = (TRAFFIC_PROFILE + ": FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scale "
+                                         "factors from a 1 GiB copy), the kernel alone on seeded operands of "
+                                         "this shape; algorithmic bytes %.3g" % alg)
+        return out
 
     out = {
         "metric": METRIC, "value": head["value"], "unit": "points/s", "n_gpus": 1, "steps": args.steps,
