@@ -114,6 +114,15 @@ int trhepd_scpot_stf(trhepd_ctx *ctx, int natm, int nv, int nsg, int ma, int mb,
  * rk4_int_init surf_rk.f90:4-25).  The table stays resident; *table_id names it. */
 int trhepd_table_build(trhepd_ctx *ctx, const trhepd_potential *pot, double dz, int ns,
                        int method, int *table_id);
+/* The table of an RKN integrator (int_method 5 or 6) over a list of ns slices of thickness
+ * steps[0..ns-1], the first starting at depth z0 (0 for a whole slab): slice l starts at z0
+ * plus the thicknesses before it, its stage depths follow hp_int_init (surf_prkn.f90:187-195)
+ * with that slice's thickness.  trhepd_propagate_batch and the batch entry points run such a
+ * table like any other, each slice with its own thickness.  No counterpart in the reference,
+ * whose slices are uniform (surfio.f90:197-210); for slice lists chosen by
+ * trhepd_adaptive_steps. */
+int trhepd_table_build_steps(trhepd_ctx *ctx, const trhepd_potential *pot, int method, int ns,
+                             const double *steps, double z0, int *table_id);
 /* number of columns K and components nv of a table */
 int trhepd_table_shape(trhepd_ctx *ctx, int table_id, int *nv, int *ncols);
 /* copies a table back to the host: v, vi complex (nv, K) */
@@ -186,6 +195,29 @@ int trhepd_batch_run(trhepd_ctx *ctx, int batch_id);
 int trhepd_batch_run_many(trhepd_ctx *ctx, int n, const int *batch_ids);
 int trhepd_batch_fetch(trhepd_ctx *ctx, int batch_id, double *f, double *f2);
 int trhepd_batch_release(trhepd_ctx *ctx, int batch_id);
+
+/* ---- adaptive slice thickness (the unused controller of h211b_control.f90:1-16) ---- */
+
+/* h211b_limit (h211b_control.f90:1-7): 1 + atan(u - 1), the limiter of a step ratio. */
+double trhepd_h211b_limit(double u);
+/* h211b_filter (h211b_control.f90:9-16): c1^(1/(4k)) c0^(1/(4k)) rho^(-1/k), the H211b
+ * filter of the last two tolerance-to-error ratios c1 (latest), c0 and step ratio rho. */
+double trhepd_h211b_filter(double c1, double c0, double rho, int k);
+/* Chooses slice thicknesses for the slab [0, zmax] with the controller above: pilot angles
+ * (nb, nang, iv, gamma, f, nb0 as for trhepd_propagate_batch; 48 < nb <= 600) advance slice
+ * by slice on the device; each slice's error is the step-doubling estimate
+ * max over angles of max|P_h - P_h/2| / max|P_h/2| of the ratio-form state after one slice of
+ * h and after two of h/2 (both renormalised), and the slice is accepted when it is <= tol,
+ * or when h is hmin (counted in *nforced, which may be NULL).  The next step is
+ * h * h211b_limit(h211b_filter(tol/err, previous tol/err, h/h_previous, p + 1)) for the
+ * method's order p (4 for sp4, 6 for sp6), kept within [hmin, hmax]; a rejected slice is
+ * retried at that ratio, at most 0.9.  steps receives the *nsteps accepted thicknesses (at
+ * most max_steps; they sum to zmax).  No counterpart in the reference (it defines the
+ * controller but integrates uniform slices). */
+int trhepd_adaptive_steps(trhepd_ctx *ctx, const trhepd_potential *pot, int method, double zmax,
+                          double tol, double h0, double hmin, double hmax, int nb, int nang,
+                          const int *iv, const double *gamma, const double *f, int nb0,
+                          int max_steps, double *steps, int *nsteps, int *nforced);
 
 /* ---- captured runs: batches of uneven beam counts in one launch ------------------
  * No counterpart in the reference (it runs one deck per process).  trhepd_graph_create

@@ -66,6 +66,16 @@ def lib():
         L.orc_integrate.argtypes = [i, d, i, i, i, zp, zp, ip, zp, zp, ip]
         L.orc_integrate.restype = i
         L.orc_gamma.argtypes = [i, ip, ip, d, d, d, d, d, d, zp]
+        L.orc_table_depth_steps.argtypes = [i, dp, i, d, i]
+        L.orc_table_depth_steps.restype = d
+        L.orc_table_steps.argtypes = [C.c_void_p, i, dp, i, d, zp, zp]
+        L.orc_table_steps.restype = i
+        L.orc_integrate_steps.argtypes = [i, dp, i, i, i, zp, zp, ip, zp, zp, ip]
+        L.orc_integrate_steps.restype = i
+        L.orc_h211b_limit.argtypes = [d]
+        L.orc_h211b_limit.restype = d
+        L.orc_h211b_filter.argtypes = [d, d, d, i]
+        L.orc_h211b_filter.restype = d
         L.orc_intensity.argtypes = [i, zp, zp, i, d, dp]
         L.orc_matAbdivB.argtypes = [i, zp, i]
         L.orc_matAbdivB.restype = i
@@ -365,6 +375,68 @@ def load_case(path_dir: str, dz2: float, method: int) -> Surface:
     bp = deckio.BulkP.read(os.path.join(path_dir, "bulkP.b"))
     sd = deckio.SurfDeck.parse(open(os.path.join(path_dir, "surf.txt")).read(), bp.ndom)
     return setup_surface(bp, sd, dz2, method)
+
+
+def table_steps(surf: Surface, steps, z0: float = 0.0):
+    """The RKN table (int_method 5/6) over a list of slice thicknesses, as C-order (K, nv)."""
+    L = lib()
+    h = np.ascontiguousarray(steps, dtype=np.float64)
+    K = L.orc_table_cols(surf.method, len(h))
+    v = np.zeros((K, surf.nv), dtype=np.complex128)
+    vi = np.zeros((K, surf.nv), dtype=np.complex128)
+    s, keep = _pot_struct(surf.pot)
+    if L.orc_table_steps(C.byref(s), surf.method, h, len(h), float(z0), v.reshape(-1), vi.reshape(-1)) < 0:
+        raise ValueError("slice lists are for int_method 5 and 6")
+    return v, vi
+
+
+def table_depths_steps(method: int, steps, z0: float = 0.0) -> np.ndarray:
+    L = lib()
+    h = np.ascontiguousarray(steps, dtype=np.float64)
+    K = L.orc_table_cols(method, len(h))
+    return np.array([L.orc_table_depth_steps(method, h, len(h), float(z0), c) for c in range(K)])
+
+
+def integrate_steps(surf: Surface, steps, f: np.ndarray, g: np.ndarray, v, vi):
+    """One angle over a slice list (table from table_steps): (f_out (nb, nb), stats)."""
+    L = lib()
+    h = np.ascontiguousarray(steps, dtype=np.float64)
+    ff = np.asfortranarray(f, dtype=np.complex128).reshape(-1, order="F").copy()
+    st = np.zeros(2, dtype=np.int32)
+    ivf = np.ascontiguousarray(surf.iv.reshape(-1, order="F"))
+    if L.orc_integrate_steps(surf.method, h, len(h), surf.nb, surf.nv, ff, g, ivf, v.reshape(-1), vi.reshape(-1),
+                             st) != 0:
+        raise ValueError("slice lists are for int_method 5 and 6 with nb > 1")
+    return ff.reshape((surf.nb, surf.nb), order="F"), st
+
+
+def surface_curve_steps(surf: Surface, steps, points: Optional[List[int]] = None):
+    """Rocking-curve rows [(printed_angle, f2)] integrated over a slice list."""
+    h = np.ascontiguousarray(steps, dtype=np.float64)
+    v, vi = table_steps(surf, h)
+    L = lib()
+    rows = []
+    angs = surf.angles()
+    sel = range(len(angs)) if points is None else points
+    ivf = np.ascontiguousarray(surf.iv.reshape(-1, order="F"))
+    for p in sel:
+        r1, r2, ga, az = angs[p]
+        g = gamma(surf, ga, az)
+        ff = np.asfortranarray(surf.scatter_f(r1, r2), dtype=np.complex128).reshape(-1, order="F").copy()
+        st = np.zeros(2, dtype=np.int32)
+        if L.orc_integrate_steps(surf.method, h, len(h), surf.nb, surf.nv, ff, g, ivf, v.reshape(-1),
+                                 vi.reshape(-1), st) != 0:
+            raise ValueError("slice lists are for int_method 5 and 6 with nb > 1")
+        rows.append((surf.printed_angle(ga, az), intensity(surf, ff.reshape((surf.nb, surf.nb), order="F"), g, ga)))
+    return rows
+
+
+def h211b_limit(u: float) -> float:
+    return lib().orc_h211b_limit(float(u))
+
+
+def h211b_filter(c1: float, c0: float, rho: float, k: int) -> float:
+    return lib().orc_h211b_filter(float(c1), float(c0), float(rho), int(k))
 
 
 def rel_err(f2: np.ndarray, ref: np.ndarray) -> float:

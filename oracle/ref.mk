@@ -13,6 +13,8 @@
 #   surf.exe        sim-trhepd-rheed surface program (int_method 1-8 from stdin: "dz method")
 #   bulk.exe        sim-trhepd-rheed bulk program (writes bulkP.b)
 #   surf-lapack.exe str-lapack transfer-matrix surface program (stdin: "dz")
+#   libh211b.so     the step-size controller h211b_control.f90 alone (functions h211b_limit_,
+#                   h211b_filter_; no program calls it), for checking its restatements
 REF ?= /root/reference
 FLANG ?= /opt/rocm/bin/amdflang
 MKLDIR ?= /opt/conda/lib
@@ -30,7 +32,7 @@ SIM_BULK := bulkio bulk bulksub bulkm
 COMMON := asf strfac scpot trmatg gcmi
 LAP_SURF := surfio surf surfsub domainsum surfm
 
-all: $(OUT)/surf.exe $(OUT)/bulk.exe $(OUT)/surf-lapack.exe
+all: $(OUT)/surf.exe $(OUT)/bulk.exe $(OUT)/surf-lapack.exe $(OUT)/libh211b.so
 
 # one build directory per program family, so that each program's modules are its own
 define compile
@@ -49,6 +51,10 @@ $(OUT)/bulk.exe: $(foreach f,$(SIM_BULK) $(COMMON),$(SIM)/$(f).f90)
 $(OUT)/surf-lapack.exe: $(foreach f,$(LAP_SURF) $(COMMON),$(LAP)/$(f).f90)
 	$(call compile,$(OUT)/lapack-surf,$(LAP),$(LAP_SURF) $(COMMON))
 	cd $(OUT)/lapack-surf && $(FLANG) $(FFLAGS) -o $@ $(addsuffix .o,$(LAP_SURF) $(COMMON)) $(MKL)
+
+$(OUT)/libh211b.so: $(SIM)/h211b_control.f90
+	@mkdir -p $(OUT)/h211b
+	cd $(OUT)/h211b && $(FLANG) -O3 -fPIC -shared $(SIM)/h211b_control.f90 -o $@
 
 clean:
 	rm -rf $(OUT)

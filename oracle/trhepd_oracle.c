@@ -415,6 +415,47 @@ int orc_table(const orc_pot *p, int method, double dz, int ns, zc *v, zc *vi)
     return K;
 }
 
+/* depth of table column c of an RKN table over a slice list (int_method 5/6): slice l starts
+ * at z0 plus the thicknesses before it (summed in order), its stages follow hp_int_init
+ * (surf_prkn.f90:187-195) with its own thickness, the last column ends the last slice.
+ * No counterpart in the reference (uniform slices); the schedule the device builds. */
+double orc_table_depth_steps(int method, const double *steps, int ns, double z0, int c)
+{
+    int s = method == 5 ? 6 : 11;
+    const double *bab = method == 5 ? BAB4 : BAB6;
+    int l = c / s, k = c % s;
+    if (c == s * ns) { l = ns; k = 0; }
+    double zs = z0;
+    for (int j = 0; j < l; j++) zs = zs + steps[j];
+    double iz = zs;
+    for (int j = 0; j < k; j++) iz = iz + steps[l] * bab[2 * j + 1];
+    return iz;
+}
+
+int orc_table_steps(const orc_pot *p, int method, const double *steps, int ns, double z0, zc *v, zc *vi)
+{
+    if (method != 5 && method != 6) return -1;
+    int K = orc_table_cols(method, ns);
+#pragma omp parallel for schedule(dynamic)
+    for (int c = 0; c < K; c++)
+        orc_comp_scpot(p, orc_table_depth_steps(method, steps, ns, z0, c), v + (size_t)p->nv * c, vi + (size_t)p->nv * c);
+    return K;
+}
+
+/* h211b_limit: h211b_control.f90:1-7 */
+double orc_h211b_limit(double u)
+{
+    const double kappa = 1.0;
+    return 1.0 + kappa * atan((u - 1.0) / kappa);
+}
+
+/* h211b_filter: h211b_control.f90:9-16 */
+double orc_h211b_filter(double c1, double c0, double rho, int k)
+{
+    const int b = 4;
+    return pow(c1, 1.0 / (b * k)) * pow(c0, 1.0 / (b * k)) * pow(rho, -1.0 / k);
+}
+
 /* ------------------------------------------------------------------------- */
 /* dense helpers standing in for the BLAS/LAPACK calls of the reference */
 
@@ -537,9 +578,19 @@ static void stransh_out(int n, zc *t1, const zc *g, zc *f)
 /* statistics a caller may read after an integration */
 typedef struct { int n_renorm; int n_gemm; } orc_stats;
 
-/* hp_int_spa: surf_prkn.f90:278-330 (symplectic RKN in ratio form) */
+/* hp_int_spa: surf_prkn.f90:278-330 (symplectic RKN in ratio form).  steps (ns), when not
+ * NULL, gives each slice its own thickness (a table built by orc_table_steps): the stage
+ * weights take slice l's thickness, and the next slice's first kick, merged with this end
+ * kick (psave), is recomputed with its own thickness where the two differ. */
+static void hp_int_spa_steps(double dz, const double *steps, int ns, int nb, int nv, zc *f, const zc *g,
+                             const int *iv, const zc *v, const zc *vi, int s, const double *bab, orc_stats *st);
 static void hp_int_spa(double dz, int ns, int nb, int nv, zc *f, const zc *g, const int *iv,
                        const zc *v, const zc *vi, int s, const double *bab, orc_stats *st)
+{
+    hp_int_spa_steps(dz, NULL, ns, nb, nv, f, g, iv, v, vi, s, bab, st);
+}
+static void hp_int_spa_steps(double dz0, const double *steps, int ns, int nb, int nv, zc *f, const zc *g,
+                             const int *iv, const zc *v, const zc *vi, int s, const double *bab, orc_stats *st)
 {
     int n = nb;
     zc *t1 = malloc(sizeof(zc) * (size_t)n * 2 * n), *psave = malloc(sizeof(zc) * (size_t)n * n);
@@ -547,11 +598,17 @@ static void hp_int_spa(double dz, int ns, int nb, int nv, zc *f, const zc *g, co
     zc *Q = t1, *P = t1 + (size_t)n * n;
     invstransh(n, t1, f, g);
     int xid = 1;
+    double dzp = dz0;   /* the previous slice's thickness */
     for (int l = 1; l <= ns; l++) {
+        const double dz = steps ? steps[l - 1] : dz0;
         for (int k = 1; k <= s; k++) {
             double cb = bab[2 * k - 2], ca = bab[2 * k - 1];
             if (k > 1 || l == 1) orc_vvi2mate(n, v + (size_t)nv * (s * (l - 1) + k - 1), vi + (size_t)nv * (s * (l - 1) + k - 1), iv, g, m);
-            if (k == 1 && !xid && l > 1) {
+            if (k == 1 && !xid && l > 1 && dz != dzp) {
+                /* the first kick with this slice's thickness: m still holds the previous
+                 * slice's end matrix, at the depth this slice starts */
+                zgemm_nn(n, cb * dz, Q, n, m, n, 1.0, P, n); st->n_gemm++;
+            } else if (k == 1 && !xid && l > 1) {
                 for (size_t e = 0; e < (size_t)n * n; e++) P[e] += psave[e];
             } else if (xid) {
                 for (size_t e = 0; e < (size_t)n * n; e++) P[e] += (cb * dz) * m[e];
@@ -572,6 +629,7 @@ static void hp_int_spa(double dz, int ns, int nb, int nv, zc *f, const zc *g, co
         double d = orc_cond_est(n, Q, n);
         if (d > 1e3 || l == ns) { orc_matAbdivB(n, t1, n); xid = 1; st->n_renorm++; }
         else xid = 0;
+        dzp = dz;
     }
     stransh_out(n, t1, g, f);
     free(t1); free(psave); free(m);
@@ -955,6 +1013,17 @@ int orc_integrate(int method, double dz, int ns, int nb, int nv, zc *f, const zc
     case 8: rk4_intgs(dz, ns, nb, nv, f, g, iv, v, vi, &st); break;
     default: return -1;
     }
+    if (stats) { stats[0] = st.n_renorm; stats[1] = st.n_gemm; }
+    return 0;
+}
+
+/* the RKN integrators (int_method 5, 6; nb > 1) over a slice list (orc_table_steps) */
+int orc_integrate_steps(int method, const double *steps, int ns, int nb, int nv, zc *f, const zc *g,
+                        const int *iv, const zc *v, const zc *vi, int *stats)
+{
+    orc_stats st = {0, 0};
+    if ((method != 5 && method != 6) || nb < 2) return -1;
+    hp_int_spa_steps(steps[0], steps, ns, nb, nv, f, g, iv, v, vi, method == 5 ? 6 : 11, method == 5 ? BAB4 : BAB6, &st);
     if (stats) { stats[0] = st.n_renorm; stats[1] = st.n_gemm; }
     return 0;
 }

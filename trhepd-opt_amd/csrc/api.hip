@@ -126,7 +126,7 @@ RknCoef rkn_coef(int method)
 }
 
 // counters cleared, F reset to the uploaded input, start event
-static int batch_prelude(trhepd_ctx *ctx, Batch &b)
+int batch_prelude(trhepd_ctx *ctx, Batch &b)
 {
     const size_t nn = (size_t)b.nb * b.nb;
     b.stats = Stats();
@@ -187,7 +187,8 @@ int enqueue_batches(trhepd_ctx *ctx, const std::vector<Batch *> &bs, const std::
         std::vector<const Table *> gt;
         for (int j = i; j < n; ++j)
             if (!done[j] && !bs[j]->run && bs[j]->nb == bs[i]->nb && ts[j]->method == ts[i]->method &&
-                ts[j]->dz == ts[i]->dz && bs[j]->nb0 == bs[i]->nb0 && small_fusable(bs[j]->nb, ts[j]->method)) {
+                ts[j]->dz == ts[i]->dz && ts[j]->steps == ts[i]->steps && bs[j]->nb0 == bs[i]->nb0 &&
+                small_fusable(bs[j]->nb, ts[j]->method)) {
                 g.push_back(bs[j]);
                 gt.push_back(ts[j]);
                 done[j] = true;
@@ -393,6 +394,43 @@ int trhepd_table_build(trhepd_ctx *ctx, const trhepd_potential *pot, double dz, 
     t.method = method;
     t.ns = ns;
     t.dz = dz;
+    int rc = launch_table(ctx, pot, t);
+    if (rc) return rc;
+    t.id = ctx->next_table++;
+    ctx->tables.push_back(t);
+    *table_id = t.id;
+    return TRHEPD_OK;
+}
+
+int trhepd_table_build_steps(trhepd_ctx *ctx, const trhepd_potential *pot, int method, int ns, const double *steps,
+                             double z0, int *table_id)
+{
+    if (!ctx) return TRHEPD_E_ARG;
+    if (method != TRHEPD_SP4 && method != TRHEPD_SP6)
+        return fail(ctx, TRHEPD_E_ARG, "table_build_steps: slice lists are for int_method 5 and 6, not %d", method);
+    if (ns < 1 || !steps || !table_id) return fail(ctx, TRHEPD_E_ARG, "table_build_steps: no slices");
+    double hmax = 0.0;
+    for (int l = 0; l < ns; ++l) {
+        if (!(steps[l] > 0.0) || !std::isfinite(steps[l]))
+            return fail(ctx, TRHEPD_E_ARG, "table_build_steps: slice %d has thickness %g", l + 1, steps[l]);
+        hmax = std::max(hmax, steps[l]);
+    }
+    if (!std::isfinite(z0)) return fail(ctx, TRHEPD_E_ARG, "table_build_steps: z0 is not finite");
+    if (!pot || pot->nv < 1 || pot->nvb < 1 || pot->nvb > pot->nv || pot->nelm < 1 || pot->nabr < 0 ||
+        pot->nabr + pot->nabi > 6 || pot->natms < 0 || pot->natmb < 0 || !pot->ielm || !pot->z || !pot->dchg ||
+        !pot->ghk || !pot->ocr || !pot->b || !pot->asf || (pot->natms && !pot->st0) ||
+        (pot->natmb && (!pot->st1 || !pot->st2)))
+        return fail(ctx, TRHEPD_E_ARG, "table_build_steps: bad potential descriptor");
+    for (int i = 0; i < pot->natmb + pot->natms; ++i)
+        if (pot->ielm[i] < 1 || pot->ielm[i] > pot->nelm)
+            return fail(ctx, TRHEPD_E_ARG, "ielm(%d) = %d out of 1..%d", i + 1, pot->ielm[i], pot->nelm);
+    TRY_HIP(ctx, hipSetDevice(ctx->device));
+    Table t;
+    t.method = method;
+    t.ns = ns;
+    t.dz = hmax;
+    t.steps.assign(steps, steps + ns);
+    t.z0 = z0;
     int rc = launch_table(ctx, pot, t);
     if (rc) return rc;
     t.id = ctx->next_table++;

@@ -21,6 +21,7 @@
 #include <initializer_list>
 #include <type_traits>
 #include <utility>
+#include <vector>
 
 #include <cstdio>
 #include <cstdlib>
@@ -393,8 +394,11 @@ __global__ __launch_bounds__(256, BN <= 48 ? 2 : 1) void k_stage_gemm(StageArgs 
 }
 
 // Gershgorin estimate per angle; renormalise when > 1e3 or on the last slice.
+// coef: the end kick's weight per angle, cend (c_b1 times this slice's thickness) where the
+// angle renormalises, else cmerged (that plus c_b1 times the next slice's thickness: the
+// next slice's first kick merged in)
 __global__ void k_cond(Geo g, const double *__restrict__ colsum, const double *__restrict__ diagabs, int last,
-                       int forced, double cb1dz, double *__restrict__ coef, int *__restrict__ xid,
+                       int forced, double cend, double cmerged, double *__restrict__ coef, int *__restrict__ xid,
                        int *__restrict__ ren, int *__restrict__ counters)
 {
     __shared__ double smn[256], smx[256];
@@ -432,7 +436,7 @@ __global__ void k_cond(Geo g, const double *__restrict__ colsum, const double *_
         const int r = (forced >= 0 ? forced : (d > 1e3)) || last;
         ren[a] = r;
         xid[a] = r;
-        coef[a] = r ? cb1dz : 2.0 * cb1dz;
+        coef[a] = r ? cend : cmerged;
         if (r) atomicAdd(&counters[0], 1);
     }
 }
@@ -1986,6 +1990,62 @@ int runner_create(trhepd_ctx *ctx, Batch &b, const Table &t)
     return dense_method(t.method) ? TRHEPD_OK : autotune(ctx, b, t);
 }
 
+// ---------------------------------------------------------------------------
+// state access for the adaptive slice control (adaptive.hip): P of every angle (after a
+// renormalisation Q = I, so P is the whole state), and the step-doubling difference
+
+// per angle: max over the N x N part of max(|re|, |im|) of P - ref, over that of P
+__global__ void k_pdiff(Geo g, const double2 *__restrict__ P, const double2 *__restrict__ ref, double *__restrict__ out)
+{
+    __shared__ double sd[256], sp[256];
+    const int a = blockIdx.x, tid = threadIdx.x;
+    double dm = 0.0, pm = 0.0;
+    for (int e = tid; e < g.N * g.N; e += blockDim.x) {
+        const size_t idx = (size_t)a * g.mat + (e % g.N) + (size_t)(e / g.N) * g.NP;
+        const double2 p = P[idx], r = ref[idx];
+        dm = fmax(dm, fmax(fabs(p.x - r.x), fabs(p.y - r.y)));
+        pm = fmax(pm, fmax(fabs(p.x), fabs(p.y)));
+    }
+    sd[tid] = dm;
+    sp[tid] = pm;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (tid < o) {
+            sd[tid] = fmax(sd[tid], sd[tid + o]);
+            sp[tid] = fmax(sp[tid], sp[tid + o]);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) out[a] = sp[0] > 0.0 ? sd[0] / sp[0] : sd[0];
+}
+
+size_t runner_state_bytes(const Batch &b) { return b.run ? b.run->g.mat * b.run->g.A * sizeof(double2) : 0; }
+
+int runner_save_p(trhepd_ctx *ctx, Batch &b, double2 *dst)
+{
+    TRY_HIP(ctx, hipMemcpyAsync(dst, b.run->P, runner_state_bytes(b), hipMemcpyDeviceToDevice, b.stream));
+    return TRHEPD_OK;
+}
+
+int runner_load_p(trhepd_ctx *ctx, Batch &b, const double2 *src)
+{
+    TRY_HIP(ctx, hipMemcpyAsync(b.run->P, src, runner_state_bytes(b), hipMemcpyDeviceToDevice, b.stream));
+    return TRHEPD_OK;
+}
+
+int runner_p_difference(trhepd_ctx *ctx, Batch &b, const double2 *ref, double *work, double *out)
+{
+    const Runner &R = *b.run;
+    hipLaunchKernelGGL(k_pdiff, dim3(R.g.A), dim3(256), 0, b.stream, R.g, (const double2 *)R.P, ref, work);
+    std::vector<double> h(R.g.A);
+    TRY_HIP(ctx, hipMemcpyAsync(h.data(), work, (size_t)R.g.A * 8, hipMemcpyDeviceToHost, b.stream));
+    TRY_HIP(ctx, hipStreamSynchronize(b.stream));
+    double m = 0.0;
+    for (double v : h) m = (v > m || v != v) ? v : m;   // a NaN propagates
+    *out = m;
+    return TRHEPD_OK;
+}
+
 void runner_destroy(trhepd_ctx *ctx, Batch &b)
 {
     Runner *R = b.run;
@@ -2514,22 +2574,23 @@ int runner_slice_enqueue(trhepd_ctx *ctx, Batch &b, const Table &t, int l)
                            b.stream, g, (const double2 *)R.Q, R.colsum, R.diagabs);
     } else {
         const VVi *col1 = t.d + (size_t)(s * (l - 1)) * t.nv;
+        const double h = slice_dz(t, l);
         // the s stage products of slice l use table columns s(l-1)+1 .. s l
         if (R.mixed) launch_msplit(b.stream, R.mg, b.gmap, col1 + t.nv, t.nv, s, R.md, R.em);
         if (!skip_first)
             hipLaunchKernelGGL(k_stage1, eg, eb, 0, b.stream, g, b.gmap, col1, (const double2 *)R.g2c,
-                               (const int *)R.xid, co.bab[0] * t.dz, co.bab[1] * t.dz, R.fused_prev ? 1 : 0, R.Q,
-                               R.P);
+                               (const int *)R.xid, co.bab[0] * h, co.bab[1] * h, R.fused_prev ? 1 : 0, R.Q, R.P);
         for (int k = 2; k <= s; ++k)
-            stage_launch(ctx, b, t, k == s ? 1 : 0, t.d + (size_t)(s * (l - 1) + k - 1) * t.nv,
-                         co.bab[2 * k - 2] * t.dz, co.bab[2 * k - 1] * t.dz, nullptr, k - 2);
+            stage_launch(ctx, b, t, k == s ? 1 : 0, t.d + (size_t)(s * (l - 1) + k - 1) * t.nv, co.bab[2 * k - 2] * h,
+                         co.bab[2 * k - 1] * h, nullptr, k - 2);
     }
     // periodic schedule: the decision is known on the host, no flag round trip
     const int period = batch_period(ctx, b, t);
     R.forced = period > 0 ? ((l % period == 0) || l == t.ns ? 1 : 0) : -1;
+    const double cend = co.bab[0] * slice_dz(t, l);
     hipLaunchKernelGGL(k_cond, dim3(g.A), dim3(256), 0, b.stream, g, (const double *)R.colsum,
-                       (const double *)R.diagabs, l == t.ns ? 1 : 0, R.forced, co.bab[0] * t.dz, R.coef, R.xid,
-                       R.ren, b.cnt);
+                       (const double *)R.diagabs, l == t.ns ? 1 : 0, R.forced, cend,
+                       cend + co.bab[0] * slice_dz(t, l + 1), R.coef, R.xid, R.ren, b.cnt);
     if (R.forced < 0) {
         TRY_HIP(ctx, hipMemcpyAsync(R.h, R.ren, g.A * 4, hipMemcpyDeviceToHost, b.stream));
         TRY_HIP(ctx, hipEventRecord(R.evf, b.stream));
@@ -2541,7 +2602,7 @@ int runner_slice_enqueue(trhepd_ctx *ctx, Batch &b, const Table &t, int l)
     if (rkn) {
         const VVi *col = t.d + (size_t)(s * l) * t.nv;
         const bool fuse = R.fuse_drift && R.forced != 1 && l != t.ns;
-        if (fuse) stage_launch(ctx, b, t, 0, col, 0.0, co.bab[1] * t.dz, R.coef, s - 1, R.xid);
+        if (fuse) stage_launch(ctx, b, t, 0, col, 0.0, co.bab[1] * slice_dz(t, l + 1), R.coef, s - 1, R.xid);
         else stage_launch(ctx, b, t, 2, col, 0.0, 0.0, R.coef, s - 1);
         R.fused_prev = fuse;
     }

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <array>
 #include <cstdint>
 #include <cstdio>
@@ -35,10 +36,23 @@ struct Table {
     int nv = 0;
     int ncols = 0;
     int ns = 0;
-    double dz = 0.0;
+    double dz = 0.0;            // the slice thickness (the largest one of a slice list)
     VVi *d = nullptr;   // [ncols][nv]
     size_t bytes = 0;
+    // RKN tables built from a list of slice thicknesses (trhepd_table_build_steps): slice l
+    // (1-based) is steps[l-1] thick and starts at z0 + steps[0] + ... + steps[l-2]; empty for
+    // the uniform schedule.  d_steps is the same list on the device, behind the columns.
+    std::vector<double> steps;
+    double z0 = 0.0;
+    const double *d_steps = nullptr;
 };
+
+// thickness of slice l (1-based; l past the last slice gives the last one's)
+inline double slice_dz(const Table &t, int l)
+{
+    if (t.steps.empty()) return t.dz;
+    return t.steps[(size_t)std::min(std::max(l, 1), t.ns) - 1];
+}
 
 struct Stats {
     long long n_renorm = 0;
@@ -138,6 +152,8 @@ void noalloc_attach(trhepd_ctx *ctx);
 
 Table *find_table(trhepd_ctx *ctx, int id);
 Batch *find_batch(trhepd_ctx *ctx, int id);
+// counters cleared, F reset to the uploaded input, start event (the first step of batch_start)
+int batch_prelude(trhepd_ctx *ctx, Batch &b);
 // starts a batch run on its stream: counters cleared, F reset to the uploaded input, then
 // the boundary transform (large N) or the whole integration (small N)
 int batch_start(trhepd_ctx *ctx, Batch &b, const Table &t);
@@ -251,6 +267,12 @@ void launch_fill_i32(hipStream_t st, int *p, int n, int v);
 // picks the stage-product tile variant of a new batch (timed once per shape, then cached)
 int autotune(trhepd_ctx *ctx, Batch &b, const Table &t);
 void runner_destroy(trhepd_ctx *ctx, Batch &b);
+// adaptive slice control: P of every angle to / from a device buffer of runner_state_bytes,
+// and max over angles of max|P - ref| / max|P| (work: nang doubles on the device)
+size_t runner_state_bytes(const Batch &b);
+int runner_save_p(trhepd_ctx *ctx, Batch &b, double2 *dst);
+int runner_load_p(trhepd_ctx *ctx, Batch &b, const double2 *src);
+int runner_p_difference(trhepd_ctx *ctx, Batch &b, const double2 *ref, double *work, double *out);
 int runner_begin(trhepd_ctx *ctx, Batch &b, const Table &t);
 int runner_slice_enqueue(trhepd_ctx *ctx, Batch &b, const Table &t, int l);
 int runner_slice_finish(trhepd_ctx *ctx, Batch &b, const Table &t);

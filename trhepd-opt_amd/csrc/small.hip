@@ -41,6 +41,7 @@ struct SmallArgs {
     const VVi *table;
     int nv, ns, N, method;
     double dz;
+    const double *steps;    // slice thicknesses of an RKN table built from a slice list, or null
     RknCoef co;
     const int *gmap;        // (N*N) row-major signed map
     const double2 *gamma;   // [nang][N]
@@ -340,7 +341,7 @@ __device__ __forceinline__ void small_body(const SmallArgs &A, const RknCoef &co
     constexpr int LD = SLds<NP>::LD;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int N = A.N;
-    const double dz = A.dz;
+    const double dzu = A.dz;
     for (int e = tid; e < NP * LD; e += nt) {
         S.qr[e] = 0; S.qi[e] = 0; S.pr[e] = 0; S.pi[e] = 0; S.mr[e] = 0; S.mi[e] = 0;
     }
@@ -380,8 +381,8 @@ __device__ __forceinline__ void small_body(const SmallArgs &A, const RknCoef &co
     if (A.method == TRHEPD_LF) {   // X^H = I + (dz/2) Y^H  (surf_magnus.f90:63-67)
         for (int e = tid; e < N * N; e += nt) {
             const int i = e % N, j = e / N;
-            S.qr[i + j * LD] = (dz / 2) * S.pr[i + j * LD] + (i == j ? 1.0 : 0.0);
-            S.qi[i + j * LD] = (dz / 2) * S.pi[i + j * LD];
+            S.qr[i + j * LD] = (dzu / 2) * S.pr[i + j * LD] + (i == j ? 1.0 : 0.0);
+            S.qi[i + j * LD] = (dzu / 2) * S.pi[i + j * LD];
         }
         __syncthreads();
         xid = 0;
@@ -398,6 +399,9 @@ __device__ __forceinline__ void small_body(const SmallArgs &A, const RknCoef &co
         __syncthreads();
     }
     for (int l = 1; l <= A.ns; ++l) {
+        // this slice's thickness, and the next one's (the last slice's past the end)
+        const double dz = A.steps ? A.steps[l - 1] : dzu;
+        const double dzn = A.steps ? A.steps[l < A.ns ? l : A.ns - 1] : dzu;
         if (A.method == TRHEPD_RK4) {
             rk4_slice<NP>(S, A.table, A.nv, 2 * (l - 1), N, dz, xid);
         } else if (A.method == TRHEPD_LF) {
@@ -488,7 +492,8 @@ __device__ __forceinline__ void small_body(const SmallArgs &A, const RknCoef &co
         const double d = (mn < 0.0) ? 1e100 : mx / mn;
         const bool renorm = (A.period > 0 ? (l % A.period == 0) : (d > 1e3)) || (l == A.ns);
         if (rkn) {
-            gemm_acc<NP>(S, N, (renorm ? cb1 : 2.0 * cb1) * dz);
+            // the end kick, merged with the next slice's first kick unless the angle renormalises
+            gemm_acc<NP>(S, N, renorm ? cb1 * dz : cb1 * dz + cb1 * dzn);
             __syncthreads();
         }
         if (renorm) {
@@ -648,6 +653,7 @@ int run_small(trhepd_ctx *ctx, hipStream_t stream, int period, const Table &t, i
     A.N = nb;
     A.method = t.method;
     A.dz = t.dz;
+    A.steps = t.d_steps;
     A.co = rkn_coef(t.method);
     A.gmap = d_gmap;
     A.gamma = d_gamma;
@@ -658,6 +664,7 @@ int run_small(trhepd_ctx *ctx, hipStream_t stream, int period, const Table &t, i
     A.counters = d_counters;
     A.period = period;
     if (nb == 1 && t.method == TRHEPD_SP6) {
+        if (t.d_steps) return fail(ctx, TRHEPD_E_ARG, "the one-beam sp6 form runs uniform slices only");
         hipLaunchKernelGGL(k_spa1d, dim3((nang + 63) / 64), dim3(64), 0, stream, A, nang);
     } else if (nb <= 16) {
         hipLaunchKernelGGL(k_small<16>, dim3(nang), dim3(256), 0, stream, A);
@@ -689,14 +696,15 @@ int run_small_multi(trhepd_ctx *ctx, hipStream_t stream, const std::vector<Small
         M.base.N = nb;
         M.base.method = t0.method;
         M.base.dz = t0.dz;
+        M.base.steps = t0.d_steps;
         M.base.co = rkn_coef(t0.method);
         M.base.nb0 = bs[i0].nb0;
         M.nbatch = (int)(i1 - i0);
         int blocks = 0;
         for (size_t i = i0; i < i1; ++i) {
             const SmallBatch &b = bs[i];
-            if (b.nb != nb || b.t->method != t0.method || b.t->dz != t0.dz || b.nb0 != bs[i0].nb0 ||
-                !small_fusable(b.nb, b.t->method))
+            if (b.nb != nb || b.t->method != t0.method || b.t->dz != t0.dz || b.t->steps != t0.steps ||
+                b.nb0 != bs[i0].nb0 || !small_fusable(b.nb, b.t->method))
                 return fail(ctx, TRHEPD_E_ARG, "fused small-N launch: batches differ in N, method, dz or nb0");
             SmallPart &P = M.part[i - i0];
             P.table = b.t->d;

@@ -202,10 +202,30 @@ static void table_depths(int method, double dz, int ns, std::vector<double> &d)
     }
 }
 
+// depths of an RKN table built from a slice list: slice l starts at z0 plus the thicknesses
+// of the slices before it (summed in order), its stage depths follow as in hp_int_init
+// (surf_prkn.f90:187-195) with that slice's thickness, and the last column is the end of the
+// last slice
+static void table_depths_steps(int method, const std::vector<double> &steps, double z0, std::vector<double> &d)
+{
+    const RknCoef co = rkn_coef(method);
+    double zs = z0;
+    for (const double h : steps) {
+        double iz = zs;
+        for (int k = 0; k < co.s; ++k) {
+            d.push_back(iz);
+            iz = iz + h * co.bab[2 * k + 1];
+        }
+        zs = zs + h;
+    }
+    d.push_back(zs);
+}
+
 int launch_table(trhepd_ctx *ctx, const trhepd_potential *pot, Table &t)
 {
     std::vector<double> depth;
-    table_depths(t.method, t.dz, t.ns, depth);
+    if (t.steps.empty()) table_depths(t.method, t.dz, t.ns, depth);
+    else table_depths_steps(t.method, t.steps, t.z0, depth);
     t.ncols = (int)depth.size();
     const int natm = pot->natmb + pot->natms;
     if (pot->natms > MAX_ATOMS || pot->natmb > MAX_ATOMS)
@@ -241,7 +261,9 @@ int launch_table(trhepd_ctx *ctx, const trhepd_potential *pot, Table &t)
         dev_free(ctx, dd, off);
         return hip_check(ctx, e, "table upload");
     }
-    const size_t tbytes = (size_t)t.ncols * pot->nv * sizeof(VVi);
+    // the slice list (if any) follows the columns in the same allocation
+    const size_t cbytes = (size_t)t.ncols * pot->nv * sizeof(VVi);
+    const size_t tbytes = cbytes + t.steps.size() * sizeof(double);
     rc = dev_alloc(ctx, (void **)&t.d, tbytes);
     if (rc) {
         (void)hipStreamSynchronize(ctx->stream);
@@ -249,6 +271,18 @@ int launch_table(trhepd_ctx *ctx, const trhepd_potential *pot, Table &t)
         return rc;
     }
     t.bytes = tbytes;
+    if (!t.steps.empty()) {
+        t.d_steps = (const double *)((char *)t.d + cbytes);
+        e = hipMemcpyAsync((char *)t.d + cbytes, t.steps.data(), t.steps.size() * sizeof(double),
+                           hipMemcpyHostToDevice, ctx->stream);
+        if (e) {
+            (void)hipStreamSynchronize(ctx->stream);
+            dev_free(ctx, dd, off);
+            dev_free(ctx, t.d, tbytes);
+            t.d = nullptr;
+            return hip_check(ctx, e, "table slice list upload");
+        }
+    }
     t.nv = pot->nv;
     // comp_scpot: surface atoms at zi - cc, top bulk at zi, second bulk at zi + cc
     Part ps{pot->natmb, pot->natms, pot->nv, -pot->cc, 1.0, (const double2 *)(dd + o_st0)};

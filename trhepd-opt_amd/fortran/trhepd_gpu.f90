@@ -51,6 +51,41 @@ module trhepd_gpu
                         integer(c_int), value :: ns, method
                         integer(c_int), intent(out) :: table_id
                 end function
+                ! RKN table over a list of slice thicknesses (no counterpart in the reference)
+                integer(c_int) function trhepd_table_build_steps(ctx, pot, method, ns, steps, z0, table_id) &
+                                bind(C, name="trhepd_table_build_steps")
+                        import :: c_int, c_ptr, c_double, trhepd_potential
+                        type(c_ptr), value :: ctx
+                        type(trhepd_potential), intent(in) :: pot
+                        integer(c_int), value :: method, ns
+                        real(c_double), intent(in) :: steps(*)
+                        real(c_double), value :: z0
+                        integer(c_int), intent(out) :: table_id
+                end function
+                ! h211b_control.f90:1-16, callable from C and Fortran alike
+                real(c_double) function trhepd_h211b_limit(u) bind(C, name="trhepd_h211b_limit")
+                        import :: c_double
+                        real(c_double), value :: u
+                end function
+                real(c_double) function trhepd_h211b_filter(c1, c0, rho, k) bind(C, name="trhepd_h211b_filter")
+                        import :: c_int, c_double
+                        real(c_double), value :: c1, c0, rho
+                        integer(c_int), value :: k
+                end function
+                ! slice thicknesses for [0, zmax] chosen by the H211b controller on pilot angles
+                integer(c_int) function trhepd_adaptive_steps(ctx, pot, method, zmax, tol, h0, hmin, hmax, nb, nang, &
+                                iv, gamma, f, nb0, max_steps, steps, nsteps, nforced) &
+                                bind(C, name="trhepd_adaptive_steps")
+                        import :: c_int, c_ptr, c_double, c_double_complex, trhepd_potential
+                        type(c_ptr), value :: ctx
+                        type(trhepd_potential), intent(in) :: pot
+                        integer(c_int), value :: method, nb, nang, nb0, max_steps
+                        real(c_double), value :: zmax, tol, h0, hmin, hmax
+                        integer(c_int), intent(in) :: iv(*)
+                        complex(c_double_complex), intent(in) :: gamma(*), f(*)
+                        real(c_double), intent(out) :: steps(*)
+                        integer(c_int), intent(out) :: nsteps, nforced
+                end function
                 integer(c_int) function trhepd_table_release(ctx, table_id) bind(C, name="trhepd_table_release")
                         import :: c_int, c_ptr
                         type(c_ptr), value :: ctx

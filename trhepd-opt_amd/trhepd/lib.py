@@ -30,7 +30,18 @@ SYMBOLS = ("trhepd_ctx_create", "trhepd_ctx_destroy", "trhepd_last_error", "trhe
            "trhepd_kernel_stats", "trhepd_serve_submit", "trhepd_serve_step", "trhepd_serve_result",
            "trhepd_graph_create", "trhepd_graph_launch", "trhepd_graph_release", "trhepd_bulk_reflect",
            "trhepd_propagate_batch_noalloc", "trhepd_set_bulk_batched_above", "trhepd_serve_last_timing",
-           "trhepd_serve_set_merge")
+           "trhepd_serve_set_merge", "trhepd_table_build_steps", "trhepd_adaptive_steps", "trhepd_h211b_limit",
+           "trhepd_h211b_filter")
+
+
+def h211b_limit(u: float) -> float:
+    """The library's h211b_limit (h211b_control.f90:1-7)."""
+    return load().trhepd_h211b_limit(float(u))
+
+
+def h211b_filter(c1: float, c0: float, rho: float, k: int) -> float:
+    """The library's h211b_filter (h211b_control.f90:9-16)."""
+    return load().trhepd_h211b_filter(float(c1), float(c0), float(rho), int(k))
 
 
 class TrhepdError(RuntimeError):
@@ -101,6 +112,12 @@ def load() -> C.CDLL:
     L.trhepd_serve_result.argtypes = [vp, i, vp, vp, dp]
     L.trhepd_serve_last_timing.argtypes = [vp, dp, dp]
     L.trhepd_serve_set_merge.argtypes = [vp, i, d]
+    L.trhepd_table_build_steps.argtypes = [vp, C.POINTER(Potential), i, i, vp, d, ip]
+    L.trhepd_adaptive_steps.argtypes = [vp, C.POINTER(Potential), i, d, d, d, d, d, i, i, vp, vp, vp, i, i, vp, ip, ip]
+    L.trhepd_h211b_limit.argtypes = [d]
+    L.trhepd_h211b_limit.restype = d
+    L.trhepd_h211b_filter.argtypes = [d, d, d, i]
+    L.trhepd_h211b_filter.restype = d
     for name in SYMBOLS:
         getattr(L, name).restype = getattr(L, name).restype or C.c_int
     L.trhepd_last_error.restype = C.c_char_p
@@ -235,6 +252,33 @@ class Engine:
         tid = C.c_int(0)
         self._chk(self._L.trhepd_table_build(self._h, C.byref(P), dz, ns, method, C.byref(tid)))
         return tid.value
+
+    def table_build_steps(self, pot: dict, steps, method: int, z0: float = 0.0) -> int:
+        """Table of int_method 5 or 6 over a list of slice thicknesses, the first slice
+        starting at depth z0."""
+        P, keep = self._potential(pot)
+        h = np.ascontiguousarray(steps, dtype=np.float64)
+        tid = C.c_int(0)
+        self._chk(self._L.trhepd_table_build_steps(self._h, C.byref(P), method, len(h), _p(h), float(z0),
+                                                   C.byref(tid)))
+        return tid.value
+
+    def adaptive_steps(self, pot: dict, method: int, zmax: float, tol: float, h0: float, hmin: float, hmax: float,
+                       iv: np.ndarray, gamma: np.ndarray, f: np.ndarray, nb0: int, max_steps: int = 100000):
+        """Slice thicknesses for the slab [0, zmax] chosen by the H211b controller from
+        step-doubling estimates on the pilot angles (conventions of propagate_batch).
+        Returns (steps, number of slices accepted at hmin above tol)."""
+        P, keep = self._potential(pot)
+        nang, nb = gamma.shape
+        ivf = np.asfortranarray(iv, dtype=np.int32)
+        g = np.ascontiguousarray(gamma, dtype=np.complex128)
+        ff = np.ascontiguousarray(np.asarray(f, dtype=np.complex128).transpose(0, 2, 1))
+        out = np.zeros(max_steps)
+        n, nf = C.c_int(0), C.c_int(0)
+        self._chk(self._L.trhepd_adaptive_steps(self._h, C.byref(P), method, zmax, tol, h0, hmin, hmax, nb, nang,
+                                                _p(ivf), _p(g), _p(ff), nb0, max_steps, _p(out), C.byref(n),
+                                                C.byref(nf)))
+        return out[:n.value].copy(), nf.value
 
     # -- bulk stage -------------------------------------------------------------
     def bulk_reflect(self, tid: int, nbg, iv: np.ndarray, gamma: np.ndarray, phase, ml: int, epsb: float):
