@@ -143,3 +143,30 @@ def test_adaptive_pilot_n521(engine):
     # a list chosen from local estimates is not guaranteed to beat equal slices of the same
     # count; it must stay of the same accuracy
     assert out["adaptive"][1] < 3 * out["uniform"][1] + 1e-9
+
+
+def test_refusals(engine):
+    """Slice lists are for sp4/sp6 with positive finite thicknesses; the pilot needs the
+    batched path (N > 48) and consistent bounds."""
+    surf = O.load_case(os.path.join(REF, "test23"), 0.68, 5)
+    for steps, method in (([0.1, 0.0, 0.2], 5), ([0.1, -0.1], 5), ([0.1, float("nan")], 6), ([0.1, 0.2], 4),
+                          ([0.1, 0.2], 7)):
+        with pytest.raises(lib.TrhepdError) as e:
+            engine.table_build_steps(surf.pot, steps, method)
+        assert e.value.code == lib.E_ARG
+    angs, G, F, W = _inputs(surf)
+    with pytest.raises(lib.TrhepdError) as e:   # N = 23: the pilot needs the batched path
+        engine.adaptive_steps(surf.pot, 5, 1.0, 1e-6, 0.1, 0.01, 1.0, surf.iv, G[:1], F[:1], surf.nb0)
+    assert e.value.code == lib.E_ARG
+    big = O.load_case(os.path.join(REF, "test521"), 0.68, 5)
+    ab, Gb, Fb, Wb = _inputs(big)
+    for args in ((4, 1.0, 1e-6, 0.1, 0.01, 1.0), (5, 1.0, 1e-6, 0.1, 0.2, 1.0), (5, 1.0, 0.0, 0.1, 0.01, 1.0),
+                 (5, -1.0, 1e-6, 0.1, 0.01, 1.0)):
+        with pytest.raises(lib.TrhepdError) as e:
+            engine.adaptive_steps(big.pot, args[0], args[1], args[2], args[3], args[4], args[5], big.iv, Gb, Fb,
+                                  big.nb0)
+        assert e.value.code == lib.E_ARG
+    with pytest.raises(lib.TrhepdError) as e:   # more slices than max_steps
+        engine.adaptive_steps(big.pot, 5, big.dz * big.ns, 1e-6, 0.05, 0.05, 0.05, big.iv, Gb, Fb, big.nb0,
+                              max_steps=3)
+    assert e.value.code == lib.E_ARG
