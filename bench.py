@@ -3,7 +3,7 @@
 Workload (the headline configuration): a seeded synthetic Si-surface structure with N = 521
 beams, 12 surface atoms of Z = 14, 8, 1, 40 glancing angles 0.5-4.4 deg, sp4 (int_method 5)
 at dz = 0.02 A.  One step = one full rocking curve of that structure (the 40 angles as
-concurrent batches on separate streams: two for the FP64 path, three for the mixed path)
+concurrent batches on separate streams, two of 20 angles for both paths)
 with inputs (bulk reflection matrices, gamma, potential table) resident in HBM.  value = points (angles) per second of the mixed-precision path; the FP64 path is
 timed the same way.
 
