@@ -253,6 +253,16 @@ int trhepd_last_stats(const trhepd_ctx *ctx, long long *n_renorm, long long *n_p
  * later batch runs of the context. */
 int trhepd_set_renorm_period(trhepd_ctx *ctx, int period);
 
+/* Phase offsets between large-N batches run together (trhepd_batch_run_many).  No
+ * counterpart in the reference (one deck per process).  With a periodic renormalisation
+ * schedule every batch factorises at the same slices; with offsets on, the j-th of m
+ * batches starts j/m of the first batch's period later (its stream waits for the first
+ * batch's slice), so that its LU chains overlap the other batches' stage products instead
+ * of each other.  Results do not depend on it; captured runs (trhepd_graph_create) and the
+ * reference rule (period 0) start every batch at once.  mode: 1 on, 0 off, -1 the default
+ * (on for TRHEPD_PREC_MIXED contexts, off for FP64). */
+int trhepd_set_phase_offsets(trhepd_ctx *ctx, int mode);
+
 /* Digits per operand of the mixed-precision stage products (5, 6 or 7; default 7).  The
  * digits are base 254 (int8 range): each adds 8 bits, the normwise error of one stage
  * product is about digits * 254^-digits, and the int8 matrix work grows as

@@ -86,6 +86,42 @@ def test_concurrent_batches_match_single_batch(engine):
     assert np.array_equal(f2w, f2p)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_phase_offset_batches_match_single_batch(engine, mixed_engine, prec):
+    """Concurrent batches under a fixed renormalisation period (every 3rd of 15 slices),
+    with phase offsets on (the second and third batch start after the first one's slices 1
+    and 2) and off: bit-identical intensities to one batch holding all six angles."""
+    e = engine if prec == "fp64" else mixed_engine
+    surf = O.load_case(os.path.join(REF, "test521"), 0.68, 5)
+    a5, G5, F5, W5 = _inputs(surf)
+    scale = np.linspace(0.995, 1.005, 6)
+    Gb = np.array([G5[0] * s for s in scale])
+    Fb = np.array([F5[0]] * 6)
+    Wb = np.array([W5[0]] * 6)
+    tid = e.table_build(surf.pot, surf.dz, surf.ns, 5)
+    e.set_renorm_period(3)
+    res = {}
+    try:
+        whole = e.batch_create(tid, surf.iv, Gb, Fb, surf.nb0, Wb)
+        parts = [e.batch_create(tid, surf.iv, Gb[i:i + 2], Fb[i:i + 2], surf.nb0, Wb[i:i + 2]) for i in (0, 2, 4)]
+        e.batch_run(whole)
+        res["whole"] = e.batch_fetch(whole, want_f=False)[1]
+        for mode in (1, 0):
+            e.set_phase_offsets(mode)
+            e.batch_run_many(parts)
+            assert e.last_stats()["n_renorm"] == 6 * 5   # each angle at slices 3, 6, 9, 12, 15
+            res[mode] = np.concatenate([e.batch_fetch(p, want_f=False)[1] for p in parts])
+        for b in [whole] + parts:
+            e.batch_release(b)
+    finally:
+        e.set_phase_offsets(-1)
+        e.set_renorm_period(0)
+        e.table_release(tid)
+    assert np.all(np.isfinite(res["whole"]))
+    assert np.array_equal(res[1], res[0])
+    assert np.array_equal(res["whole"], res[1])
+
+
 def test_auto_periodic_renormalisation_n521(engine):
     """Automatic period at a coarse dz: the evanescent growth per slice is so fast that the
     derived period is one slice, as the reference's rule does on this deck."""

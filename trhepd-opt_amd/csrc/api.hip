@@ -195,13 +195,44 @@ int enqueue_batches(trhepd_ctx *ctx, const std::vector<Batch *> &bs, const std::
             }
         rc = start_small_group(ctx, g, gt);
     }
-    // slices of all large-N batches interleaved: while the host waits for one batch's
-    // renormalisation flags the others' kernels are already queued on their streams
-    for (int l = 1; l <= L && !rc; ++l) {
-        for (int i = 0; i < n && !rc; ++i)
-            if (bs[i]->run && l <= ts[i]->ns) rc = runner_slice_enqueue(ctx, *bs[i], *ts[i], l);
-        for (int i = 0; i < n && !rc; ++i)
-            if (bs[i]->run && l <= ts[i]->ns) rc = runner_slice_finish(ctx, *bs[i], *ts[i]);
+    // Phase offsets: with a fixed renormalisation period every batch factorises at the same
+    // slices, and batches that start together reach their LU chains (a few workgroups per
+    // launch, bound by the panel latency) together, leaving most CUs idle.  The j-th of m
+    // large-N batches therefore starts j/m of the first one's period later: its stream waits
+    // for the first batch's slice off[j], so that its LU chains fall among the other batches'
+    // stage products.  The batches' results do not depend on it.
+    std::vector<int> off(n, 0);
+    {
+        int lead = -1, m = 0;
+        for (int i = 0; i < n; ++i)
+            if (bs[i]->run) {
+                if (lead < 0) lead = i;
+                ++m;
+            }
+        const int period = lead >= 0 ? batch_period(ctx, *bs[lead], *ts[lead]) : 0;
+        const bool on = ctx->phase_offsets < 0 ? ctx->precision == TRHEPD_PREC_MIXED : ctx->phase_offsets > 0;
+        if (m > 1 && period > 1 && period < ts[lead]->ns && on && !ctx->capturing)
+            for (int i = lead + 1, j = 1; i < n; ++i)
+                if (bs[i]->run) off[i] = std::min((j++ * period + m / 2) / m, ts[lead]->ns - 1);
+        for (int i = 0; i < n; ++i) L = std::max(L, bs[i]->run ? ts[i]->ns + off[i] : 0);
+        for (int l = 1; l <= L && !rc; ++l) {
+            // slices of all large-N batches interleaved: while the host waits for one batch's
+            // renormalisation flags the others' kernels are already queued on their streams
+            for (int i = 0; i < n && !rc; ++i)
+                if (bs[i]->run && off[i] > 0 && l == off[i] + 1) {
+                    // the lead batch has queued slices 1 .. off[i] and their renormalisations
+                    rc = hip_check(ctx, hipEventRecord(bs[i]->evs, bs[lead]->stream), "phase event");
+                    if (!rc) rc = hip_check(ctx, hipStreamWaitEvent(bs[i]->stream, bs[i]->evs, 0), "phase wait");
+                }
+            for (int i = 0; i < n && !rc; ++i) {
+                const int li = l - off[i];
+                if (bs[i]->run && li >= 1 && li <= ts[i]->ns) rc = runner_slice_enqueue(ctx, *bs[i], *ts[i], li);
+            }
+            for (int i = 0; i < n && !rc; ++i) {
+                const int li = l - off[i];
+                if (bs[i]->run && li >= 1 && li <= ts[i]->ns) rc = runner_slice_finish(ctx, *bs[i], *ts[i]);
+            }
+        }
     }
     for (int i = 0; i < n && !rc; ++i)
         if (bs[i]->run) rc = runner_end(ctx, *bs[i], *ts[i]);
@@ -679,6 +710,14 @@ int trhepd_set_renorm_period(trhepd_ctx *ctx, int period)
     if (!ctx) return TRHEPD_E_ARG;
     if (period < -1) return fail(ctx, TRHEPD_E_ARG, "renormalisation period must be >= -1 (got %d)", period);
     ctx->renorm_period = period;
+    return TRHEPD_OK;
+}
+
+int trhepd_set_phase_offsets(trhepd_ctx *ctx, int mode)
+{
+    if (!ctx) return TRHEPD_E_ARG;
+    if (mode < -1 || mode > 1) return fail(ctx, TRHEPD_E_ARG, "phase offsets: mode -1, 0 or 1 (got %d)", mode);
+    ctx->phase_offsets = mode;
     return TRHEPD_OK;
 }
 

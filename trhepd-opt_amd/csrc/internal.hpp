@@ -132,6 +132,9 @@ struct trhepd_ctx {
     std::vector<trhepd::GraphRec> graphs;
     int next_graph = 1;
     bool capturing = false;   // a stream capture is open: no host synchronisation, no timing events
+    // batches run together start a fraction of the renormalisation period apart: 1 on, 0 off,
+    // -1 on for the mixed precision only
+    int phase_offsets = -1;
 };
 
 namespace trhepd {

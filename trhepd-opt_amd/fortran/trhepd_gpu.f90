@@ -107,6 +107,11 @@ module trhepd_gpu
                         type(c_ptr), value :: ctx
                         integer(c_int), value :: period
                 end function
+                integer(c_int) function trhepd_set_phase_offsets(ctx, mode) bind(C, name="trhepd_set_phase_offsets")
+                        import :: c_int, c_ptr
+                        type(c_ptr), value :: ctx
+                        integer(c_int), value :: mode
+                end function
                 integer(c_int) function trhepd_set_mixed_digits(ctx, digits) bind(C, name="trhepd_set_mixed_digits")
                         import :: c_int, c_ptr
                         type(c_ptr), value :: ctx

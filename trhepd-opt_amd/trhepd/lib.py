@@ -26,7 +26,7 @@ SYMBOLS = ("trhepd_ctx_create", "trhepd_ctx_destroy", "trhepd_last_error", "trhe
            "trhepd_asfcrr", "trhepd_srfghk", "trhepd_asfcef", "trhepd_scpot_stf",
            "trhepd_table_build", "trhepd_table_shape", "trhepd_table_fetch", "trhepd_table_release",
            "trhepd_propagate_batch", "trhepd_batch_create", "trhepd_batch_run", "trhepd_batch_run_many", "trhepd_batch_fetch",
-           "trhepd_batch_release", "trhepd_last_stats", "trhepd_set_renorm_period", "trhepd_set_mixed_digits", "trhepd_profile",
+           "trhepd_batch_release", "trhepd_last_stats", "trhepd_set_renorm_period", "trhepd_set_phase_offsets", "trhepd_set_mixed_digits", "trhepd_profile",
            "trhepd_kernel_stats", "trhepd_serve_submit", "trhepd_serve_step", "trhepd_serve_result",
            "trhepd_graph_create", "trhepd_graph_launch", "trhepd_graph_release", "trhepd_bulk_reflect",
            "trhepd_propagate_batch_noalloc", "trhepd_set_bulk_batched_above", "trhepd_serve_last_timing",
@@ -100,6 +100,7 @@ def load() -> C.CDLL:
     L.trhepd_batch_release.argtypes = [vp, i]
     L.trhepd_profile.argtypes = [vp, i]
     L.trhepd_set_renorm_period.argtypes = [vp, i]
+    L.trhepd_set_phase_offsets.argtypes = [vp, i]
     L.trhepd_set_mixed_digits.argtypes = [vp, i]
     L.trhepd_set_bulk_batched_above.argtypes = [vp, i]
     L.trhepd_kernel_stats.argtypes = [vp, C.POINTER(C.c_longlong), dp]
@@ -438,6 +439,11 @@ class Engine:
     def set_renorm_period(self, period: int):
         """0: the reference's Gershgorin rule; K > 0: every angle after every K-th slice."""
         self._chk(self._L.trhepd_set_renorm_period(self._h, int(period)))
+
+    def set_phase_offsets(self, mode: int):
+        """Start concurrent large-N batches a fraction of the renormalisation period apart:
+        1 on, 0 off, -1 the default (on for the mixed precision)."""
+        self._chk(self._L.trhepd_set_phase_offsets(self._h, int(mode)))
 
     def set_mixed_digits(self, digits: int):
         """Digits per operand of the mixed-precision stage products (5, 6 or 7)."""
