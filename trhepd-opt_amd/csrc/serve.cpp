@@ -3,9 +3,10 @@
 // A request is one structure (its potential table inputs, dz, ns, integrator, beam set) and
 // a set of glancing angles; it replaces one srfref angle loop (surflf.f90:79-153).  Requests
 // wait in arrival order until their device memory fits the context's HBM budget; an admitted
-// request owns a table, a batch and a stream, and every step of the loop advances all
-// admitted requests by one slice, so the kernels of different requests overlap on the device
-// and requests that arrive later join while earlier ones are in flight.  A request that
+// request owns a table, a batch and a stream, and every step of the loop advances the
+// admitted requests by one or more slices, so the kernels of different requests overlap on
+// the device and requests that arrive later join while earlier ones are in flight (a request
+// with far more remaining work than the smallest yields to it for a bounded time).  A request that
 // reaches its last slice is copied back and its device memory released at once, which lets
 // the next queued request in.  Latency is measured from submit to the step that observed
 // completion (host steady clock).
@@ -70,6 +71,8 @@ struct PotCopy {
 enum { QUEUED = 0, ACTIVE = 1, FINISHING = 2, DONE = 3, FAILED = 4 };
 constexpr int RUN_AHEAD = 4;        // slices a request may be enqueued ahead of the device
 constexpr int SLICES_PER_STEP = 16;  // slices one step may enqueue for a periodic-schedule request
+constexpr double HOLD_RATIO = 8.0;     // remaining work, relative to the smallest, above which a request yields
+constexpr double HOLD_MAX_MS = 2000.0; // longest a request yields in total
 
 struct Request {
     int id = 0, state = QUEUED;
@@ -85,6 +88,9 @@ struct Request {
     std::vector<int> members;            // the lead: every request of its batch, in angle order
     int slice = 0;                       // next slice to enqueue (1-based)
     hipEvent_t ev[RUN_AHEAD] = {};       // end of the last RUN_AHEAD slices
+    bool holding = false;                // yielding to requests with far less remaining work
+    Clock::time_point held_since;
+    double held_ms = 0.0;                // time yielded in earlier spells
     Clock::time_point t_submit, t_admit, t_done;
     int rc = TRHEPD_OK;
     long long n_renorm = 0;              // renormalisations summed over the angles of its batch
@@ -350,17 +356,47 @@ int trhepd_serve_step(trhepd_ctx *ctx, int *n_active, int *n_queued, int *n_done
     //    decision: each is given as many slices as its run-ahead events allow (up to
     //    SLICES_PER_STEP), without blocking, so that cheap requests are not held to the pace of
     //    the most expensive one.  The host blocks only when no request could advance.
+    //    Shortest remaining work first, within a bound: the kernels of all streams share a few
+    //    hardware queues, so one slice of a small request would wait behind several launches
+    //    of every large one.  A periodic-schedule request whose remaining stage work (angles x
+    //    N^3 x stages x slices left) exceeds HOLD_RATIO times the smallest active one's
+    //    enqueues nothing new (what it has queued still runs) until HOLD_MAX_MS of such
+    //    yielding in total, after which it is never held again.  Results do not depend on it.
     std::vector<Request *> act, gated;
     for (auto &kv : S.reqs)
         if (kv.second.state == ACTIVE && !kv.second.lead) act.push_back(&kv.second);
     std::vector<Request *> host_rule;
     bool moved = false;
-    for (Request *r : act) {
+    const Clock::time_point now = Clock::now();
+    std::vector<double> rem(act.size(), -1.0);
+    double mrem = HUGE_VAL;
+    for (size_t i = 0; i < act.size(); ++i) {
+        const Batch *b = find_batch(ctx, act[i]->batch_id);
+        const Table *t = find_table(ctx, act[i]->table_id);
+        if (!b->run || batch_period(ctx, *b, *t) <= 0) continue;
+        const double n = b->nb;
+        rem[i] = (double)b->nang * n * n * n * rkn_coef(t->method).s * (t->ns - act[i]->slice + 1);
+        mrem = std::min(mrem, rem[i]);
+    }
+    for (size_t i = 0; i < act.size(); ++i) {
+        Request *r = act[i];
         Batch *b = find_batch(ctx, r->batch_id);
         const Table *t = find_table(ctx, r->table_id);
-        if (!b->run || batch_period(ctx, *b, *t) <= 0) {
+        if (rem[i] < 0.0) {
             host_rule.push_back(r);
             continue;
+        }
+        const double held = r->held_ms + (r->holding ? std::chrono::duration<double, std::milli>(now - r->held_since).count() : 0.0);
+        if (rem[i] > HOLD_RATIO * mrem && held < HOLD_MAX_MS) {
+            if (!r->holding) {
+                r->holding = true;
+                r->held_since = now;
+            }
+            continue;
+        }
+        if (r->holding) {
+            r->held_ms = held;
+            r->holding = false;
         }
         for (int k = 0; k < SLICES_PER_STEP && r->state == ACTIVE; ++k) {
             // slice l may be enqueued once slice l - RUN_AHEAD has finished
