@@ -282,9 +282,13 @@ int trhepd_set_mixed_digits(trhepd_ctx *ctx, int digits);
  * trhepd_serve_step runs one step of the loop: requests whose last kernels have finished
  * are copied back and their device memory is released; queued requests are admitted in
  * arrival order while their table and batch fit the context's HBM budget (a request that
- * cannot fit even with nothing else admitted fails with TRHEPD_E_BUDGET); every admitted
- * request is advanced by one slice, each on its own stream, so their kernels overlap.
- * The counts of admitted, queued and finished-but-uncollected requests are returned.
+ * cannot fit even with nothing else admitted fails with TRHEPD_E_BUDGET); the admitted
+ * requests are advanced, each on its own stream, so their kernels overlap: one slice each
+ * under the reference's renormalisation rule, up to 16 under a periodic schedule (at most
+ * 4 slices ahead of the device).  A periodic-schedule request whose remaining stage work is
+ * more than 8 times the smallest active request's yields (enqueues nothing new) until it has
+ * yielded 2 s in total.  The counts of admitted, queued and finished-but-uncollected
+ * requests are returned.
  *
  * trhepd_serve_result returns TRHEPD_E_PENDING until the request has finished; then it
  * copies f (nb, nb, nang) and f2 (nb, nang) (either may be NULL), the latency from submit
