@@ -122,6 +122,40 @@ def test_phase_offset_batches_match_single_batch(engine, mixed_engine, prec):
     assert np.array_equal(res["whole"], res[1])
 
 
+def test_phase_offsets_with_batches_of_different_tables(mixed_engine):
+    """Phase offsets when the batches run different tables: the lead (dz 0.68, 15 slices)
+    is shorter than the batch that starts one slice behind it (dz 0.47, 21 slices); a third
+    (dz 0.68) starts three slices behind.  Each batch's intensities equal its run alone."""
+    e = mixed_engine
+    cases = [O.load_case(os.path.join(REF, "test521"), dz, 5) for dz in (0.68, 0.47, 0.68)]
+    tids, bids, alone = [], [], []
+    e.set_renorm_period(4)
+    try:
+        for k, surf in enumerate(cases):
+            a5, G5, F5, W5 = _inputs(surf)
+            G = np.array([G5[0] * (1.0 + 0.002 * k), G5[0] * (0.999 - 0.002 * k)])
+            F = np.array([F5[0]] * 2)
+            W = np.array([W5[0]] * 2)
+            tids.append(e.table_build(surf.pot, surf.dz, surf.ns, 5))
+            bids.append(e.batch_create(tids[-1], surf.iv, G, F, surf.nb0, W))
+        e.set_phase_offsets(1)
+        for b in bids:
+            e.batch_run(b)
+            alone.append(e.batch_fetch(b, want_f=False)[1])
+        e.batch_run_many(bids)
+        together = [e.batch_fetch(b, want_f=False)[1] for b in bids]
+        for b in bids:
+            e.batch_release(b)
+    finally:
+        e.set_phase_offsets(-1)
+        e.set_renorm_period(0)
+        for t in tids:
+            e.table_release(t)
+    for a, t in zip(alone, together):
+        assert np.all(np.isfinite(a))
+        assert np.array_equal(a, t)
+
+
 def test_auto_periodic_renormalisation_n521(engine):
     """Automatic period at a coarse dz: the evanescent growth per slice is so fast that the
     derived period is one slice, as the reference's rule does on this deck."""
