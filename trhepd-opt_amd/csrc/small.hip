@@ -92,6 +92,67 @@ __device__ void gather_m(SLds<NP> &S, const VVi *__restrict__ col, int N)
     }
 }
 
+// The next slice matrix of the RKN loop, loaded one stage ahead: the table entries of the
+// thread's elements of M (e = tid + 256 q is element (e % N, e / N); workgroups of 256
+// threads) are held in registers while the current stage runs, and m_commit writes M into
+// LDS with the values gather_m would.  Only LDS-only barriers lie between the two, so the
+// loads stay in flight across them.
+template <int NP>
+struct MNext {
+    static constexpr int E = (NP * NP + 255) / 256;
+    VVi t[E];
+    int g[E];
+};
+
+template <int NP>
+__device__ __forceinline__ void m_issue(const SLds<NP> &S, const VVi *__restrict__ col, int N, MNext<NP> &m)
+{
+#pragma unroll
+    for (int q = 0; q < MNext<NP>::E; ++q) {
+        const int e = threadIdx.x + 256 * q;
+        const bool in = e < N * N;
+        const int i = e % N, j = e / N;
+        const int g = in ? S.gm[i * N + j] : 0;
+        m.g[q] = g;
+        if (in) m.t[q] = col[g > 0 ? g - 1 : (g < 0 ? -g - 1 : 0)];
+    }
+}
+
+template <int NP>
+__device__ __forceinline__ void m_commit(SLds<NP> &S, int N, const MNext<NP> &m)
+{
+    constexpr int LD = SLds<NP>::LD;
+#pragma unroll
+    for (int q = 0; q < MNext<NP>::E; ++q) {
+        const int e = threadIdx.x + 256 * q;
+        if (e >= N * N) continue;
+        const int i = e % N, j = e / N, g = m.g[q];
+        const VVi t = m.t[q];
+        double vr, vi_;
+        if (g > 0) {
+            vr = -(t.v.x + t.vi.x);
+            vi_ = t.v.y + t.vi.y;
+        } else if (g < 0) {
+            vr = -t.v.x + t.vi.x;
+            vi_ = -t.v.y + t.vi.y;
+        } else {
+            vr = -(t.v.x + t.vi.x) - S.g2r[j];
+            vi_ = (t.v.y + t.vi.y) - S.g2i[j];
+        }
+        S.mr[i * LD + j] = vr;
+        S.mi[i * LD + j] = vi_;
+    }
+}
+
+// Workgroup barrier ordering LDS only: it does not wait for the wave's global loads in
+// flight, as a full __syncthreads() does on this target.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // P += alpha * Q * M on the matrix cores; each wave owns output blocks blk = wave + 4t.
 template <int NP>
 __device__ void gemm_acc(SLds<NP> &S, int N, double alpha)
@@ -398,6 +459,14 @@ __device__ __forceinline__ void small_body(const SmallArgs &A, const RknCoef &co
         }
         __syncthreads();
     }
+    MNext<NP> mpre;
+    int mcol = 0;   // the table column whose entries mpre holds (RKN)
+    const int mlast = s * A.ns;   // the last column of an RKN table
+    auto next_m = [&]() {
+        if (mcol < mlast) m_issue<NP>(S, A.table + (size_t)(mcol + 1) * A.nv, N, mpre);
+        ++mcol;
+    };
+    if (rkn) m_issue<NP>(S, A.table, N, mpre);
     for (int l = 1; l <= A.ns; ++l) {
         // this slice's thickness, and the next one's (the last slice's past the end)
         const double dz = A.steps ? A.steps[l - 1] : dzu;
@@ -432,13 +501,19 @@ __device__ __forceinline__ void small_body(const SmallArgs &A, const RknCoef &co
             }
             __syncthreads();
         } else {
+            // M of table column s (l - 1) + k - 1 for stage k, s l for the end kick: the
+            // columns are used in order, each loaded one stage ahead (mcol: the column whose
+            // entries mpre holds).  After a renormalisation the first kick's column is the
+            // previous end kick's, still in LDS.
             for (int k = 1; k <= s; ++k) {
                 const double cb = co.bab[2 * k - 2], ca = co.bab[2 * k - 1];
-                const VVi *col = A.table + (size_t)(s * (l - 1) + k - 1) * A.nv;
                 if (k == 1) {
                     if (xid) {
-                        gather_m<NP>(S, col, N);
-                        __syncthreads();
+                        if (mcol == 0) {   // the first slice: M not yet in LDS
+                            m_commit<NP>(S, N, mpre);
+                            next_m();
+                            lds_barrier();
+                        }
                         for (int e = tid; e < N * N; e += nt) {
                             const int i = e % N, j = e / N;
                             const double pr = S.pr[i + j * LD] + (cb * dz) * S.mr[i * LD + j];
@@ -455,18 +530,19 @@ __device__ __forceinline__ void small_body(const SmallArgs &A, const RknCoef &co
                             S.qi[i + j * LD] += (ca * dz) * S.pi[i + j * LD];
                         }
                     }
-                    __syncthreads();
+                    lds_barrier();
                 } else {
-                    gather_m<NP>(S, col, N);
-                    __syncthreads();
+                    m_commit<NP>(S, N, mpre);   // column s (l - 1) + k - 1
+                    next_m();
+                    lds_barrier();
                     gemm_acc<NP>(S, N, cb * dz);
-                    __syncthreads();
+                    lds_barrier();
                     for (int e = tid; e < N * N; e += nt) {
                         const int i = e % N, j = e / N;
                         S.qr[i + j * LD] += (ca * dz) * S.pr[i + j * LD];
                         S.qi[i + j * LD] += (ca * dz) * S.pi[i + j * LD];
                     }
-                    __syncthreads();
+                    lds_barrier();
                 }
             }
         }
@@ -482,8 +558,13 @@ __device__ __forceinline__ void small_body(const SmallArgs &A, const RknCoef &co
             S.red0[j] = dj - t;
             S.red1[j] = dj + t;
         }
-        if (rkn) gather_m<NP>(S, A.table + (size_t)(s * l) * A.nv, N);
-        __syncthreads();
+        if (rkn) {
+            m_commit<NP>(S, N, mpre);   // column s l, the end kick's
+            next_m();
+            lds_barrier();
+        } else {
+            __syncthreads();
+        }
         double mn = hypot(S.qr[0], S.qi[0]), mx = mn;
         for (int j = 0; j < N; ++j) {
             mn = fmin(mn, S.red0[j]);
@@ -494,7 +575,7 @@ __device__ __forceinline__ void small_body(const SmallArgs &A, const RknCoef &co
         if (rkn) {
             // the end kick, merged with the next slice's first kick unless the angle renormalises
             gemm_acc<NP>(S, N, renorm ? cb1 * dz : cb1 * dz + cb1 * dzn);
-            __syncthreads();
+            lds_barrier();
         }
         if (renorm) {
             lu_solve<NP>(S, N);
