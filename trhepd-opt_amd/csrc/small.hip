@@ -11,6 +11,7 @@
 // slice (psave, surf_prkn.f90:306-315).  cond_est reads only Q, which that kick does not
 // change, so the renormalisation decision is taken first and the two kicks are applied
 // as one product with coefficient 2*c_b1 when no renormalisation follows.
+#include "device.hpp"
 #include "internal.hpp"
 
 namespace trhepd {
@@ -142,15 +143,6 @@ __device__ __forceinline__ void m_commit(SLds<NP> &S, int N, const MNext<NP> &m)
         S.mr[i * LD + j] = vr;
         S.mi[i * LD + j] = vi_;
     }
-}
-
-// Workgroup barrier ordering LDS only: it does not wait for the wave's global loads in
-// flight, as a full __syncthreads() does on this target.
-__device__ __forceinline__ void lds_barrier()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 // P += alpha * Q * M on the matrix cores; each wave owns output blocks blk = wave + 4t.
@@ -315,12 +307,16 @@ __device__ void rk4_slice(SLds<NP> &S, const VVi *__restrict__ table, int nv, in
 }
 
 // [Q | P] <- [LU | Q^{-1} P]: zgetrf(N, 2N) with |re|+|im| partial pivoting, then
-// back substitution with the upper factor (matAbdivB, matcomp.f90:53-69).
+// back substitution with the upper factor (matAbdivB, matcomp.f90:53-69).  The back
+// substitution keeps P in registers (N <= 64: lane i holds row i, wave w columns w, w + 4,
+// ...) and needs no barrier.
 template <int NP>
 __device__ void lu_solve(SLds<NP> &S, int N)
 {
+    static_assert(NP <= 64, "a row per lane in the back substitution");
     constexpr int LD = SLds<NP>::LD;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nt = blockDim.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nt = blockDim.x, nw = nt >> 6;
+    const bool row = lane < N;
     for (int k = 0; k < N; ++k) {
         if (wave == 0) {
             double best = -1.0;
@@ -329,11 +325,7 @@ __device__ void lu_solve(SLds<NP> &S, int N)
                 const double v = fabs(S.qr[i + k * LD]) + fabs(S.qi[i + k * LD]);
                 if (v > best) { best = v; bi = i; }
             }
-            for (int off = 32; off > 0; off >>= 1) {
-                const double ob = __shfl_xor(best, off);
-                const int oi = __shfl_xor(bi, off);
-                if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
-            }
+            pivot_wave(best, bi);   // the first maximum, whatever the order of combination
             if (lane == 0) {
                 S.piv = bi;
                 if (best == 0.0) S.flag = 1;
@@ -377,17 +369,39 @@ __device__ void lu_solve(SLds<NP> &S, int N)
         }
         __syncthreads();
     }
-    // back substitution, one right-hand column per thread
-    for (int j = tid; j < N; j += nt) {
-        for (int k = N - 1; k >= 0; --k) {
-            const double2 x = cdiv(S.pr[k + j * LD], S.pi[k + j * LD], S.qr[k + k * LD], S.qi[k + k * LD]);
-            S.pr[k + j * LD] = x.x;
-            S.pi[k + j * LD] = x.y;
-            for (int i = 0; i < k; ++i) {
-                const double ur = S.qr[i + k * LD], ui = S.qi[i + k * LD];
-                S.pr[i + j * LD] -= x.x * ur - x.y * ui;
-                S.pi[i + j * LD] -= x.x * ui + x.y * ur;
+    // back substitution: X = U^{-1} P by columns, P(k, j) / U(k, k) as a product with the
+    // reciprocal of the pivot
+    constexpr int JQ = (NP + 3) / 4;
+    double xr[JQ], xi[JQ];
+#pragma unroll
+    for (int q = 0; q < JQ; ++q) {
+        const int j = wave + nw * q;
+        xr[q] = (row && j < N) ? S.pr[lane + j * LD] : 0.0;
+        xi[q] = (row && j < N) ? S.pi[lane + j * LD] : 0.0;
+    }
+    const double2 rd = row ? cdiv(1.0, 0.0, S.qr[lane * (LD + 1)], S.qi[lane * (LD + 1)]) : make_double2(0, 0);
+    for (int k = N - 1; k >= 0; --k) {
+        const double rr = readlane_f64(rd.x, k), ri = readlane_f64(rd.y, k);
+        const double ur = lane < k ? S.qr[lane + k * LD] : 0.0, ui = lane < k ? S.qi[lane + k * LD] : 0.0;
+#pragma unroll
+        for (int q = 0; q < JQ; ++q) {
+            const double pr = readlane_f64(xr[q], k), pi = readlane_f64(xi[q], k);
+            const double x_r = pr * rr - pi * ri, x_i = pr * ri + pi * rr;
+            if (lane == k) {
+                xr[q] = x_r;
+                xi[q] = x_i;
+            } else if (lane < k) {
+                xr[q] -= x_r * ur - x_i * ui;
+                xi[q] -= x_r * ui + x_i * ur;
             }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < JQ; ++q) {
+        const int j = wave + nw * q;
+        if (row && j < N) {
+            S.pr[lane + j * LD] = xr[q];
+            S.pi[lane + j * LD] = xi[q];
         }
     }
     __syncthreads();
