@@ -157,14 +157,50 @@ __device__ void gemm_acc(SLds<NP> &S, int N, double alpha)
         const int bi = blk / nbk, bj = blk - bi * nbk;
         const int ar = bi * 16 + l16, bc = bj * 16 + l16;
         d4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
-        for (int s = 0; s < ks; ++s) {
-            const int kk = s * 4 + kq;
-            const double a_r = S.qr[ar + kk * LD], a_i = S.qi[ar + kk * LD];
-            const double b_r = S.mr[kk * LD + bc], b_i = S.mi[kk * LD + bc];
-            cr = mfma4(a_r, b_r, cr);
-            cr = mfma4(-a_i, b_i, cr);
-            ci = mfma4(a_r, b_i, ci);
-            ci = mfma4(a_i, b_r, ci);
+        if constexpr (NP <= 32) {
+            // all k steps unrolled to the padded size, so that the LDS loads issue ahead of
+            // the products (steps past ks are skipped).  At NP = 48 the operands of 12 steps
+            // would not fit beside the kernel's other registers.
+            double a_r[NP / 4], a_i[NP / 4], b_r[NP / 4], b_i[NP / 4];
+#pragma unroll
+            for (int s = 0; s < NP / 4; ++s) {
+                const int kk = s * 4 + kq;
+                a_r[s] = S.qr[ar + kk * LD];
+                a_i[s] = S.qi[ar + kk * LD];
+                b_r[s] = S.mr[kk * LD + bc];
+                b_i[s] = S.mi[kk * LD + bc];
+            }
+#pragma unroll
+            for (int s = 0; s < NP / 4; ++s) {
+                if (s >= ks) break;
+                cr = mfma4(a_r[s], b_r[s], cr);
+                cr = mfma4(-a_i[s], b_i[s], cr);
+                ci = mfma4(a_r[s], b_i[s], ci);
+                ci = mfma4(a_i[s], b_r[s], ci);
+            }
+        } else {
+            // the operands of step s + 1 are loaded before the products of step s
+            auto ld = [&](int s, double &a_r, double &a_i, double &b_r, double &b_i) {
+                const int kk = s * 4 + kq;
+                a_r = S.qr[ar + kk * LD];
+                a_i = S.qi[ar + kk * LD];
+                b_r = S.mr[kk * LD + bc];
+                b_i = S.mi[kk * LD + bc];
+            };
+            double a_r, a_i, b_r, b_i;
+            ld(0, a_r, a_i, b_r, b_i);
+            for (int s = 0; s < ks; ++s) {
+                double n_r, n_i, m_r, m_i;
+                ld(min(s + 1, NP / 4 - 1), n_r, n_i, m_r, m_i);
+                cr = mfma4(a_r, b_r, cr);
+                cr = mfma4(-a_i, b_i, cr);
+                ci = mfma4(a_r, b_i, ci);
+                ci = mfma4(a_i, b_r, ci);
+                a_r = n_r;
+                a_i = n_i;
+                b_r = m_r;
+                b_i = m_i;
+            }
         }
         const int j = bj * 16 + l16;
 #pragma unroll
