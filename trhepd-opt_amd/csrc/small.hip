@@ -36,6 +36,9 @@ struct SLds {
     double red0[NP], red1[NP];
     int piv;
     int flag;
+    // NP = 48: each wave's share of the ninth output block of a stage product (d4 re, d4 im
+    // per lane)
+    double part[NP == 48 ? 4 * 8 * 64 : 1];
 };
 
 struct SmallArgs {
@@ -146,6 +149,8 @@ __device__ __forceinline__ void m_commit(SLds<NP> &S, int N, const MNext<NP> &m)
 }
 
 // P += alpha * Q * M on the matrix cores; each wave owns output blocks blk = wave + 4t.
+// At NP = 48 (nine blocks on four waves) wave w owns blocks w and w + 4 and the k steps
+// 3w .. 3w + 2 of the ninth, whose four partial products are added in wave order.
 template <int NP>
 __device__ void gemm_acc(SLds<NP> &S, int N, double alpha)
 {
@@ -153,14 +158,15 @@ __device__ void gemm_acc(SLds<NP> &S, int N, double alpha)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int nbk = (N + 15) >> 4, ks = (N + 3) >> 2;
     const int kq = lane >> 4, l16 = lane & 15;
-    for (int blk = wave; blk < nbk * nbk; blk += nw) {
-        const int bi = blk / nbk, bj = blk - bi * nbk;
+    // the product of block (bi, bj) over k steps [s0, s1)
+    auto prod = [&](int bi, int bj, int s0, int s1, d4 &cr, d4 &ci) {
         const int ar = bi * 16 + l16, bc = bj * 16 + l16;
-        d4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+        cr = d4{0, 0, 0, 0};
+        ci = d4{0, 0, 0, 0};
         if constexpr (NP <= 32) {
             // all k steps unrolled to the padded size, so that the LDS loads issue ahead of
-            // the products (steps past ks are skipped).  At NP = 48 the operands of 12 steps
-            // would not fit beside the kernel's other registers.
+            // the products (s0 is 0 here; steps from s1 on are skipped).  At NP = 48 the
+            // operands of 12 steps would not fit beside the kernel's other registers.
             double a_r[NP / 4], a_i[NP / 4], b_r[NP / 4], b_i[NP / 4];
 #pragma unroll
             for (int s = 0; s < NP / 4; ++s) {
@@ -172,7 +178,7 @@ __device__ void gemm_acc(SLds<NP> &S, int N, double alpha)
             }
 #pragma unroll
             for (int s = 0; s < NP / 4; ++s) {
-                if (s >= ks) break;
+                if (s >= s1) break;
                 cr = mfma4(a_r[s], b_r[s], cr);
                 cr = mfma4(-a_i[s], b_i[s], cr);
                 ci = mfma4(a_r[s], b_i[s], ci);
@@ -188,8 +194,8 @@ __device__ void gemm_acc(SLds<NP> &S, int N, double alpha)
                 b_i = S.mi[kk * LD + bc];
             };
             double a_r, a_i, b_r, b_i;
-            ld(0, a_r, a_i, b_r, b_i);
-            for (int s = 0; s < ks; ++s) {
+            ld(min(s0, NP / 4 - 1), a_r, a_i, b_r, b_i);
+            for (int s = s0; s < s1; ++s) {
                 double n_r, n_i, m_r, m_i;
                 ld(min(s + 1, NP / 4 - 1), n_r, n_i, m_r, m_i);
                 cr = mfma4(a_r, b_r, cr);
@@ -202,6 +208,8 @@ __device__ void gemm_acc(SLds<NP> &S, int N, double alpha)
                 b_i = m_i;
             }
         }
+    };
+    auto add = [&](int bi, int bj, const d4 &cr, const d4 &ci) {
         const int j = bj * 16 + l16;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -211,6 +219,43 @@ __device__ void gemm_acc(SLds<NP> &S, int N, double alpha)
                 S.pi[i + j * LD] += alpha * ci[r];
             }
         }
+    };
+    if (NP == 48 && nbk == 3 && nw == 4) {
+        d4 cr, ci;
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int blk = wave + 4 * t;
+            prod(blk / 3, blk % 3, 0, ks, cr, ci);
+            add(blk / 3, blk % 3, cr, ci);
+        }
+        prod(2, 2, min(3 * wave, ks), min(3 * wave + 3, ks), cr, ci);
+        double *pw = S.part + (size_t)wave * 8 * 64;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            pw[r * 64 + lane] = cr[r];
+            pw[(4 + r) * 64 + lane] = ci[r];
+        }
+        lds_barrier();
+        // wave w adds element r = w of every lane's share, in wave order
+        const int r = wave;
+        double sr = S.part[r * 64 + lane], si = S.part[(4 + r) * 64 + lane];
+#pragma unroll
+        for (int w = 1; w < 4; ++w) {
+            sr += S.part[(size_t)w * 8 * 64 + r * 64 + lane];
+            si += S.part[(size_t)w * 8 * 64 + (4 + r) * 64 + lane];
+        }
+        const int i = 32 + kq + 4 * r, j = 32 + l16;
+        if (i < N && j < N) {
+            S.pr[i + j * LD] += alpha * sr;
+            S.pi[i + j * LD] += alpha * si;
+        }
+        return;
+    }
+    for (int blk = wave; blk < nbk * nbk; blk += nw) {
+        const int bi = blk / nbk, bj = blk - bi * nbk;
+        d4 cr, ci;
+        prod(bi, bj, 0, ks, cr, ci);
+        add(bi, bj, cr, ci);
     }
 }
 
