@@ -96,47 +96,62 @@ __device__ void gather_m(SLds<NP> &S, const VVi *__restrict__ col, int N)
     }
 }
 
-// The next slice matrix of the RKN loop, loaded one stage ahead: the table entries of the
-// thread's elements of M (e = tid + 256 q is element (e % N, e / N); workgroups of 256
-// threads) are held in registers while the current stage runs, and m_commit writes M into
-// LDS with the values gather_m would.  Only LDS-only barriers lie between the two, so the
-// loads stay in flight across them.
+// The thread's elements of an N x N matrix, e = tid + 256 q = i + j N (workgroups of 256
+// threads), worked out once per integration: ij = i + 256 j (-1 past N^2), and for the
+// slice matrix the table entry of M(i, j) and its form, gk = 4 (|g| - 1) + 2 (upper), + 1
+// (lower), + 0 (diagonal, entry 0).
 template <int NP>
-struct MNext {
+struct Elems {
     static constexpr int E = (NP * NP + 255) / 256;
-    VVi t[E];
-    int g[E];
+    int ij[E], gk[E];
 };
 
 template <int NP>
-__device__ __forceinline__ void m_issue(const SLds<NP> &S, const VVi *__restrict__ col, int N, MNext<NP> &m)
+__device__ __forceinline__ void elems_init(const SLds<NP> &S, int N, Elems<NP> &el)
 {
 #pragma unroll
-    for (int q = 0; q < MNext<NP>::E; ++q) {
+    for (int q = 0; q < Elems<NP>::E; ++q) {
         const int e = threadIdx.x + 256 * q;
         const bool in = e < N * N;
         const int i = e % N, j = e / N;
         const int g = in ? S.gm[i * N + j] : 0;
-        m.g[q] = g;
-        if (in) m.t[q] = col[g > 0 ? g - 1 : (g < 0 ? -g - 1 : 0)];
+        el.ij[q] = in ? i + 256 * j : -1;
+        el.gk[q] = g > 0 ? 4 * (g - 1) + 2 : (g < 0 ? 4 * (-g - 1) + 1 : 0);
     }
 }
 
+// The next slice matrix of the RKN loop, loaded one stage ahead: the table entries of the
+// thread's elements of M are held in registers while the current stage runs, and m_commit
+// writes M into LDS with the values gather_m would.  Only LDS-only barriers lie between
+// the two, so the loads stay in flight across them.
 template <int NP>
-__device__ __forceinline__ void m_commit(SLds<NP> &S, int N, const MNext<NP> &m)
+struct MNext {
+    VVi t[Elems<NP>::E];
+};
+
+template <int NP>
+__device__ __forceinline__ void m_issue(const VVi *__restrict__ col, const Elems<NP> &el, MNext<NP> &m)
+{
+    // every load is issued (past N^2 entry 0, not used): a load under a branch would be
+    // waited for where the branches join
+#pragma unroll
+    for (int q = 0; q < Elems<NP>::E; ++q) m.t[q] = col[el.gk[q] >> 2];
+}
+
+template <int NP>
+__device__ __forceinline__ void m_commit(SLds<NP> &S, const Elems<NP> &el, const MNext<NP> &m)
 {
     constexpr int LD = SLds<NP>::LD;
 #pragma unroll
-    for (int q = 0; q < MNext<NP>::E; ++q) {
-        const int e = threadIdx.x + 256 * q;
-        if (e >= N * N) continue;
-        const int i = e % N, j = e / N, g = m.g[q];
+    for (int q = 0; q < Elems<NP>::E; ++q) {
+        if (el.ij[q] < 0) continue;
+        const int i = el.ij[q] & 255, j = el.ij[q] >> 8, form = el.gk[q] & 3;
         const VVi t = m.t[q];
         double vr, vi_;
-        if (g > 0) {
+        if (form == 2) {
             vr = -(t.v.x + t.vi.x);
             vi_ = t.v.y + t.vi.y;
-        } else if (g < 0) {
+        } else if (form == 1) {
             vr = -t.v.x + t.vi.x;
             vi_ = -t.v.y + t.vi.y;
         } else {
@@ -554,14 +569,22 @@ __device__ __forceinline__ void small_body(const SmallArgs &A, const RknCoef &co
         }
         __syncthreads();
     }
+    Elems<NP> el;
+    elems_init<NP>(S, N, el);
     MNext<NP> mpre;
     int mcol = 0;   // the table column whose entries mpre holds (RKN)
     const int mlast = s * A.ns;   // the last column of an RKN table
-    auto next_m = [&]() {
-        if (mcol < mlast) m_issue<NP>(S, A.table + (size_t)(mcol + 1) * A.nv, N, mpre);
+    auto next_m = [&]() {   // past the last column the last one again (not used)
+        m_issue<NP>(A.table + (size_t)min(mcol + 1, mlast) * A.nv, el, mpre);
         ++mcol;
     };
-    if (rkn) m_issue<NP>(S, A.table, N, mpre);
+    if (rkn) m_issue<NP>(A.table, el, mpre);
+    // f(i, j) for each of the thread's elements, as `for (e = tid; e < N * N; e += 256)`
+    auto each = [&](auto f) {
+#pragma unroll
+        for (int q = 0; q < Elems<NP>::E; ++q)
+            if (el.ij[q] >= 0) f(el.ij[q] & 255, el.ij[q] >> 8);
+    };
     for (int l = 1; l <= A.ns; ++l) {
         // this slice's thickness, and the next one's (the last slice's past the end)
         const double dz = A.steps ? A.steps[l - 1] : dzu;
@@ -605,38 +628,35 @@ __device__ __forceinline__ void small_body(const SmallArgs &A, const RknCoef &co
                 if (k == 1) {
                     if (xid) {
                         if (mcol == 0) {   // the first slice: M not yet in LDS
-                            m_commit<NP>(S, N, mpre);
+                            m_commit<NP>(S, el, mpre);
                             next_m();
                             lds_barrier();
                         }
-                        for (int e = tid; e < N * N; e += nt) {
-                            const int i = e % N, j = e / N;
+                        each([&](int i, int j) {
                             const double pr = S.pr[i + j * LD] + (cb * dz) * S.mr[i * LD + j];
                             const double pi = S.pi[i + j * LD] + (cb * dz) * S.mi[i * LD + j];
                             S.pr[i + j * LD] = pr;
                             S.pi[i + j * LD] = pi;
                             S.qr[i + j * LD] = (ca * dz) * pr + (i == j ? 1.0 : 0.0);
                             S.qi[i + j * LD] = (ca * dz) * pi;
-                        }
+                        });
                     } else {
-                        for (int e = tid; e < N * N; e += nt) {
-                            const int i = e % N, j = e / N;
+                        each([&](int i, int j) {
                             S.qr[i + j * LD] += (ca * dz) * S.pr[i + j * LD];
                             S.qi[i + j * LD] += (ca * dz) * S.pi[i + j * LD];
-                        }
+                        });
                     }
                     lds_barrier();
                 } else {
-                    m_commit<NP>(S, N, mpre);   // column s (l - 1) + k - 1
+                    m_commit<NP>(S, el, mpre);   // column s (l - 1) + k - 1
                     next_m();
                     lds_barrier();
                     gemm_acc<NP>(S, N, cb * dz);
                     lds_barrier();
-                    for (int e = tid; e < N * N; e += nt) {
-                        const int i = e % N, j = e / N;
+                    each([&](int i, int j) {
                         S.qr[i + j * LD] += (ca * dz) * S.pr[i + j * LD];
                         S.qi[i + j * LD] += (ca * dz) * S.pi[i + j * LD];
-                    }
+                    });
                     lds_barrier();
                 }
             }
@@ -654,7 +674,7 @@ __device__ __forceinline__ void small_body(const SmallArgs &A, const RknCoef &co
             S.red1[j] = dj + t;
         }
         if (rkn) {
-            m_commit<NP>(S, N, mpre);   // column s l, the end kick's
+            m_commit<NP>(S, el, mpre);   // column s l, the end kick's
             next_m();
             lds_barrier();
         } else {
