@@ -661,14 +661,24 @@ __device__ __forceinline__ void small_body(const SmallArgs &A, const RknCoef &co
                 }
             }
         }
-        // cond_est of Q (Gershgorin, matcomp.f90:498-528)
-        for (int j = tid; j < N; j += nt) {
+        // cond_est of Q (Gershgorin, matcomp.f90:498-528), needed only by the reference's rule;
+        // each column's sum in row order, its loads issued eight rows at a time
+        for (int j = tid; j < N && A.period <= 0; j += nt) {
             double t = 0.0;
-            for (int i = 0; i < N; ++i)
-                if (i != j) {
-                    const double re = S.qr[i + j * LD], im = S.qi[i + j * LD];
-                    t += sqrt(re * re + im * im);
+            for (int i0 = 0; i0 < N; i0 += 8) {
+                double re[8], im[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int i = min(i0 + u, N - 1);
+                    re[u] = S.qr[i + j * LD];
+                    im[u] = S.qi[i + j * LD];
                 }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int i = i0 + u;
+                    if (i < N && i != j) t += sqrt(re[u] * re[u] + im[u] * im[u]);
+                }
+            }
             const double dj = hypot(S.qr[j + j * LD], S.qi[j + j * LD]);
             S.red0[j] = dj - t;
             S.red1[j] = dj + t;
@@ -680,12 +690,15 @@ __device__ __forceinline__ void small_body(const SmallArgs &A, const RknCoef &co
         } else {
             __syncthreads();
         }
-        double mn = hypot(S.qr[0], S.qi[0]), mx = mn;
-        for (int j = 0; j < N; ++j) {
-            mn = fmin(mn, S.red0[j]);
-            mx = fmax(mx, S.red1[j]);
+        double d = 0.0;
+        if (A.period <= 0) {
+            double mn = hypot(S.qr[0], S.qi[0]), mx = mn;
+            for (int j = 0; j < N; ++j) {
+                mn = fmin(mn, S.red0[j]);
+                mx = fmax(mx, S.red1[j]);
+            }
+            d = (mn < 0.0) ? 1e100 : mx / mn;
         }
-        const double d = (mn < 0.0) ? 1e100 : mx / mn;
         const bool renorm = (A.period > 0 ? (l % A.period == 0) : (d > 1e3)) || (l == A.ns);
         if (rkn) {
             // the end kick, merged with the next slice's first kick unless the angle renormalises
