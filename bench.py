@@ -217,12 +217,48 @@ def run_serving(args, precision, pool, budget):
     eng.set_renorm_period(args.renorm_period)
     eng.serve_set_merge(args.serve_merge_angles, args.serve_merge_window_ms)
     try:
-        run = serve.serve(eng, reqs, arrivals=arr)
+        run = serve.serve(eng, reqs, arrivals=arr, timeout_s=args.serve_timeout_s,
+                          progress=lambda d, n, a, q: log("serving: %d of %d requests finished, %d admitted, "
+                                                          "%d queued" % (d, n, a, q)))
     finally:
         eng.close()
     out = serve.summary(reqs, run)
     out["f2"] = [r[1] for r in run["results"]]
     return out
+
+
+def serving_leg(eng64, args, paths):
+    """The serving stream for each precision path, with the mixed results' error vs FP64."""
+    import numpy as np
+    from trhepd import lib, serve
+    t0 = time.perf_counter()
+    pool = serve.request_pool(eng64, args.serve_pool, seed=args.seed)
+    log("serving pool: %d structures, N %s, angles %s (%.1f s)" % (len(pool), [r.nb for r in pool],
+                                                                   [r.nang for r in pool], time.perf_counter() - t0))
+    budget = int(args.serve_budget_gb * (1 << 30))
+    sv = {"workload": "first %d requests of a Poisson stream (mean %.0f req/s) over %d seeded structures with N "
+                      "from {23,47,97,199,307,521}, 3-12 atoms, 8-20 A slabs, 20-56 angles, sp4 dz=0.02; "
+                      "continuous batching under a %.0f GiB HBM budget, requests of one structure merged into "
+                      "batches of up to %d angles (%.0f ms window)"
+                      % (args.serve_requests, args.serve_rate, len(pool), args.serve_budget_gb,
+                         args.serve_merge_angles, args.serve_merge_window_ms),
+          "beams": [r.nb for r in pool], "angles": [r.nang for r in pool]}
+    f2s = {}
+    for p in paths:
+        r = run_serving(args, lib.PREC_FP64 if p == "fp64" else lib.PREC_MIXED, pool, budget)
+        f2s[p] = r.pop("f2")
+        sv[p] = r
+        log("serving %s: %.2f points/s p50 %.0f ms p99 %.0f ms (queued p50 %.0f ms, service p50 %.0f ms; "
+            "peak %d admitted)" % (p, r["points_per_s"], r["p50_ms"], r["p99_ms"], r["queued_ms"]["p50"],
+                                   r["service_ms"]["p50"], r["peak_admitted"]))
+    if "fp64" in f2s and "mixed" in f2s:
+        errs = []
+        for a, b in zip(f2s["mixed"], f2s["fp64"]):
+            mx = np.max(np.abs(b), axis=1, keepdims=True)
+            errs.append(float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-12 * mx))))
+        sv["mixed_max_rel_err_vs_fp64"] = max(errs)
+        sv["mixed_over_fp64"] = sv["mixed"]["points_per_s"] / sv["fp64"]["points_per_s"]
+    return sv
 
 
 def main():
@@ -265,6 +301,10 @@ def main():
                     help="requests of one structure merge into batches of at most this many angles (0: off)")
     ap.add_argument("--serve-merge-window-ms", type=float, default=200.0,
                     help="how long the head of the queue waits for requests of its structure to join it")
+    ap.add_argument("--serve-timeout-s", type=float, default=900.0, help="longest a serving run may take")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="time only the serving stream (--serve-requests requests, the --paths given), for the "
+                         "full-size stream whose device time exceeds a default run's")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the slice loop shards by structure and angle with no "
@@ -281,6 +321,12 @@ def main():
         small = run_small_configs(eng64, argparse.Namespace(**dict(vars(args), small_steps=max(1, args.small_steps))))
         eng64.close()
         print(json.dumps({"small_n": small}))
+        return
+    if args.serve_only:
+        paths = [p.strip() for p in args.paths.split(",") if p.strip()]
+        sv = serving_leg(eng64, args, paths)
+        eng64.close()
+        print(json.dumps({"serving": sv}))
         return
     st = synth.config(3, seed=args.seed)
     t0 = time.perf_counter()
@@ -419,34 +465,7 @@ def main():
             uv["mixed_over_fp64"] = uv["mixed"]["points_per_s"] / uv["fp64"]["points_per_s"]
         out["uneven"] = uv
     if args.serve_requests > 0:
-        from trhepd import serve
-        t0 = time.perf_counter()
-        pool = serve.request_pool(eng64, args.serve_pool, seed=args.seed)
-        log("serving pool: %d structures, N %s, angles %s (%.1f s)" % (len(pool), [r.nb for r in pool],
-                                                                       [r.nang for r in pool], time.perf_counter() - t0))
-        budget = int(args.serve_budget_gb * (1 << 30))
-        sv = {"workload": "first %d requests of a Poisson stream (mean %.0f req/s) over %d seeded structures with N "
-                          "from {23,47,97,199,307,521}, 3-12 atoms, 8-20 A slabs, 20-56 angles, sp4 dz=0.02; "
-                          "continuous batching under a %.0f GiB HBM budget, requests of one structure merged into "
-                          "batches of up to %d angles (%.0f ms window)"
-                          % (args.serve_requests, args.serve_rate, len(pool), args.serve_budget_gb,
-                             args.serve_merge_angles, args.serve_merge_window_ms),
-              "beams": [r.nb for r in pool], "angles": [r.nang for r in pool]}
-        f2s = {}
-        for p in paths:
-            r = run_serving(args, lib.PREC_FP64 if p == "fp64" else lib.PREC_MIXED, pool, budget)
-            f2s[p] = r.pop("f2")
-            sv[p] = r
-            log("serving %s: %.2f points/s p50 %.0f ms p99 %.0f ms (queued p50 %.0f ms, service p50 %.0f ms; "
-                "peak %d admitted)" % (p, r["points_per_s"], r["p50_ms"], r["p99_ms"], r["queued_ms"]["p50"],
-                                       r["service_ms"]["p50"], r["peak_admitted"]))
-        if "fp64" in f2s and "mixed" in f2s:
-            errs = []
-            for a, b in zip(f2s["mixed"], f2s["fp64"]):
-                mx = np.max(np.abs(b), axis=1, keepdims=True)
-                errs.append(float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-12 * mx))))
-            sv["mixed_max_rel_err_vs_fp64"] = max(errs)
-            sv["mixed_over_fp64"] = sv["mixed"]["points_per_s"] / sv["fp64"]["points_per_s"]
+        sv = serving_leg(eng64, args, paths)
         out["serving"] = sv
         out["p99_ms"] = sv.get("mixed", sv.get("fp64"))["p99_ms"]
     if not args.no_cpu:

@@ -62,20 +62,26 @@ def poisson_arrivals(n: int, rate: float, seed: int = 0) -> np.ndarray:
 
 
 def serve(eng: "lib.Engine", reqs: Sequence[Request], arrivals: Optional[Sequence[float]] = None,
-          want_f: bool = False, timeout_s: float = 900.0) -> Dict[str, object]:
+          want_f: bool = False, timeout_s: float = 900.0, progress=None,
+          progress_every_s: float = 30.0) -> Dict[str, object]:
     """Runs the request loop until every request has finished.
 
     Requests are submitted when the wall clock passes their arrival time (all at once when
     `arrivals` is None).  Returns per-request results (f2, latency_ms, optional f) in
-    submission order, the maximum number admitted at once, and the span of the run."""
+    submission order, the maximum number admitted at once, and the span of the run.
+    `progress(finished, total, admitted, queued)` is called every `progress_every_s` seconds."""
     arrivals = np.zeros(len(reqs)) if arrivals is None else np.asarray(arrivals, dtype=float)
     order = np.argsort(arrivals, kind="stable")
     ids: Dict[int, int] = {}
     out: List[Optional[tuple]] = [None] * len(reqs)
     t0 = time.perf_counter()
     nxt, left, peak = 0, len(reqs), 0
+    t_report, active, queued = progress_every_s, 0, 0
     while left:
         now = time.perf_counter() - t0
+        if progress is not None and now >= t_report:
+            progress(len(reqs) - left, len(reqs), active, queued)
+            t_report = now + progress_every_s
         if now > timeout_s:
             raise TimeoutError("serving loop exceeded %.0f s" % timeout_s)
         while nxt < len(order) and arrivals[order[nxt]] <= now:
