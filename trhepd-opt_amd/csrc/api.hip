@@ -29,6 +29,62 @@ int hip_check(trhepd_ctx *ctx, hipError_t e, const char *what)
                 hipGetErrorString(e));
 }
 
+// Released device blocks are kept for reuse instead of being returned with hipFree, which
+// waits for all work on the device: a context that runs several batches at once (serving)
+// releases blocks while other batches' kernels are queued.  dev_free records an event on
+// the context's stream and on every batch stream, the streams any kernel using the block
+// was launched on, and the block is handed out again only after all of them have completed.
+
+// pending blocks whose events have all completed become ready
+static void pool_poll(trhepd_ctx *ctx)
+{
+    for (size_t i = 0; i < ctx->pool_pending.size();) {
+        PoolBlock &b = ctx->pool_pending[i];
+        bool done = true;
+        for (hipEvent_t e : b.ev) {
+            const hipError_t q = hipEventQuery(e);
+            if (q == hipErrorNotReady) {
+                done = false;
+                break;
+            }
+        }
+        if (!done) {
+            ++i;
+            continue;
+        }
+        for (hipEvent_t e : b.ev) ctx->pool_events.push_back(e);
+        ctx->pool_ready.emplace(b.bytes, b.p);
+        ctx->pool_pending.erase(ctx->pool_pending.begin() + i);
+    }
+}
+
+// returns ready blocks to the device until the pool holds at most `keep` bytes
+static void pool_trim(trhepd_ctx *ctx, size_t keep)
+{
+    while (ctx->pool_bytes > keep && !ctx->pool_ready.empty()) {
+        auto it = std::prev(ctx->pool_ready.end());   // the largest first
+        (void)hipFree(it->second);
+        ctx->pool_bytes -= it->first;
+        ctx->pool_ready.erase(it);
+    }
+}
+
+// every pooled block back to the device (the device idle first)
+static void pool_release_all(trhepd_ctx *ctx)
+{
+    (void)hipDeviceSynchronize();
+    pool_poll(ctx);
+    pool_trim(ctx, 0);
+    for (PoolBlock &b : ctx->pool_pending) {   // events that failed to complete
+        (void)hipFree(b.p);
+        for (hipEvent_t e : b.ev) (void)hipEventDestroy(e);
+    }
+    ctx->pool_pending.clear();
+    for (hipEvent_t e : ctx->pool_events) (void)hipEventDestroy(e);
+    ctx->pool_events.clear();
+    ctx->pool_bytes = 0;
+}
+
 int dev_alloc(trhepd_ctx *ctx, void **p, size_t bytes)
 {
     *p = nullptr;
@@ -36,20 +92,83 @@ int dev_alloc(trhepd_ctx *ctx, void **p, size_t bytes)
     if (ctx->budget && ctx->held + bytes > ctx->budget)
         return fail(ctx, TRHEPD_E_BUDGET, "allocation of %zu bytes exceeds the HBM budget (%zu of %zu held)",
                     bytes, ctx->held, ctx->budget);
+    pool_poll(ctx);
+    // a released block of at least the size and at most half again (2 MiB more for small ones)
+    auto it = ctx->pool_ready.lower_bound(bytes);
+    if (it != ctx->pool_ready.end() && it->first - bytes <= std::max(bytes / 2, (size_t)2 << 20) &&
+        (!ctx->budget || ctx->held + it->first <= ctx->budget)) {
+        *p = it->second;
+        ctx->pool_bytes -= it->first;
+        ctx->held += it->first;
+        ctx->pool_size[*p] = it->first;
+        ctx->pool_ready.erase(it);
+        return TRHEPD_OK;
+    }
+    // the pool counts against the budget beside the live blocks
+    if (ctx->budget) {
+        const size_t room = ctx->budget - ctx->held - bytes;
+        if (ctx->pool_bytes > room) {
+            pool_trim(ctx, room);
+            if (ctx->pool_bytes > room) {   // pending blocks: wait for the device, then trim
+                (void)hipDeviceSynchronize();
+                pool_poll(ctx);
+                pool_trim(ctx, room);
+            }
+        }
+    }
     hipError_t e = hipMalloc(p, bytes);
+    if (e == hipErrorOutOfMemory && ctx->pool_bytes) {
+        (void)hipGetLastError();
+        (void)hipDeviceSynchronize();
+        pool_poll(ctx);
+        pool_trim(ctx, 0);
+        e = hipMalloc(p, bytes);
+    }
     if (e != hipSuccess) {
         *p = nullptr;
         return hip_check(ctx, e, "hipMalloc");
     }
     ctx->held += bytes;
+    ctx->pool_size[*p] = bytes;
     return TRHEPD_OK;
 }
 
 void dev_free(trhepd_ctx *ctx, void *p, size_t bytes)
 {
     if (!p) return;
-    (void)hipFree(p);
-    ctx->held -= bytes;
+    auto it = ctx->pool_size.find(p);
+    const size_t sz = it != ctx->pool_size.end() ? it->second : bytes;
+    if (it != ctx->pool_size.end()) ctx->pool_size.erase(it);
+    ctx->held -= sz;
+    // inside a stream capture an event record would become part of the graph
+    bool pooled = !ctx->capturing && !ctx->pool_off;
+    PoolBlock b;
+    b.p = p;
+    b.bytes = sz;
+    auto mark = [&](hipStream_t st) {
+        if (!pooled) return;
+        hipEvent_t e = nullptr;
+        if (!ctx->pool_events.empty()) {
+            e = ctx->pool_events.back();
+            ctx->pool_events.pop_back();
+        } else if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            pooled = false;
+            return;
+        }
+        b.ev.push_back(e);
+        if (hipEventRecord(e, st) != hipSuccess) pooled = false;
+    };
+    mark(ctx->stream);
+    for (const Batch &bt : ctx->batches) mark(bt.stream);
+    if (!pooled) {
+        (void)hipGetLastError();
+        for (hipEvent_t e : b.ev) ctx->pool_events.push_back(e);
+        (void)hipDeviceSynchronize();
+        (void)hipFree(p);
+        return;
+    }
+    ctx->pool_pending.push_back(std::move(b));
+    ctx->pool_bytes += sz;
 }
 
 int prof_mark(trhepd_ctx *ctx, hipStream_t stream)
@@ -294,6 +413,8 @@ int trhepd_ctx_destroy(trhepd_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     server_destroy(ctx);
+    (void)hipDeviceSynchronize();
+    ctx->pool_off = true;   // the streams go away below
     for (GraphRec &gr : ctx->graphs) {
         (void)hipGraphExecDestroy(gr.exec);
         (void)hipGraphDestroy(gr.graph);
@@ -309,6 +430,7 @@ int trhepd_ctx_destroy(trhepd_ctx *ctx)
         (void)hipStreamDestroy(b.stream);
     }
     for (hipEvent_t e : ctx->prof.ev) (void)hipEventDestroy(e);
+    pool_release_all(ctx);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
     (void)hipStreamDestroy(ctx->stream);
@@ -575,6 +697,7 @@ int trhepd_batch_create(trhepd_ctx *ctx, int table_id, int nb, int nang, const i
     int rc2 = e ? hip_check(ctx, e, "batch upload") : TRHEPD_OK;
     if (!rc2 && (nb > 48 || magnus)) rc2 = runner_create(ctx, b, *t);
     if (rc2) {
+        if (b.stream) (void)hipStreamSynchronize(b.stream);   // b is not among the batches dev_free marks
         runner_destroy(ctx, b);
         dev_free(ctx, b.d, b.bytes);
         if (b.ev0) (void)hipEventDestroy(b.ev0);

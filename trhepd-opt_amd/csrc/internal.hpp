@@ -103,6 +103,14 @@ struct Prof {
     double ms = 0.0;
 };
 
+// A device block the library has released: it may be handed out again once every stream
+// that could still use it has passed the point of release (one event per stream).
+struct PoolBlock {
+    void *p = nullptr;
+    size_t bytes = 0;
+    std::vector<hipEvent_t> ev;
+};
+
 }  // namespace trhepd
 
 struct trhepd_ctx {
@@ -135,6 +143,14 @@ struct trhepd_ctx {
     // batches run together start a fraction of the renormalisation period apart: 1 on, 0 off,
     // -1 on for the mixed precision only
     int phase_offsets = -1;
+    // released device blocks kept for reuse (hipFree waits for the whole device): pending
+    // until their events complete, then ready; pool_bytes counts both against the budget
+    std::vector<trhepd::PoolBlock> pool_pending;
+    std::multimap<size_t, void *> pool_ready;
+    std::map<void *, size_t> pool_size;   // size of every block dev_alloc has handed out
+    std::vector<hipEvent_t> pool_events;  // spare events
+    size_t pool_bytes = 0;
+    bool pool_off = false;                // the context is being destroyed: hipFree at once
 };
 
 namespace trhepd {
