@@ -169,6 +169,13 @@ void dev_free(trhepd_ctx *ctx, void *p, size_t bytes)
     }
     ctx->pool_pending.push_back(std::move(b));
     ctx->pool_bytes += sz;
+    // without a budget the pool is held to a fixed size, so that it does not keep memory
+    // other users of the device need
+    constexpr size_t kPoolCap = (size_t)8 << 30;
+    if (!ctx->budget && ctx->pool_bytes > kPoolCap) {
+        pool_poll(ctx);
+        pool_trim(ctx, kPoolCap);
+    }
 }
 
 int prof_mark(trhepd_ctx *ctx, hipStream_t stream)
