@@ -55,7 +55,8 @@ typedef struct trhepd_ctx trhepd_ctx;
 typedef struct {
     int device;              /* HIP device ordinal */
     int precision;           /* TRHEPD_PREC_* */
-    size_t hbm_budget;       /* bytes this context may hold on the device (0: no limit) */
+    size_t hbm_budget;       /* bytes this context may hold on the device, live and kept for
+                                reuse (0: no limit) */
 } trhepd_opts;
 
 /* Potential-table inputs: the module mod_scpot3 globals (surfio.f90:18-27) that
@@ -79,7 +80,9 @@ typedef struct {
 int trhepd_ctx_create(const trhepd_opts *opts, trhepd_ctx **out);
 int trhepd_ctx_destroy(trhepd_ctx *ctx);
 const char *trhepd_last_error(const trhepd_ctx *ctx);
-/* bytes currently held on the device by the context */
+/* bytes currently held on the device by the context's live tables, batches and work areas.
+ * Released blocks are kept for reuse (the budget counts them too; without a budget at most
+ * 8 GiB) and returned to the device by trhepd_ctx_destroy. */
 size_t trhepd_ctx_bytes(const trhepd_ctx *ctx);
 
 /* ---- per-structure setup (replaces scpot.f90:11-110, surfsub.f90:9-41) -------- */
