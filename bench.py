@@ -215,6 +215,8 @@ def run_serving(args, precision, pool, budget):
     arr = serve.poisson_arrivals(len(reqs), args.serve_rate, seed=args.seed)
     eng = lib.Engine(0, precision=precision, hbm_budget=budget)
     eng.set_renorm_period(args.renorm_period)
+    if precision == lib.PREC_MIXED:
+        eng.set_mixed_digits(args.mixed_digits)   # as the curve and uneven legs
     eng.serve_set_merge(args.serve_merge_angles, args.serve_merge_window_ms)
     try:
         run = serve.serve(eng, reqs, arrivals=arr, timeout_s=args.serve_timeout_s,
@@ -239,9 +241,9 @@ def serving_leg(eng64, args, paths):
     sv = {"workload": "first %d requests of a Poisson stream (mean %.0f req/s) over %d seeded structures with N "
                       "from {23,47,97,199,307,521}, 3-12 atoms, 8-20 A slabs, 20-56 angles, sp4 dz=0.02; "
                       "continuous batching under a %.0f GiB HBM budget, requests of one structure merged into "
-                      "batches of up to %d angles (%.0f ms window)"
+                      "batches of up to %d angles (%.0f ms window); mixed path at %d digits"
                       % (args.serve_requests, args.serve_rate, len(pool), args.serve_budget_gb,
-                         args.serve_merge_angles, args.serve_merge_window_ms),
+                         args.serve_merge_angles, args.serve_merge_window_ms, args.mixed_digits),
           "beams": [r.nb for r in pool], "angles": [r.nang for r in pool]}
     f2s = {}
     for p in paths:
